@@ -1,0 +1,151 @@
+/* mt_amr.h -- C-ABI of the MI355X AMR lattice-Boltzmann update.
+ *
+ * One opaque case handle holds the device-resident state of one AMR flow case:
+ * per level, the Morton-sorted cell tables (bricks of 4x4x4 cells, neighbour
+ * and parent/child index tables) and the SoA populations f / fcol with the
+ * macroscopic fields rho / u.  The entry points are the seams through which the
+ * reference's LBM_Manager drives its update; each cites the reference
+ * interface it replaces.  Plain pointers and sizes only; every call returns an
+ * int status (MT_OK = 0) and mt_last_error() gives the text of the last failure.
+ * Host arrays are read/written synchronously; device work runs on the case's
+ * own HIP stream.
+ */
+#ifndef MT_AMR_H
+#define MT_AMR_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define MT_OK 0
+#define MT_ERR_ARG (-1)    /* bad argument or unsupported configuration        */
+#define MT_ERR_HIP (-2)    /* a HIP runtime call failed                         */
+#define MT_ERR_STATE (-3)  /* call out of order (e.g. before mt_case_finalize)  */
+#define MT_ERR_LOOKUP (-4) /* a required key is absent: the reference's
+                              std::out_of_range from D_map_define::at()         */
+
+/* ---- cell kinds (uint16 bit flags) --------------------------------------
+ * The level-set contract of header/Lat_Manager.hpp:245-316.                  */
+#define MT_KIND_FLUID 0x0001u     /* lat_f                                     */
+#define MT_KIND_F2C 0x0002u       /* lat_overlap_F2C                           */
+#define MT_KIND_C2F 0x0004u       /* lat_overlap_C2F                           */
+#define MT_KIND_GHOST_C2F 0x0008u /* ghost_overlap_C2F                         */
+#define MT_KIND_GHOST_F2C 0x0010u /* ghost_overlap_F2C (flag on lat_f cells)   */
+#define MT_KIND_SOLID 0x0020u     /* lat_sf                                    */
+#define MT_KIND_BCRES 0x0040u     /* lat_bc (frozen level-0 reservoir)         */
+#define MT_KIND_DIS 0x0080u       /* has an entry in dis                       */
+#define MT_FACE_W 0x0100u         /* level-0 cell lies on face x-min ...       */
+#define MT_FACE_E 0x0200u
+#define MT_FACE_S 0x0400u
+#define MT_FACE_N 0x0800u
+#define MT_FACE_B 0x1000u
+#define MT_FACE_T 0x2000u
+
+/* ---- face boundary kinds (faces ordered W,E,S,N,Bot,Top) ---------------- */
+#define MT_BC_NONE 0      /* missing neighbours keep their f (Stream_Manager.cpp:60-62) */
+#define MT_BC_RESERVOIR 1 /* lat_bc reservoir + non-equilibrium extrapolation on its f  */
+#define MT_BC_ZOUHE 2     /* wet-node velocity (Zou-He / Hecht-Harting closure)         */
+#define MT_BC_OUTFLOW 3   /* zero-gradient copy of the unknown populations              */
+#define MT_BC_PERIODIC 4  /* wrapped neighbour tables                                   */
+
+#define MT_COLLIDE_LBGK 0
+#define MT_COLLIDE_CUMULANT 1
+#define MT_IB_NONE 0
+#define MT_IB_FH 1 /* IB_Method::FH                */
+#define MT_IB_DW 2 /* IB_Method::DISTANCE_WEIGHTED */
+
+#define MT_FP64 0          /* populations stored and computed in FP64               */
+#define MT_FP32_SHIFTED 1  /* stored as FP32 f_q - w_q, FP32 arithmetic             */
+#define MT_FP16_STORAGE 2  /* stored as FP16 f_q - w_q, FP32 arithmetic             */
+
+/* field selectors for mt_get_field / mt_set_field */
+#define MT_FIELD_F 0    /* [Q][n]  post-stream populations (user[L].df.f)       */
+#define MT_FIELD_FCOL 1 /* [Q][n]  post-collision populations (user[L].df.fcol) */
+#define MT_FIELD_RHO 2  /* [n]     density map                                   */
+#define MT_FIELD_U 3    /* [3][n]  velocity map                                  */
+
+typedef struct mt_case_desc {
+  int32_t Q;          /* 19 or 27                                                 */
+  int32_t Lmax;       /* finest level, 0..6                                       */
+  int32_t collision;  /* MT_COLLIDE_*                                             */
+  int32_t ib;         /* MT_IB_*                                                  */
+  int32_t precision;  /* MT_FP64 / MT_FP32_SHIFTED / MT_FP16_STORAGE              */
+  int32_t dims0[3];   /* level-0 extent in cells                                  */
+  int32_t face_kind[6];
+  double face_u[6][3];/* wall / reservoir velocity per face (lattice units)       */
+  double tau0;        /* level-0 relaxation time; tau_L = 2^L (tau0-1/2)+1/2     */
+  double les_C;       /* Smagorinsky constant, 0 = off                            */
+  double rho0;        /* initial density and He-Luo reference density             */
+  double u0[3];       /* initial velocity                                         */
+} mt_case_desc;
+
+typedef struct mt_case mt_case;
+
+const char* mt_last_error(void);
+
+/* ---- lifecycle: replaces LBM_Manager::LBM_Manager + allocMacrosAndDDF
+ *      (LBM_Manager.cpp:32-95, 482-592) -------------------------------------- */
+int mt_case_create(const mt_case_desc* desc, mt_case** out);
+int mt_case_destroy(mt_case* c);
+/* one level's cell set (any order); kinds use MT_KIND_* (Lat_Manager.hpp:245-316) */
+int mt_case_upload_level(mt_case* c, int level, const uint64_t* keys, const uint16_t* kinds, int64_t n);
+/* Lat_Manager::dis: n finest-level keys, n x (Q-1) link fractions (-1 = no wall) */
+int mt_case_upload_dis(mt_case* c, const uint64_t* keys, const double* fractions, int64_t n);
+/* sorts the keys on the device, builds bricks, neighbour / parent-child tables,
+ * sets f = fcol = feq(rho0,u0) (solid 0) -- allocMacrosAndDDF + initialBC       */
+int mt_case_finalize(mt_case* c);
+int mt_case_level_info(mt_case* c, int level, int64_t* n_cells, int64_t* n_bricks);
+
+/* ---- key lookup: replaces D_map_define<T>::find (mortonTrieMap.cpp:116-126,
+ *      header/Grid_Class.h:116-137): slot index or -1 per key                   */
+int mt_find(mt_case* c, int level, const uint64_t* keys, int64_t n, int32_t* slot_out);
+
+/* ---- state access in the caller's key order (user[L].df / density / velocity,
+ *      header/user.h:18-55,154-167); MT_ERR_LOOKUP if a key is absent          */
+int mt_get_field(mt_case* c, int level, int field, const uint64_t* keys, int64_t n, double* out);
+int mt_set_field(mt_case* c, int level, int field, const uint64_t* keys, int64_t n, const double* in);
+
+/* ---- the seams of LBM_Manager::LBMkernel (LBM_Simulator.cpp:20-229) ---- */
+/* LBM_Manager::calculateMacros(level)                 LBM_Manager.cpp:1003-1050 */
+int mt_macros(mt_case* c, int level);
+/* Collision_Model::collide(level)   lbm_kernel/Collision_Manager.cpp:13-53,
+ *                                    lbm_kernel/Collision_Cumulant.cpp:22-159   */
+int mt_collide(mt_case* c, int level);
+/* Stream_Model::stream(level)       lbm_kernel/Stream_Manager.cpp:43-76         */
+int mt_stream(mt_case* c, int level);
+/* LBM_Manager::AMR_transDDF_explosion(level)          LBM_Manager.cpp:1059-1129 */
+int mt_explode(mt_case* c, int level);
+/* LBM_Manager::AMR_transDDF_coalescence(level)        LBM_Manager.cpp:1131-1169 */
+int mt_coalesce(mt_case* c, int level);
+/* D3Q19_BoundaryCondtion_Manager::applyBoundaryCondition(face, "AfterStream"):
+ * all six faces W,E,S,N,Bot,Top in turn (header/BC/D3Q19_BC_Manager.hpp:220-244,
+ * LBM_Simulator.cpp:89-96)                                                      */
+int mt_apply_bc(mt_case* c);
+/* ComplexBoundary_Model::treat() + accumIBForce      ComplexBoundary_Manager.cpp:43-256,
+ *                                                     LBM_Manager.cpp:846-852;
+ * force3 (may be NULL) receives this call's momentum-exchange force            */
+int mt_ib_treat(mt_case* c, double* force3);
+
+/* ---- whole schedule: n coarse steps of LBM_Manager::fluidSimulate's loop body
+ *      (LBMkernel(0); calculateMacros()), LBM_Simulator.cpp:11-17.  Runs the
+ *      fused per-level kernels, captured as HIP graphs when graphs are on.    */
+int mt_step(mt_case* c, int n_coarse);
+int mt_set_graphs(mt_case* c, int on);
+/* per-kernel event timing of mt_step (off by default; forces graphs off):
+ * ms[level] accumulates the fused update kernel's time, calls[level] its count */
+int mt_set_timing(mt_case* c, int on);
+int mt_get_timing(mt_case* c, double* ms, int64_t* calls, int max_levels);
+int mt_sync(mt_case* c);
+
+/* ---- reductions (FP64, fixed order) ------------------------------------- */
+/* mass and momentum over lat_f and F2C of one level, and a NaN count       */
+int mt_reduce(mt_case* c, int level, double* mass, double* momentum3, int64_t* nan_count);
+/* accumulated IB force since the last reset (LBM_Manager.cpp:846-852, 964-979) */
+int mt_ib_force(mt_case* c, double* sum3, int64_t* count, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MT_AMR_H */

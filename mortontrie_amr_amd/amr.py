@@ -1,0 +1,246 @@
+"""Python front end of libmt_amr.so (the C-ABI in include/mt_amr.h).
+
+`AmrCase` mirrors the state and the methods LBM_Manager drives (LBM_Manager.hpp:58-88):
+calculateMacros / collide / stream / AMR_transDDF_explosion / coalescence /
+applyBoundaryCondition / treat, plus `step(n)`, the fluidSimulate loop body.
+The library must be built (``python -m mortontrie_amr_amd.build``); there is no
+fallback path: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import cases as _cases
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmt_amr.so")
+
+FP64, FP32_SHIFTED, FP16_STORAGE = 0, 1, 2
+FIELD_F, FIELD_FCOL, FIELD_RHO, FIELD_U = 0, 1, 2, 3
+
+_dp = C.POINTER(C.c_double)
+_u64p = C.POINTER(C.c_uint64)
+_u16p = C.POINTER(C.c_uint16)
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+
+
+class CaseDesc(C.Structure):
+    _fields_ = [
+        ("Q", C.c_int32), ("Lmax", C.c_int32), ("collision", C.c_int32), ("ib", C.c_int32),
+        ("precision", C.c_int32), ("dims0", C.c_int32 * 3), ("face_kind", C.c_int32 * 6),
+        ("face_u", (C.c_double * 3) * 6), ("tau0", C.c_double), ("les_C", C.c_double),
+        ("rho0", C.c_double), ("u0", C.c_double * 3),
+    ]
+
+
+EXPORTS = [
+    "mt_last_error", "mt_case_create", "mt_case_destroy", "mt_case_upload_level", "mt_case_upload_dis",
+    "mt_case_finalize", "mt_case_level_info", "mt_find", "mt_get_field", "mt_set_field", "mt_macros",
+    "mt_collide", "mt_stream", "mt_explode", "mt_coalesce", "mt_apply_bc", "mt_ib_treat", "mt_step",
+    "mt_set_graphs", "mt_set_timing", "mt_get_timing", "mt_sync", "mt_reduce", "mt_ib_force",
+]
+
+_lib = None
+
+
+class MtError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MtError(f"{LIB_PATH} is missing: build it with `python -m mortontrie_amr_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    L.mt_last_error.restype = C.c_char_p
+    L.mt_case_create.argtypes = [C.POINTER(CaseDesc), C.POINTER(C.c_void_p)]
+    L.mt_case_destroy.argtypes = [C.c_void_p]
+    L.mt_case_upload_level.argtypes = [C.c_void_p, C.c_int, _u64p, _u16p, C.c_int64]
+    L.mt_case_upload_dis.argtypes = [C.c_void_p, _u64p, _dp, C.c_int64]
+    L.mt_case_finalize.argtypes = [C.c_void_p]
+    L.mt_case_level_info.argtypes = [C.c_void_p, C.c_int, _i64p, _i64p]
+    L.mt_find.argtypes = [C.c_void_p, C.c_int, _u64p, C.c_int64, _i32p]
+    L.mt_get_field.argtypes = [C.c_void_p, C.c_int, C.c_int, _u64p, C.c_int64, _dp]
+    L.mt_set_field.argtypes = [C.c_void_p, C.c_int, C.c_int, _u64p, C.c_int64, _dp]
+    for n in ("mt_macros", "mt_collide", "mt_stream", "mt_explode", "mt_coalesce"):
+        getattr(L, n).argtypes = [C.c_void_p, C.c_int]
+    L.mt_apply_bc.argtypes = [C.c_void_p]
+    L.mt_ib_treat.argtypes = [C.c_void_p, _dp]
+    L.mt_step.argtypes = [C.c_void_p, C.c_int]
+    L.mt_set_graphs.argtypes = [C.c_void_p, C.c_int]
+    L.mt_set_timing.argtypes = [C.c_void_p, C.c_int]
+    L.mt_get_timing.argtypes = [C.c_void_p, _dp, _i64p, C.c_int]
+    L.mt_sync.argtypes = [C.c_void_p]
+    L.mt_reduce.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _i64p]
+    L.mt_ib_force.argtypes = [C.c_void_p, _dp, _i64p, C.c_int]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise MtError(f"mt error {rc}: {load().mt_last_error().decode()}")
+
+
+def _u64(a):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    return a, a.ctypes.data_as(_u64p)
+
+
+class AmrCase:
+    """A device-resident AMR case built from a `cases` dictionary."""
+
+    def __init__(self, case, precision=FP64, graphs=True):
+        L = load()
+        self.case = case
+        self.Q = case["Q"]
+        self.Lmax = case["Lmax"]
+        d = CaseDesc()
+        d.Q, d.Lmax, d.collision, d.ib, d.precision = self.Q, self.Lmax, case["collision"], case["ib"], precision
+        for a in range(3):
+            d.dims0[a] = case["dims0"][a]
+            d.u0[a] = case["u0"][a]
+        for f in range(6):
+            d.face_kind[f] = case["faces"][f][0]
+            for a in range(3):
+                d.face_u[f][a] = case["faces"][f][1 + a]
+        d.tau0, d.les_C, d.rho0 = case["tau0"], case["les_C"], case["rho0"]
+        h = C.c_void_p()
+        _check(L.mt_case_create(C.byref(d), C.byref(h)))
+        self.h = h
+        for lvl, lv in enumerate(case["levels"]):
+            keys, pk = _u64(lv["keys"])
+            kinds = np.ascontiguousarray(lv["kinds"], dtype=np.uint16)
+            _check(L.mt_case_upload_level(self.h, lvl, pk, kinds.ctypes.data_as(_u16p), keys.size))
+        if len(case["dis_keys"]):
+            dk, pdk = _u64(case["dis_keys"])
+            dv = np.ascontiguousarray(case["dis_vals"], dtype=np.float64)
+            _check(L.mt_case_upload_dis(self.h, pdk, dv.ctypes.data_as(_dp), dk.size))
+        _check(L.mt_case_finalize(self.h))
+        _check(L.mt_set_graphs(self.h, 1 if graphs else 0))
+
+    def close(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.mt_case_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # ---- info / lookup
+    def level_info(self, level):
+        n = C.c_int64()
+        b = C.c_int64()
+        _check(load().mt_case_level_info(self.h, level, C.byref(n), C.byref(b)))
+        return n.value, b.value
+
+    def find(self, level, keys):
+        keys, pk = _u64(keys)
+        out = np.empty(keys.size, dtype=np.int32)
+        _check(load().mt_find(self.h, level, pk, keys.size, out.ctypes.data_as(_i32p)))
+        return out
+
+    # ---- fields
+    def get(self, level, field, keys=None):
+        if keys is None:
+            keys = self.case["levels"][level]["keys"]
+        keys, pk = _u64(keys)
+        rows = {FIELD_F: self.Q, FIELD_FCOL: self.Q, FIELD_RHO: 1, FIELD_U: 3}[field]
+        out = np.empty(rows * keys.size)
+        _check(load().mt_get_field(self.h, level, field, pk, keys.size, out.ctypes.data_as(_dp)))
+        return out.reshape(rows, keys.size) if rows > 1 else out
+
+    def set(self, level, field, values, keys=None):
+        if keys is None:
+            keys = self.case["levels"][level]["keys"]
+        keys, pk = _u64(keys)
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        _check(load().mt_set_field(self.h, level, field, pk, keys.size, v.ctypes.data_as(_dp)))
+
+    def f(self, level, keys=None):
+        return self.get(level, FIELD_F, keys)
+
+    def fcol(self, level, keys=None):
+        return self.get(level, FIELD_FCOL, keys)
+
+    def rho(self, level, keys=None):
+        return self.get(level, FIELD_RHO, keys)
+
+    def u(self, level, keys=None):
+        return self.get(level, FIELD_U, keys)
+
+    # ---- seams (LBM_Manager / strategy method names)
+    def macros(self, level):
+        _check(load().mt_macros(self.h, level))
+
+    def collide(self, level):
+        _check(load().mt_collide(self.h, level))
+
+    def stream(self, level):
+        _check(load().mt_stream(self.h, level))
+
+    def explode(self, level):
+        _check(load().mt_explode(self.h, level))
+
+    def coalesce(self, level):
+        _check(load().mt_coalesce(self.h, level))
+
+    def bc_after_stream(self):
+        _check(load().mt_apply_bc(self.h))
+
+    def ib_treat(self):
+        F = np.zeros(3)
+        _check(load().mt_ib_treat(self.h, F.ctypes.data_as(_dp)))
+        return F
+
+    # ---- schedule
+    def step(self, n=1):
+        _check(load().mt_step(self.h, n))
+
+    def sync(self):
+        _check(load().mt_sync(self.h))
+
+    def set_graphs(self, on):
+        _check(load().mt_set_graphs(self.h, 1 if on else 0))
+
+    def set_timing(self, on):
+        _check(load().mt_set_timing(self.h, 1 if on else 0))
+
+    def timing(self):
+        ms = np.zeros(8)
+        calls = np.zeros(8, dtype=np.int64)
+        _check(load().mt_get_timing(self.h, ms.ctypes.data_as(_dp), calls.ctypes.data_as(_i64p), 8))
+        return ms[: self.Lmax + 1], calls[: self.Lmax + 1]
+
+    def reduce(self, level):
+        m = C.c_double()
+        p = np.zeros(3)
+        n = C.c_int64()
+        _check(load().mt_reduce(self.h, level, C.byref(m), p.ctypes.data_as(_dp), C.byref(n)))
+        return m.value, p, n.value
+
+    def ib_force(self, reset=False):
+        s = np.zeros(3)
+        n = C.c_int64()
+        _check(load().mt_ib_force(self.h, s.ctypes.data_as(_dp), C.byref(n), 1 if reset else 0))
+        return s, n.value
+
+    def drag_coefficient(self, reset=True):
+        """Cd = 2 Fx / (rho U^2 pi d^2 / 4), Fx the mean IB force per treat() call
+        (LBM_Manager.cpp:964-979); U in level-0 units, d in finest-level cells."""
+        s, n = self.ib_force(reset=reset)
+        if n == 0:
+            return float("nan")
+        U, d = self.case["cd"]["U"], self.case["cd"]["d"]
+        return 2.0 * (s[0] / n) / (U * U * np.pi * d * d / 4.0)
+
+
+def taus(case):
+    return _cases.taus(case)

@@ -1,0 +1,438 @@
+"""Case generators: analytic geometry -> per-level Morton-keyed cell sets.
+
+This is the input side of the hot path, the role the reference's
+``Lat_Manager::voxelize`` plays (Lat_Voxelize.cpp:17-33) for the analytic
+geometries the workloads name (lid-driven cavity, refined sphere in a channel,
+refined boxes).  It produces, per refinement level, the same *set contract* the
+reference's level sets carry (header/Lat_Manager.hpp:245-316):
+
+* ``lat_f``            -> KIND_FLUID
+* ``lat_overlap_F2C``  -> KIND_F2C   (coarse cells covered by the fine band)
+* ``lat_overlap_C2F``  -> KIND_C2F   (fine children of F2C cells)
+* ``ghost_overlap_C2F``-> KIND_GHOST_C2F
+* ``ghost_overlap_F2C``-> KIND_GHOST_F2C (a flag on lat_f cells)
+* ``lat_sf``           -> KIND_SOLID (finest level, populations zero)
+* ``lat_bc[26]``       -> KIND_BCRES (level-0 frozen equilibrium reservoir)
+* ``dis``              -> KIND_DIS + per-cell link fractions
+
+Keys follow the reference layout (Morton_Assist.cpp:185-193, Lat_Voxelize.cpp:245):
+x bit i -> key bit 3i, y -> 3i+1, z -> 3i+2, of the cell's lower corner in
+finest-level units, so a level-L cell key is encode(i,j,k) << 3*(Lmax-L).
+
+Everything here is plain numpy on dense per-level boolean grids; the sets it
+returns are uploaded unsorted and the device builds its tables from them.
+"""
+from __future__ import annotations
+
+import math
+import numpy as np
+
+# ---- cell kinds (uint16 bit flags), shared with include/mt_amr.h ----------
+KIND_FLUID = 0x0001
+KIND_F2C = 0x0002
+KIND_C2F = 0x0004
+KIND_GHOST_C2F = 0x0008
+KIND_GHOST_F2C = 0x0010
+KIND_SOLID = 0x0020
+KIND_BCRES = 0x0040
+KIND_DIS = 0x0080
+FACE_W, FACE_E, FACE_S, FACE_N, FACE_B, FACE_T = (0x0100 << i for i in range(6))
+FACE_BITS = (FACE_W, FACE_E, FACE_S, FACE_N, FACE_B, FACE_T)
+
+# ---- face boundary kinds ---------------------------------------------------
+BC_NONE = 0        # missing neighbours keep their previous f (reference stream rule)
+BC_RESERVOIR = 1   # frozen feq reservoir cells (compiled reference layout) + NEE on their f
+BC_ZOUHE = 2       # wet-node velocity boundary (Hecht-Harting closure, generalised)
+BC_OUTFLOW = 3     # zero-gradient copy from the inward neighbour
+BC_PERIODIC = 4    # neighbour tables wrap
+
+COLLIDE_LBGK = 0
+COLLIDE_CUMULANT = 1
+
+IB_NONE = 0
+IB_FH = 1
+IB_DW = 2
+
+# D3Q19 / D3Q27 velocity sets in the reference order (header/Constants.h:492-538)
+EX27 = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0,
+                 1, -1, 1, -1, -1, 1, -1, 1], dtype=np.int64)
+EY27 = np.array([0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1,
+                 1, -1, -1, 1, -1, 1, 1, -1], dtype=np.int64)
+EZ27 = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1,
+                 1, -1, 1, -1, 1, -1, 1, -1], dtype=np.int64)
+
+
+def velocities(Q):
+    if Q not in (19, 27):
+        raise ValueError("Q must be 19 or 27")
+    return EX27[:Q], EY27[:Q], EZ27[:Q]
+
+
+def part1by2(v):
+    """Spread the low 21 bits of v so bit i lands on bit 3i."""
+    v = np.asarray(v).astype(np.uint64) & np.uint64(0x1FFFFF)
+    v = (v | (v << np.uint64(32))) & np.uint64(0x1F00000000FFFF)
+    v = (v | (v << np.uint64(16))) & np.uint64(0x1F0000FF0000FF)
+    v = (v | (v << np.uint64(8))) & np.uint64(0x100F00F00F00F00F)
+    v = (v | (v << np.uint64(4))) & np.uint64(0x10C30C30C30C30C3)
+    v = (v | (v << np.uint64(2))) & np.uint64(0x1249249249249249)
+    return v
+
+
+def compact1by2(v):
+    v = np.asarray(v).astype(np.uint64) & np.uint64(0x1249249249249249)
+    v = (v ^ (v >> np.uint64(2))) & np.uint64(0x10C30C30C30C30C3)
+    v = (v ^ (v >> np.uint64(4))) & np.uint64(0x100F00F00F00F00F)
+    v = (v ^ (v >> np.uint64(8))) & np.uint64(0x1F0000FF0000FF)
+    v = (v ^ (v >> np.uint64(16))) & np.uint64(0x1F00000000FFFF)
+    v = (v ^ (v >> np.uint64(32))) & np.uint64(0x1FFFFF)
+    return v
+
+
+def encode_key(x, y, z, level, Lmax):
+    """Reference cell key of the level-`level` cell (x,y,z) (Lat_Voxelize.cpp:245)."""
+    s = np.uint64(Lmax - level)
+    x = np.asarray(x).astype(np.uint64) << s
+    y = np.asarray(y).astype(np.uint64) << s
+    z = np.asarray(z).astype(np.uint64) << s
+    return part1by2(x) | (part1by2(y) << np.uint64(1)) | (part1by2(z) << np.uint64(2))
+
+
+def decode_key(key, level, Lmax):
+    key = np.asarray(key).astype(np.uint64)
+    s = np.uint64(Lmax - level)
+    x = compact1by2(key) >> s
+    y = compact1by2(key >> np.uint64(1)) >> s
+    z = compact1by2(key >> np.uint64(2)) >> s
+    return x.astype(np.int64), y.astype(np.int64), z.astype(np.int64)
+
+
+# ---- dense-grid helpers ----------------------------------------------------
+def _shift(mask, dx, dy, dz, periodic=(False, False, False)):
+    """out[x,y,z] = mask[x+dx, y+dy, z+dz]; False outside unless periodic."""
+    out = mask
+    for axis, d in enumerate((dx, dy, dz)):
+        if d == 0:
+            continue
+        if periodic[axis]:
+            out = np.roll(out, -d, axis=axis)
+        else:
+            rolled = np.zeros_like(out)
+            n = out.shape[axis]
+            src = [slice(None)] * 3
+            dst = [slice(None)] * 3
+            if d > 0:
+                src[axis] = slice(d, n)
+                dst[axis] = slice(0, n - d)
+            else:
+                src[axis] = slice(0, n + d)
+                dst[axis] = slice(-d, n)
+            rolled[tuple(dst)] = out[tuple(src)]
+            out = rolled
+    return out
+
+
+def _dilate(mask, offsets, periodic=(False, False, False)):
+    out = mask.copy()
+    for (dx, dy, dz) in offsets:
+        if dx == dy == dz == 0:
+            continue
+        out |= _shift(mask, -dx, -dy, -dz, periodic)
+    return out
+
+
+_OFF26 = [(dx, dy, dz) for dx in (-1, 0, 1) for dy in (-1, 0, 1) for dz in (-1, 0, 1)]
+
+
+def _qoffsets(Q):
+    ex, ey, ez = velocities(Q)
+    return [(int(ex[q]), int(ey[q]), int(ez[q])) for q in range(1, Q)]
+
+
+def _upsample(mask):
+    return mask.repeat(2, axis=0).repeat(2, axis=1).repeat(2, axis=2)
+
+
+def _centers(dims, level):
+    h = 1.0 / (1 << level)
+    xs = (np.arange(dims[0]) + 0.5) * h
+    ys = (np.arange(dims[1]) + 0.5) * h
+    zs = (np.arange(dims[2]) + 0.5) * h
+    return xs[:, None, None], ys[None, :, None], zs[None, None, :]
+
+
+def _emit(kind_grid, level, Lmax):
+    x, y, z = np.nonzero(kind_grid)
+    keys = encode_key(x, y, z, level, Lmax)
+    kinds = kind_grid[x, y, z].astype(np.uint16)
+    order = np.argsort(keys, kind="stable")
+    return {"keys": keys[order], "kinds": kinds[order]}
+
+
+def _face_bits(kind_grid, dims, faces, cellmask):
+    """Mark level-0 cells lying on each non-periodic domain face."""
+    for f in range(6):
+        if faces[f][0] == BC_PERIODIC:
+            continue
+        axis, side = f // 2, f % 2
+        sl = [slice(None)] * 3
+        sl[axis] = 0 if side == 0 else dims[axis] - 1
+        sub = kind_grid[tuple(sl)]
+        m = cellmask[tuple(sl)]
+        sub[m] |= FACE_BITS[f]
+        kind_grid[tuple(sl)] = sub
+
+
+def build_nested(dims0, Lmax, refine, solid=None, faces=None, Q=19, ib_dist=None,
+                 reservoir=False):
+    """Generic nested-refinement set builder.
+
+    refine(L, cx, cy, cz) -> bool grid: level-L cells to be covered by level L+1.
+    solid(cx, cy, cz) -> bool grid evaluated on the finest level.
+    ib_dist(cx, cy, cz, ex, ey, ez, h) -> per-cell link fraction or NaN (finest).
+    reservoir=True puts a frozen lat_bc shell around the level-0 box (the compiled
+    reference layout, Lat_Voxelize.cpp:242-288) instead of wet-node face cells.
+    """
+    faces = faces or [(BC_NONE, 0.0, 0.0, 0.0)] * 6
+    periodic = tuple(faces[2 * a][0] == BC_PERIODIC for a in range(3))
+    qoff = _qoffsets(Q)
+    levels = []
+    D = np.ones(dims0, dtype=bool)
+    C2F = np.zeros(dims0, dtype=bool)
+    prev = None
+    grids = []
+    for L in range(Lmax + 1):
+        dims = tuple(int(d) << L for d in dims0)
+        cx, cy, cz = _centers(dims, L)
+        if L > 0:
+            D = _upsample(prev["R"])
+            C2F = _upsample(prev["F2C"])
+        shell = np.zeros(dims, bool)
+        if L == 0 and reservoir:
+            shell[0, :, :] = shell[-1, :, :] = True
+            shell[:, 0, :] = shell[:, -1, :] = True
+            shell[:, :, 0] = shell[:, :, -1] = True
+        if L < Lmax:
+            R = D & refine(L, cx, cy, cz)
+            if L == 0 and reservoir:
+                if (R & _dilate(shell, _OFF26)).any():
+                    raise ValueError("refinement touches the boundary layer")
+            notR = ~R
+            band = R & _dilate(notR, _OFF26, periodic if L == 0 else (False,) * 3)
+            edge = np.zeros(dims, bool)
+            edge[0, :, :] = edge[-1, :, :] = edge[:, 0, :] = edge[:, -1, :] = True
+            edge[:, :, 0] = edge[:, :, -1] = True
+            if (R & _dilate(edge, _OFF26)).any():
+                raise ValueError(f"level {L}: refinement touches the domain edge")
+            interior = R & ~band
+            if (_dilate(R, _OFF26) & C2F).any():
+                raise ValueError(f"level {L}: refinement too close to the C2F band")
+        else:
+            R = np.zeros(dims, bool)
+            band = np.zeros(dims, bool)
+            interior = R
+        sol = np.zeros(dims, bool)
+        if L == Lmax and solid is not None:
+            sol = D & solid(cx, cy, cz) & ~C2F
+        elif solid is not None:
+            if (D & ~interior & solid(cx, cy, cz)).any():
+                raise ValueError(f"level {L}: solid is not fully inside the finest level")
+        bcres = np.zeros(dims, bool)
+        if L == 0 and reservoir:
+            bcres = shell.copy()
+        fluid = D & ~interior & ~band & ~C2F & ~sol & ~bcres
+        kind = np.zeros(dims, dtype=np.uint16)
+        kind[fluid] |= KIND_FLUID
+        kind[band] |= KIND_F2C
+        kind[C2F] |= KIND_C2F
+        kind[bcres] |= KIND_BCRES
+        if L > 0:
+            ghost = _dilate(C2F, qoff) & ~D
+            # the parents of the ghost layer must be coarse fluid cells
+            par = prev["kind"].repeat(2, 0).repeat(2, 1).repeat(2, 2)
+            if (ghost & ((par & KIND_FLUID) == 0)).any():
+                raise ValueError(f"level {L}: ghost_C2F parent is not a fluid cell")
+            kind[ghost] |= KIND_GHOST_C2F
+            # ghost_F2C on the coarse level (Lat_Voxelize.cpp:715-731)
+            pk = prev["kind"]
+            gf = _dilate(prev["F2C"], qoff) & ((pk & KIND_FLUID) != 0)
+            pk[gf] |= KIND_GHOST_F2C
+        if L == Lmax and solid is not None:
+            # keep only solid cells that some fluid cell can pull from
+            live = _dilate(fluid | C2F, qoff) & sol
+            kind[sol & ~live] = 0
+            kind[sol & live] |= KIND_SOLID
+        if L == 0:
+            if reservoir:
+                _face_bits(kind, dims, [(BC_RESERVOIR, 0, 0, 0)] * 6, bcres)
+            else:
+                _face_bits(kind, dims, faces, fluid)
+        rec = {"R": R, "F2C": band, "kind": kind, "dims": dims}
+        grids.append(rec)
+        prev = rec
+    dis_keys = np.zeros(0, np.uint64)
+    dis_vals = np.zeros((0, Q - 1))
+    if ib_dist is not None:
+        top = grids[Lmax]
+        dims = top["dims"]
+        cx, cy, cz = _centers(dims, Lmax)
+        h = 1.0 / (1 << Lmax)
+        ex, ey, ez = velocities(Q)
+        fl = (top["kind"] & (KIND_FLUID | KIND_C2F)) != 0
+        vals = np.full(dims + (Q - 1,), -1.0)
+        for q in range(1, Q):
+            t = ib_dist(cx, cy, cz, ex[q], ey[q], ez[q], h)
+            ok = fl & ~np.isnan(t)
+            vals[..., q - 1][ok] = t[ok]
+        has = fl & (vals >= 0).any(axis=-1)
+        top["kind"][has] |= KIND_DIS
+        x, y, z = np.nonzero(has)
+        dk = encode_key(x, y, z, Lmax, Lmax)
+        order = np.argsort(dk)
+        dis_keys = dk[order]
+        dis_vals = vals[x, y, z][order]
+    for L in range(Lmax + 1):
+        levels.append(_emit(grids[L]["kind"], L, Lmax))
+    return levels, dis_keys, dis_vals
+
+
+def _base_case(Q, Lmax, tau0, collision, dims0, faces, levels, dis_keys, dis_vals,
+               u0=(0.0, 0.0, 0.0), rho0=1.0, ib=IB_NONE, les_C=0.0, cd=None, name=""):
+    return {
+        "name": name, "Q": Q, "Lmax": Lmax, "tau0": float(tau0),
+        "collision": collision, "les_C": float(les_C), "rho0": float(rho0),
+        "u0": tuple(float(v) for v in u0), "dims0": tuple(int(d) for d in dims0),
+        "faces": [tuple(f) for f in faces], "levels": levels,
+        "dis_keys": dis_keys, "dis_vals": np.ascontiguousarray(dis_vals, dtype=np.float64),
+        "ib": ib, "cd": cd or {},
+    }
+
+
+def taus(case):
+    """Per-level relaxation time tau_L = 2^L (tau_0 - 1/2) + 1/2 (LBM_Manager.cpp:814-818)."""
+    return [(1 << L) * (case["tau0"] - 0.5) + 0.5 for L in range(case["Lmax"] + 1)]
+
+
+# ---- workloads ---------------------------------------------------------------
+def cavity(n=64, Q=19, tau=0.6, lid=0.05, collision=COLLIDE_LBGK, refine_box=None, name=None):
+    """Lid-driven cavity: n^3 wet-node cells, Zou-He no-slip on five faces and a
+    Zou-He velocity lid (u_x=lid) on z-max; f0 = feq(1, 0).  With refine_box =
+    ((x0,y0,z0),(x1,y1,z1)) in coarse cells, that box is refined once."""
+    faces = [(BC_ZOUHE, 0.0, 0.0, 0.0)] * 5 + [(BC_ZOUHE, float(lid), 0.0, 0.0)]
+    dims0 = (n, n, n) if np.isscalar(n) else tuple(n)
+    Lmax = 0 if refine_box is None else 1
+
+    def refine(L, cx, cy, cz):
+        (x0, y0, z0), (x1, y1, z1) = refine_box
+        return ((cx >= x0) & (cx < x1) & (cy >= y0) & (cy < y1) & (cz >= z0) & (cz < z1))
+
+    levels, dk, dv = build_nested(dims0, Lmax, refine, faces=faces, Q=Q)
+    return _base_case(Q, Lmax, tau, collision, dims0, faces, levels, dk, dv,
+                      name=name or f"cavity{dims0[0]}")
+
+
+def reservoir_box(dims0=(16, 12, 12), Q=19, tau=0.6, u0=(0.05, 0.0, 0.0), collision=COLLIDE_LBGK):
+    """The compiled reference layout: a frozen lat_bc shell holding feq(rho0,u0)
+    around lat_f (LBM_Manager.cpp:482-505, Lat_Voxelize.cpp:242-288), NEE on all
+    faces (LBM_Manager.cpp:52-80)."""
+    faces = [(BC_RESERVOIR,) + tuple(u0)] * 6
+    levels, dk, dv = build_nested(dims0, 0, None, faces=faces, Q=Q, reservoir=True)
+    return _base_case(Q, 0, tau, collision, dims0, faces, levels, dk, dv, u0=u0,
+                      name="reservoir_box")
+
+
+def sphere_ib_dist(center, radius):
+    C = np.asarray(center, dtype=np.float64)
+
+    def dist(cx, cy, cz, ex, ey, ez, h):
+        dx = cx - C[0]
+        dy = cy - C[1]
+        dz = cz - C[2]
+        a = h * h * float(ex * ex + ey * ey + ez * ez)
+        b = 2.0 * h * (ex * dx + ey * dy + ez * dz)
+        c = dx * dx + dy * dy + dz * dz - radius * radius
+        disc = b * b - 4.0 * a * c
+        with np.errstate(invalid="ignore"):
+            t = (-b - np.sqrt(disc)) / (2.0 * a)
+        ok = (disc >= 0) & (c > 0) & (t >= 0.0) & (t <= 1.0)
+        return np.where(ok, t, np.nan)
+
+    return dist
+
+
+def sphere_amr(coarse=(64, 32, 32), radius=8.0, center=(16.0, 16.0, 16.0), shells=(6.0, 3.0),
+               Q=27, collision=COLLIDE_CUMULANT, U=0.04, Re=100.0, ib=IB_FH, les_C=0.0,
+               periodic_yz=True, name=None):
+    """Refined sphere in a channel: Zou-He inflow u_x=U at x=0, outflow at x-max,
+    periodic y and z; level L+1 covers coarse cells whose centre lies within
+    radius+shells[L] of the sphere centre; cells inside the sphere are solid on the
+    finest level and the cut links carry analytic fractions.  Re = U*2R/nu0."""
+    Lmax = len(shells)
+    nu0 = U * 2.0 * radius / Re
+    tau0 = 3.0 * nu0 + 0.5
+    C = np.asarray(center, dtype=np.float64)
+    pk = BC_PERIODIC if periodic_yz else BC_ZOUHE
+    faces = [(BC_ZOUHE, U, 0.0, 0.0), (BC_OUTFLOW, 0.0, 0.0, 0.0)] + [(pk, 0.0, 0.0, 0.0)] * 4
+
+    def r_of(cx, cy, cz):
+        return np.sqrt((cx - C[0]) ** 2 + (cy - C[1]) ** 2 + (cz - C[2]) ** 2)
+
+    def refine(L, cx, cy, cz):
+        return r_of(cx, cy, cz) <= radius + shells[L]
+
+    def solid(cx, cy, cz):
+        return r_of(cx, cy, cz) < radius
+
+    levels, dk, dv = build_nested(coarse, Lmax, refine, solid=solid, faces=faces, Q=Q,
+                                  ib_dist=sphere_ib_dist(C, radius))
+    d_lb = 2.0 * radius * (1 << Lmax)
+    return _base_case(Q, Lmax, tau0, collision, coarse, faces, levels, dk, dv,
+                      u0=(U, 0.0, 0.0), ib=ib, les_C=les_C,
+                      cd={"U": U, "d": d_lb}, name=name or "sphere_amr")
+
+
+def level_counts(case):
+    out = []
+    for lv in case["levels"]:
+        k = lv["kinds"]
+        out.append({
+            "cells": int(k.size),
+            "fluid": int(((k & KIND_FLUID) != 0).sum()),
+            "f2c": int(((k & KIND_F2C) != 0).sum()),
+            "c2f": int(((k & KIND_C2F) != 0).sum()),
+            "ghost_c2f": int(((k & KIND_GHOST_C2F) != 0).sum()),
+            "solid": int(((k & KIND_SOLID) != 0).sum()),
+            "bcres": int(((k & KIND_BCRES) != 0).sum()),
+            "dis": int(((k & KIND_DIS) != 0).sum()),
+        })
+    return out
+
+
+def updates_per_coarse_step(case):
+    """Cell updates per coarse step, a level-L sub-step counting once: sum_L (lat_f+C2F)_L * 2^L."""
+    tot = 0
+    for L, c in enumerate(level_counts(case)):
+        tot += (c["fluid"] + c["c2f"]) << L
+    return tot
+
+
+def reference_sphere(coarse=(24, 16, 16), radius=3.0, center=(8.0, 8.0, 8.0), shells=(2.5, 1.25), Q=19,
+                     collision=COLLIDE_LBGK, U=0.05, Re=100.0, ib=IB_FH, name=None):
+    """The configuration the compiled reference runs (LBM_Manager.cpp:38-80,
+    header/Constants.h:164-235): LBGK, a frozen lat_bc reservoir holding
+    feq(rho0, U) on all six faces with NEE on its f, FH link interpolation plus
+    multi-direct forcing on the finest level, every cell initialised to feq(1, U)."""
+    Lmax = len(shells)
+    nu0 = U * 2.0 * radius / Re
+    tau0 = 3.0 * nu0 + 0.5
+    C = np.asarray(center, dtype=np.float64)
+    faces = [(BC_RESERVOIR, U, 0.0, 0.0)] * 6
+
+    def r_of(cx, cy, cz):
+        return np.sqrt((cx - C[0]) ** 2 + (cy - C[1]) ** 2 + (cz - C[2]) ** 2)
+
+    levels, dk, dv = build_nested(coarse, Lmax, lambda L, cx, cy, cz: r_of(cx, cy, cz) <= radius + shells[L],
+                                  solid=lambda cx, cy, cz: r_of(cx, cy, cz) < radius, faces=faces, Q=Q,
+                                  ib_dist=sphere_ib_dist(C, radius), reservoir=True)
+    return _base_case(Q, Lmax, tau0, collision, coarse, faces, levels, dk, dv, u0=(U, 0.0, 0.0), ib=ib,
+                      cd={"U": U, "d": 2.0 * radius * (1 << Lmax)}, name=name or "reference_sphere")

@@ -1,0 +1,478 @@
+// mt_api.hip -- the C-ABI (include/mt_amr.h) and the coarse-step schedule.
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mt_case.h"
+
+namespace mt {
+
+static thread_local std::string g_err;
+void set_error(const std::string& s) { g_err = s; }
+
+#define CHECK_CASE(c)                                   \
+  do {                                                  \
+    if (!(c)) { set_error("null case"); return MT_ERR_ARG; } \
+  } while (0)
+#define CHECK_FINAL(c)                                                         \
+  do {                                                                         \
+    CHECK_CASE(c);                                                             \
+    if (!(c)->finalized) { set_error("case not finalized"); return MT_ERR_STATE; } \
+  } while (0)
+#define CHECK_LEVEL(c, L)                                                      \
+  do {                                                                         \
+    if ((L) < 0 || (L) > (c)->Lmax) { set_error("level out of range"); return MT_ERR_ARG; } \
+  } while (0)
+#define TRY(x)                    \
+  do {                            \
+    int r_ = (x);                 \
+    if (r_ != MT_OK) return r_;   \
+  } while (0)
+
+static bool has_reservoir(const Case& c) {
+  for (int f = 0; f < 6; ++f)
+    if (c.d.face_kind[f] == MT_BC_RESERVOIR) return true;
+  return false;
+}
+
+// Return the case to reference-state buffers: after mt_step the current fcol
+// buffer already holds the next collision and the reference's fcol is the other.
+static void settle(Case& c) {
+  if (!c.ahead) return;
+  for (int L = 0; L <= c.Lmax; ++L) c.lv[L].pf ^= 1;
+  c.ahead = false;
+}
+
+// The prologue of a step: calculateMacros(L) + collide(L) on every level, the
+// first two calls of each level's LBMkernel sub-step (LBM_Simulator.cpp:26-28).
+static int prologue(Case& c) {
+  for (int L = 0; L <= c.Lmax; ++L) {
+    TRY(launch_macros(c, L));
+    TRY(launch_collide(c, L));
+  }
+  c.ahead = true;
+  return MT_OK;
+}
+
+// LBMkernel(L) (LBM_Simulator.cpp:20-229) over the fused kernels.  On entry
+// fcol[pf] of every level holds the collision of the sub-step about to stream.
+static int run_level(Case& c, int L, bool last) {
+  DevLevel& l = c.lv[L];
+  const bool res0 = (L == 0) && has_reservoir(c);
+  // sub-step A
+  if (L < c.Lmax) TRY(run_level(c, L + 1, L == 0 ? last : false));
+  if (L > 0) TRY(launch_explode(c, L));
+  TRY(launch_fused(c, L, (L == 0 && last) || res0, res0));
+  if (L < c.Lmax) TRY(launch_coalesce(c, L, true));
+  l.pf ^= 1;
+  l.pr ^= 1;
+  if (L == 0) return MT_OK;
+  // sub-step B: no explosion (LBM_Simulator.cpp:152-217)
+  if (L < c.Lmax) TRY(run_level(c, L + 1, last));
+  TRY(launch_fused(c, L, last, false));
+  if (L < c.Lmax) TRY(launch_coalesce(c, L, true));
+  l.pf ^= 1;
+  l.pr ^= 1;
+  return MT_OK;
+}
+
+static int parity_key(const Case& c, bool last) {
+  int k = last ? 1 : 0;
+  for (int L = 0; L <= c.Lmax; ++L) k |= (c.lv[L].pf << (1 + 2 * L)) | (c.lv[L].pr << (2 + 2 * L));
+  return k;
+}
+
+static int coarse_step(Case& c, bool last) {
+  if (!c.graphs || c.timing) return run_level(c, 0, last);
+  int key = parity_key(c, last);
+  auto it = c.graph_exec.find(key);
+  int pf[kMaxLevels], pr[kMaxLevels];
+  for (int L = 0; L <= c.Lmax; ++L) { pf[L] = c.lv[L].pf; pr[L] = c.lv[L].pr; }
+  if (it == c.graph_exec.end()) {
+    hipGraph_t g;
+    MT_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    int r = run_level(c, 0, last);
+    hipError_t e = hipStreamEndCapture(c.stream, &g);
+    if (r) return r;
+    if (e != hipSuccess) { set_error(std::string("graph capture: ") + hipGetErrorString(e)); return MT_ERR_HIP; }
+    hipGraphExec_t ex;
+    MT_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+    MT_HIP(hipGraphDestroy(g));
+    c.graph_exec[key] = ex;
+    it = c.graph_exec.find(key);
+    for (int L = 0; L <= c.Lmax; ++L) { c.lv[L].pf = pf[L]; c.lv[L].pr = pr[L]; }
+  }
+  MT_HIP(hipGraphLaunch(it->second, c.stream));
+  // parities advance as in the captured schedule: level L flips 2^L times
+  // per coarse step except level 0 which flips once
+  for (int L = 0; L <= c.Lmax; ++L) {
+    int flips = L == 0 ? 1 : (1 << L);
+    c.lv[L].pf = pf[L] ^ (flips & 1);
+    c.lv[L].pr = pr[L] ^ (flips & 1);
+  }
+  return MT_OK;
+}
+
+static int drain_timing(Case& c) {
+  if (c.ev_pending.empty()) return MT_OK;
+  MT_HIP(hipStreamSynchronize(c.stream));
+  for (auto& p : c.ev_pending) {
+    float ms = 0;
+    MT_HIP(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+    c.t_ms[p.first] += ms;
+    c.t_calls[p.first] += 1;
+    c.ev_pool.push_back(p.second.first);
+    c.ev_pool.push_back(p.second.second);
+  }
+  c.ev_pending.clear();
+  return MT_OK;
+}
+
+}  // namespace mt
+
+using namespace mt;
+
+struct mt_case : public mt::Case {};
+
+extern "C" {
+
+const char* mt_last_error(void) { return g_err.c_str(); }
+
+int mt_case_create(const mt_case_desc* d, mt_case** out) {
+  if (!d || !out) { set_error("null argument"); return MT_ERR_ARG; }
+  if (d->Q != 19 && d->Q != 27) { set_error("Q must be 19 or 27"); return MT_ERR_ARG; }
+  if (d->Lmax < 0 || d->Lmax >= kMaxLevels) { set_error("Lmax out of range"); return MT_ERR_ARG; }
+  if (d->collision == MT_COLLIDE_CUMULANT && d->Q != 27) {
+    // Collision_Cumulant::collide requires D3Q27 (Collision_Cumulant.cpp:24-26)
+    set_error("the cumulant collision needs Q = 27");
+    return MT_ERR_ARG;
+  }
+  if (d->precision < MT_FP64 || d->precision > MT_FP16_STORAGE) { set_error("bad precision"); return MT_ERR_ARG; }
+  if (d->tau0 <= 0.5) { set_error("tau0 must exceed 1/2"); return MT_ERR_ARG; }
+  for (int f = 0; f < 6; ++f)
+    if (d->face_kind[f] < MT_BC_NONE || d->face_kind[f] > MT_BC_PERIODIC) { set_error("bad face kind"); return MT_ERR_ARG; }
+  for (int a = 0; a < 3; ++a) {
+    bool p0 = d->face_kind[2 * a] == MT_BC_PERIODIC, p1 = d->face_kind[2 * a + 1] == MT_BC_PERIODIC;
+    if (p0 != p1) { set_error("periodic faces must come in pairs"); return MT_ERR_ARG; }
+  }
+  mt_case* c = new mt_case;
+  c->d = *d;
+  c->Q = d->Q;
+  c->Lmax = d->Lmax;
+  c->prec = d->precision;
+  // tau_L = 2^L (tau0 - 1/2) + 1/2: nu_lb doubles per level (LBM_Manager.cpp:814-818)
+  for (int L = 0; L <= c->Lmax; ++L) c->tau[L] = double(1 << L) * (d->tau0 - 0.5) + 0.5;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    delete c;
+    return MT_ERR_HIP;
+  }
+  *out = c;
+  return MT_OK;
+}
+
+int mt_case_destroy(mt_case* c) {
+  if (!c) return MT_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (auto& g : c->graph_exec) hipGraphExecDestroy(g.second);
+  for (auto e : c->ev_pool) hipEventDestroy(e);
+  for (auto& p : c->ev_pending) {
+    hipEventDestroy(p.second.first);
+    hipEventDestroy(p.second.second);
+  }
+  for (void* p : c->allocs) hipFree(p);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return MT_OK;
+}
+
+int mt_case_upload_level(mt_case* c, int level, const uint64_t* keys, const uint16_t* kinds, int64_t n) {
+  CHECK_CASE(c);
+  CHECK_LEVEL(c, level);
+  if (c->finalized) { set_error("case already finalized"); return MT_ERR_STATE; }
+  if (n < 0 || (n && (!keys || !kinds))) { set_error("bad key array"); return MT_ERR_ARG; }
+  if (n > (int64_t)1 << 30) { set_error("too many cells on one level"); return MT_ERR_ARG; }
+  DevLevel& l = c->lv[level];
+  l.h_keys.assign(keys, keys + n);
+  l.h_kinds.assign(kinds, kinds + n);
+  return MT_OK;
+}
+
+int mt_case_upload_dis(mt_case* c, const uint64_t* keys, const double* vals, int64_t n) {
+  CHECK_CASE(c);
+  if (c->finalized) { set_error("case already finalized"); return MT_ERR_STATE; }
+  if (n < 0 || (n && (!keys || !vals))) { set_error("bad dis array"); return MT_ERR_ARG; }
+  c->h_dis_keys.assign(keys, keys + n);
+  c->h_dis_vals.assign(vals, vals + n * (c->Q - 1));
+  return MT_OK;
+}
+
+int mt_case_finalize(mt_case* c) {
+  CHECK_CASE(c);
+  if (c->finalized) { set_error("case already finalized"); return MT_ERR_STATE; }
+  TRY(build_tables(*c));
+  int64_t max_fp = 1;
+  for (int L = 0; L <= c->Lmax; ++L) {
+    DevLevel& l = c->lv[L];
+    int64_t ns = l.n_slots;
+    l.f = c->alloc<char>(ns * c->Q * (int64_t)c->pop_bytes());
+    for (int j = 0; j < 2; ++j) {
+      l.fcol[j] = c->alloc<char>(ns * c->Q * (int64_t)c->pop_bytes());
+      l.rho[j] = c->alloc<char>(ns * (int64_t)c->real_bytes());
+      l.u[j] = c->alloc<char>(3 * ns * (int64_t)c->real_bytes());
+      if (!l.fcol[j] || !l.rho[j] || !l.u[j]) { set_error("out of device memory"); return MT_ERR_HIP; }
+    }
+    if (!l.f) { set_error("out of device memory"); return MT_ERR_HIP; }
+    max_fp = std::max<int64_t>(max_fp, (ns + 255) / 256);
+    max_fp = std::max<int64_t>(max_fp, (l.n_active + 3) / 4);
+  }
+  c->force_part_cap = max_fp;
+  c->force_part = c->alloc<double>(3 * max_fp);
+  c->force_acc = c->alloc<double>(8);
+  c->nee_tmp = c->alloc<double>((int64_t)c->Q * std::max<int64_t>(1, c->lv[0].n_bcres));
+  c->red_cap = 5 * 1024;
+  c->red_tmp = c->alloc<double>(c->red_cap);
+  if (!c->force_part || !c->force_acc || !c->nee_tmp || !c->red_tmp) { set_error("out of device memory"); return MT_ERR_HIP; }
+  MT_HIP(hipMemsetAsync(c->force_acc, 0, 8 * sizeof(double), c->stream));
+  TRY(init_state(*c));
+  MT_HIP(hipStreamSynchronize(c->stream));
+  for (int L = 0; L <= c->Lmax; ++L) {
+    std::vector<uint64_t>().swap(c->lv[L].h_keys);
+    std::vector<uint16_t>().swap(c->lv[L].h_kinds);
+  }
+  c->finalized = true;
+  return MT_OK;
+}
+
+int mt_case_level_info(mt_case* c, int level, int64_t* n_cells, int64_t* n_bricks) {
+  CHECK_FINAL(c);
+  CHECK_LEVEL(c, level);
+  if (n_cells) *n_cells = c->lv[level].n_cells;
+  if (n_bricks) *n_bricks = c->lv[level].n_bricks;
+  return MT_OK;
+}
+
+int mt_find(mt_case* c, int level, const uint64_t* keys, int64_t n, int32_t* slot_out) {
+  CHECK_FINAL(c);
+  CHECK_LEVEL(c, level);
+  if (n <= 0) return MT_OK;
+  uint64_t* dk = nullptr;
+  int32_t* ds = nullptr;
+  MT_HIP(hipMalloc((void**)&dk, 8 * n));
+  MT_HIP(hipMalloc((void**)&ds, 4 * n));
+  MT_HIP(hipMemcpyAsync(dk, keys, 8 * n, hipMemcpyHostToDevice, c->stream));
+  int r = find_slots(*c, level, dk, n, ds);
+  if (r == MT_OK) {
+    hipError_t e = hipMemcpyAsync(slot_out, ds, 4 * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { set_error(hipGetErrorString(e)); r = MT_ERR_HIP; }
+  }
+  hipStreamSynchronize(c->stream);
+  hipFree(dk);
+  hipFree(ds);
+  return r;
+}
+
+static int field_io(mt_case* c, int level, int field, const uint64_t* keys, int64_t n, double* out,
+                    const double* in) {
+  CHECK_FINAL(c);
+  CHECK_LEVEL(c, level);
+  if (field < MT_FIELD_F || field > MT_FIELD_U) { set_error("bad field"); return MT_ERR_ARG; }
+  if (n <= 0) return MT_OK;
+  int rows = field <= MT_FIELD_FCOL ? c->Q : (field == MT_FIELD_RHO ? 1 : 3);
+  uint64_t* dk = nullptr;
+  int32_t* ds = nullptr;
+  double* dv = nullptr;
+  MT_HIP(hipMalloc((void**)&dk, 8 * n));
+  MT_HIP(hipMalloc((void**)&ds, 4 * n));
+  MT_HIP(hipMalloc((void**)&dv, 8 * n * rows));
+  MT_HIP(hipMemcpyAsync(dk, keys, 8 * n, hipMemcpyHostToDevice, c->stream));
+  int r = find_slots(*c, level, dk, n, ds);
+  std::vector<int32_t> hs(n);
+  if (r == MT_OK) {
+    hipError_t e = hipMemcpyAsync(hs.data(), ds, 4 * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { set_error(hipGetErrorString(e)); r = MT_ERR_HIP; }
+  }
+  if (r == MT_OK)
+    for (int64_t i = 0; i < n; ++i)
+      if (hs[i] < 0) {
+        char buf[96];
+        snprintf(buf, sizeof buf, "key %llu absent on level %d", (unsigned long long)keys[i], level);
+        set_error(buf);
+        r = MT_ERR_LOOKUP;
+        break;
+      }
+  if (r == MT_OK) {
+    DevLevel& l = c->lv[level];
+    int buf = field == MT_FIELD_FCOL ? (c->ahead ? 1 - l.pf : l.pf) : l.pr;
+    if (out) {
+      r = launch_gather(*c, level, field, ds, n, dv, buf);
+      if (r == MT_OK) {
+        hipError_t e = hipMemcpyAsync(out, dv, 8 * n * rows, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) { set_error(hipGetErrorString(e)); r = MT_ERR_HIP; }
+      }
+    } else {
+      settle(*c);
+      buf = field == MT_FIELD_FCOL ? l.pf : l.pr;
+      hipError_t e = hipMemcpyAsync(dv, in, 8 * n * rows, hipMemcpyHostToDevice, c->stream);
+      if (e != hipSuccess) { set_error(hipGetErrorString(e)); r = MT_ERR_HIP; }
+      if (r == MT_OK) r = launch_scatter(*c, level, field, ds, n, dv, buf);
+      // rho/u live in both buffers for cells the step does not rewrite
+      if (r == MT_OK && field >= MT_FIELD_RHO) r = launch_scatter(*c, level, field, ds, n, dv, 1 - buf);
+      if (r == MT_OK && field == MT_FIELD_FCOL) {
+        // keep the second fcol buffer coherent for cells no kernel rewrites
+        r = launch_scatter(*c, level, field, ds, n, dv, 1 - buf);
+      }
+    }
+  }
+  hipStreamSynchronize(c->stream);
+  hipFree(dk);
+  hipFree(ds);
+  hipFree(dv);
+  return r;
+}
+
+int mt_get_field(mt_case* c, int level, int field, const uint64_t* keys, int64_t n, double* out) {
+  if (!out && n > 0) { set_error("null output"); return MT_ERR_ARG; }
+  return field_io(c, level, field, keys, n, out, nullptr);
+}
+int mt_set_field(mt_case* c, int level, int field, const uint64_t* keys, int64_t n, const double* in) {
+  if (!in && n > 0) { set_error("null input"); return MT_ERR_ARG; }
+  return field_io(c, level, field, keys, n, nullptr, in);
+}
+
+int mt_macros(mt_case* c, int level) {
+  CHECK_FINAL(c);
+  CHECK_LEVEL(c, level);
+  settle(*c);
+  return launch_macros(*c, level);
+}
+int mt_collide(mt_case* c, int level) {
+  CHECK_FINAL(c);
+  CHECK_LEVEL(c, level);
+  settle(*c);
+  return launch_collide(*c, level);
+}
+int mt_stream(mt_case* c, int level) {
+  CHECK_FINAL(c);
+  CHECK_LEVEL(c, level);
+  settle(*c);
+  return launch_stream(*c, level);
+}
+int mt_explode(mt_case* c, int level) {
+  CHECK_FINAL(c);
+  CHECK_LEVEL(c, level);
+  if (level == 0) { set_error("explosion needs a coarser level"); return MT_ERR_ARG; }
+  settle(*c);
+  // the seam writes the single logical fcol; the ghost layer is mirrored into
+  // the second buffer as the schedule does
+  return launch_explode(*c, level);
+}
+int mt_coalesce(mt_case* c, int level) {
+  CHECK_FINAL(c);
+  CHECK_LEVEL(c, level);
+  if (level == c->Lmax) { set_error("coalescence needs a finer level"); return MT_ERR_ARG; }
+  settle(*c);
+  return launch_coalesce(*c, level, false);
+}
+int mt_apply_bc(mt_case* c) {
+  CHECK_FINAL(c);
+  settle(*c);
+  return launch_bc(*c);
+}
+int mt_ib_treat(mt_case* c, double* force3) {
+  CHECK_FINAL(c);
+  settle(*c);
+  if (c->d.ib == MT_IB_NONE || c->lv[c->Lmax].n_dis == 0) {
+    if (force3) force3[0] = force3[1] = force3[2] = 0;
+    return MT_OK;
+  }
+  TRY(launch_ib(*c, true));
+  if (force3) {
+    double acc[8];
+    MT_HIP(hipMemcpyAsync(acc, c->force_acc, sizeof acc, hipMemcpyDeviceToHost, c->stream));
+    MT_HIP(hipStreamSynchronize(c->stream));
+    force3[0] = acc[4];
+    force3[1] = acc[5];
+    force3[2] = acc[6];
+  }
+  return MT_OK;
+}
+
+int mt_step(mt_case* c, int n) {
+  CHECK_FINAL(c);
+  if (n < 0) { set_error("negative step count"); return MT_ERR_ARG; }
+  if (n == 0) return MT_OK;
+  if (!c->ahead) TRY(prologue(*c));
+  for (int i = 0; i < n; ++i) TRY(coarse_step(*c, i == n - 1));
+  if (c->timing) TRY(drain_timing(*c));
+  return MT_OK;
+}
+
+int mt_set_graphs(mt_case* c, int on) {
+  CHECK_CASE(c);
+  c->graphs = on != 0;
+  return MT_OK;
+}
+int mt_set_timing(mt_case* c, int on) {
+  CHECK_CASE(c);
+  c->timing = on != 0;
+  for (int L = 0; L < kMaxLevels; ++L) {
+    c->t_ms[L] = 0;
+    c->t_calls[L] = 0;
+  }
+  return MT_OK;
+}
+int mt_get_timing(mt_case* c, double* ms, int64_t* calls, int max_levels) {
+  CHECK_CASE(c);
+  TRY(drain_timing(*c));
+  for (int L = 0; L < max_levels && L < kMaxLevels; ++L) {
+    if (ms) ms[L] = c->t_ms[L];
+    if (calls) calls[L] = c->t_calls[L];
+  }
+  return MT_OK;
+}
+int mt_sync(mt_case* c) {
+  CHECK_CASE(c);
+  MT_HIP(hipStreamSynchronize(c->stream));
+  return MT_OK;
+}
+
+int mt_reduce(mt_case* c, int level, double* mass, double* momentum3, int64_t* nan_count) {
+  CHECK_FINAL(c);
+  CHECK_LEVEL(c, level);
+  double out[5];
+  TRY(launch_reduce(*c, level, out));
+  if (mass) *mass = out[0];
+  if (momentum3) {
+    momentum3[0] = out[1];
+    momentum3[1] = out[2];
+    momentum3[2] = out[3];
+  }
+  if (nan_count) *nan_count = (int64_t)out[4];
+  return MT_OK;
+}
+
+int mt_ib_force(mt_case* c, double* sum3, int64_t* count, int reset) {
+  CHECK_FINAL(c);
+  double acc[8];
+  MT_HIP(hipMemcpyAsync(acc, c->force_acc, sizeof acc, hipMemcpyDeviceToHost, c->stream));
+  MT_HIP(hipStreamSynchronize(c->stream));
+  if (sum3) {
+    sum3[0] = acc[0];
+    sum3[1] = acc[1];
+    sum3[2] = acc[2];
+  }
+  if (count) *count = (int64_t)acc[3];
+  if (reset) {
+    MT_HIP(hipMemsetAsync(c->force_acc, 0, 4 * sizeof(double), c->stream));
+    MT_HIP(hipStreamSynchronize(c->stream));
+  }
+  return MT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+// mt_case.h -- host-side representation of one device-resident AMR case.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mt_amr.h"
+
+namespace mt {
+
+constexpr int kMaxLevels = 7;
+
+struct DevLevel {
+  int64_t n_cells = 0, n_bricks = 0, n_slots = 0;
+  int32_t dims[3] = {0, 0, 0};
+  // upload staging (host)
+  std::vector<uint64_t> h_keys;
+  std::vector<uint16_t> h_kinds;
+  // tables
+  uint64_t* bcode = nullptr;  // [n_bricks] sorted brick Morton codes
+  int32_t* bnbr = nullptr;    // [n_bricks][27] neighbour bricks (-1 none)
+  uint64_t* key = nullptr;    // [n_slots] reference key, ~0 when empty
+  uint16_t* kind = nullptr;   // [n_slots] MT_KIND_* (0 = empty)
+  uint32_t* pmask = nullptr;  // [n_slots] bit q: neighbour along e_q holds populations
+  // state (storage type depends on precision)
+  void* f = nullptr;          // [Q][n_slots]
+  void* fcol[2] = {nullptr, nullptr};
+  void* rho[2] = {nullptr, nullptr};  // [n_slots]
+  void* u[2] = {nullptr, nullptr};    // [3][n_slots]
+  int pf = 0;                 // current fcol buffer
+  int pr = 0;                 // current rho/u buffer
+  // work lists
+  int32_t* active = nullptr;  // bricks holding lat_f or C2F cells
+  int64_t n_active = 0;
+  int32_t* f2c = nullptr;     // F2C slots of this level (coarse side)
+  int64_t n_f2c = 0;
+  int32_t* f2c_child = nullptr;        // [n_f2c][8] child slots on level+1
+  int32_t* f2c_child_write = nullptr;  // [n_f2c][8] explosion target or -1 (child in dis)
+  int32_t* gf2c = nullptr;             // ghost_F2C slots
+  int64_t n_gf2c = 0;
+  int32_t* gf2c_child = nullptr;       // [n_gf2c][8] ghost_C2F target or -1
+  int32_t* dis_row = nullptr;          // [n_slots] row in dis table or -1 (finest level)
+  double* dis = nullptr;               // [n_dis][Q-1]
+  int64_t n_dis = 0;
+  int32_t* bcres = nullptr;
+  int64_t n_bcres = 0;
+};
+
+struct Case {
+  mt_case_desc d;
+  int Q = 19, Lmax = 0, prec = MT_FP64;
+  double tau[kMaxLevels];
+  DevLevel lv[kMaxLevels];
+  std::vector<uint64_t> h_dis_keys;
+  std::vector<double> h_dis_vals;
+  bool finalized = false;
+  bool ahead = false;  // fcol[pf] already holds the next collision (mt_step state)
+  bool graphs = true;
+  bool timing = false;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+  // IB force
+  double* force_part = nullptr;  // per-block partials of the finest fused kernel
+  int64_t force_part_cap = 0;
+  double* force_acc = nullptr;   // [4]: sum x,y,z, count
+  double* nee_tmp = nullptr;     // NEE Jacobi staging
+  double* red_tmp = nullptr;     // reduction partials
+  int64_t red_cap = 0;
+  // graphs: key = (start parity of level 0) | (final << 1)
+  std::map<int, hipGraphExec_t> graph_exec;
+  // timing
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+  double t_ms[kMaxLevels] = {0};
+  int64_t t_calls[kMaxLevels] = {0};
+  size_t pop_bytes() const { return prec == MT_FP64 ? 8 : (prec == MT_FP32_SHIFTED ? 4 : 2); }
+  size_t real_bytes() const { return prec == MT_FP64 ? 8 : 4; }
+
+  template <typename T>
+  T* alloc(int64_t n) {
+    void* p = nullptr;
+    if (n <= 0) n = 1;
+    if (hipMalloc(&p, sizeof(T) * n) != hipSuccess) return nullptr;
+    allocs.push_back(p);
+    return (T*)p;
+  }
+};
+
+void set_error(const std::string& s);
+#define MT_HIP(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) {                                                         \
+      mt::set_error(std::string(#x) + ": " + hipGetErrorString(e_));               \
+      return MT_ERR_HIP;                                                            \
+    }                                                                               \
+  } while (0)
+
+// tables (mt_tables.hip)
+int build_tables(Case& c);
+int find_slots(Case& c, int level, const uint64_t* d_keys, int64_t n, int32_t* d_out);
+// kernels (mt_kernels.hip)
+int init_state(Case& c);
+int launch_macros(Case& c, int L);
+int launch_collide(Case& c, int L);
+int launch_stream(Case& c, int L);
+int launch_explode(Case& c, int L);
+int launch_coalesce(Case& c, int L, bool collide_after);
+int launch_bc(Case& c);
+int launch_ib(Case& c, bool reduce_into_acc);
+int launch_fused(Case& c, int L, bool write_f, bool bc_nee_after);
+int launch_gather(Case& c, int L, int field, const int32_t* d_slots, int64_t n, double* d_out, int buf);
+int launch_scatter(Case& c, int L, int field, const int32_t* d_slots, int64_t n, const double* d_in, int buf);
+int launch_reduce(Case& c, int L, double* h_out);
+
+}  // namespace mt

@@ -1,0 +1,1037 @@
+// mt_kernels.hip -- the per-level LBM update kernels.
+//
+// Two families share one set of device functions:
+//  * seam kernels, one per reference method (calculateMacros, collide, stream,
+//    explosion, coalescence, boundary strategies, treat), each reading and
+//    writing the single logical f / fcol / rho / u state exactly where the
+//    reference's method does;
+//  * the fused kernel used by mt_step: for every lat_f / C2F cell of a level it
+//    pulls fcol from the neighbours, applies the level-0 wet-node boundary or the
+//    finest-level solid boundary, computes the moments and collides, writing
+//    the next sub-step's fcol into the other buffer.  The reference runs
+//    stream -> boundary -> (next sub-step) macros -> collide on the same f with
+//    no other writer in between (LBM_Simulator.cpp:20-229), so the fused order
+//    computes the same values while reading and writing each population once.
+#include <algorithm>
+#include <type_traits>
+#include <vector>
+
+#include "mt_case.h"
+#include "mt_device.h"
+
+namespace mt {
+
+constexpr uint16_t kHasPop = MT_KIND_FLUID | MT_KIND_F2C | MT_KIND_C2F | MT_KIND_GHOST_C2F | MT_KIND_SOLID | MT_KIND_BCRES;
+
+struct BcCfg {
+  int kind[6];
+  double u[6][3];
+  int wet_mask;   // faces with MT_BC_ZOUHE or MT_BC_OUTFLOW
+};
+
+struct LvArgs {
+  int64_t ns;
+  const int32_t* __restrict__ bnbr;
+  const uint16_t* __restrict__ kind;
+  const uint32_t* __restrict__ pmask;
+  void* f;
+  const void* fcol_in;
+  void* fcol_out;
+  const void* rho_in;
+  const void* u_in;
+  void* rho_out;
+  void* u_out;
+};
+
+// ------------------------------------------------------------------ addressing
+// slot of the neighbour at (DX,DY,DZ) in {-1,0,1}^3; N holds the 27
+// neighbour-brick indices (wave-uniform).  Written so that, once the caller's
+// q loop is unrolled, every N[] index is a constant.
+__device__ __forceinline__ int nbr_slot(const int* N, int lx, int ly, int lz, int DX, int DY, int DZ) {
+  const bool cx = DX > 0 ? (lx == 3) : (DX < 0 ? (lx == 0) : false);
+  const bool cy = DY > 0 ? (ly == 3) : (DY < 0 ? (ly == 0) : false);
+  const bool cz = DZ > 0 ? (lz == 3) : (DZ < 0 ? (lz == 0) : false);
+  const int OX = DX + 1, OY = DY + 1, OZ = DZ + 1;
+  int nb = cz ? (cy ? (cx ? N[OX + 3 * OY + 9 * OZ] : N[1 + 3 * OY + 9 * OZ])
+                    : (cx ? N[OX + 3 + 9 * OZ] : N[1 + 3 + 9 * OZ]))
+              : (cy ? (cx ? N[OX + 3 * OY + 9] : N[1 + 3 * OY + 9]) : (cx ? N[OX + 3 + 9] : N[13]));
+  int nl = brick_local((lx + DX) & 3, (ly + DY) & 3, (lz + DZ) & 3);
+  return nb < 0 ? -1 : ((nb << 6) | nl);
+}
+// general offset (|d| <= 4) through the table in memory (rare paths)
+__device__ __forceinline__ int slot_off(const int32_t* bnbr, int s, int dx, int dy, int dz) {
+  int lx, ly, lz;
+  brick_coords(s & 63, lx, ly, lz);
+  return slot_at(bnbr + (int64_t)(s >> 6) * 27, lx, ly, lz, dx, dy, dz);
+}
+
+template <typename T>
+__device__ __forceinline__ T ld_real(const void* p, int64_t i) { return ((const T*)p)[i]; }
+template <typename T>
+__device__ __forceinline__ void st_real(void* p, int64_t i, T v) { ((T*)p)[i] = v; }
+
+// ------------------------------------------------------------------ boundaries
+// Wet-node level-0 boundary on one cell (definitions restated in the oracle;
+// the uncompiled reference text is BC/ZouHe_Velocity.cpp:170-532 and
+// BC/Outflow.cpp:12-136).  Under the pull f_q(x) <- fcol_q(x+e_q) the unknowns
+// on a face with outward normal n are the e_q.n = +1 populations.
+// nf(dx,dy,dz,q) returns the post-stream f_q of the cell at offset d.
+template <int Q, typename R, typename NF>
+__device__ __forceinline__ void wet_bc(R* f, uint16_t k, const BcCfg& bc, NF nf) {
+  const int act = ((k >> 8) & 63) & bc.wet_mask;
+  if (!act) return;
+  if (__popc(act) == 1) {
+    const int face = __ffs(act) - 1;
+    const int ax = face >> 1;
+    const int sg = (face & 1) ? 1 : -1;
+    if (bc.kind[face] == MT_BC_OUTFLOW) {
+      const int nx = ax == 0 ? sg : 0, ny = ax == 1 ? sg : 0, nz = ax == 2 ? sg : 0;
+#pragma unroll
+      for (int q = 1; q < Q; ++q) {
+        const int ea = ax == 0 ? kEX[q] : (ax == 1 ? kEY[q] : kEZ[q]);
+        if (ea * sg == 1) f[q] = nf(-nx, -ny, -nz, q);
+      }
+      return;
+    }
+    const R uw[3] = {(R)bc.u[face][0], (R)bc.u[face][1], (R)bc.u[face][2]};
+    R S0 = 0, Sin = 0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int en = (ax == 0 ? kEX[q] : (ax == 1 ? kEY[q] : kEZ[q])) * sg;
+      if (en == 0) S0 += f[q];
+      else if (en == -1) Sin += f[q];
+    }
+    const R un = uw[ax] * (R)sg;
+    const R rho = (S0 + (R)2 * Sin) / ((R)1 - un);
+    const int t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+    R N1 = 0, N2 = 0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int en = (ax == 0 ? kEX[q] : (ax == 1 ? kEY[q] : kEZ[q])) * sg;
+      const int e1 = t1 == 0 ? kEX[q] : kEY[q];
+      if (en == 0 && e1) N1 += (R)e1 * f[q];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int en = (ax == 0 ? kEX[q] : (ax == 1 ? kEY[q] : kEZ[q])) * sg;
+      const int e2 = t2 == 1 ? kEY[q] : kEZ[q];
+      if (en == 0 && e2) N2 += (R)e2 * f[q];
+    }
+    N1 = (R)0.5 * N1 - rho * uw[t1] / (R)3.0;
+    N2 = (R)0.5 * N2 - rho * uw[t2] / (R)3.0;
+    R g[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) g[q] = f[q];
+#pragma unroll
+    for (int q = 1; q < Q; ++q) {
+      const int en = (ax == 0 ? kEX[q] : (ax == 1 ? kEY[q] : kEZ[q])) * sg;
+      if (en == 1) {
+        const int e1 = t1 == 0 ? kEX[q] : kEY[q];
+        const int e2 = t2 == 1 ? kEY[q] : kEZ[q];
+        const R w = (R)wq<Q>(q);
+        const R eu = (R)kEX[q] * uw[0] + (R)kEY[q] * uw[1] + (R)kEZ[q] * uw[2];
+        const R tr = (R)e1 * N1 + (R)e2 * N2;
+        f[q] = g[qinv(q)] + (R)6 * w * rho * eu - (R)36 * w * tr;
+      }
+    }
+    return;
+  }
+  // edge / corner: non-equilibrium extrapolation from the inward diagonal cell
+  int dx = 0, dy = 0, dz = 0, uface = -1;
+  for (int ff = 0; ff < 6; ++ff)
+    if (act & (1 << ff)) {
+      const int sg = (ff & 1) ? 1 : -1, ax = ff >> 1;
+      if (ax == 0) dx -= sg; else if (ax == 1) dy -= sg; else dz -= sg;
+      if (bc.kind[ff] == MT_BC_ZOUHE) uface = ff;
+    }
+  R fn[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) fn[q] = nf(dx, dy, dz, q);
+  R rn, ux, uy, uz;
+  cell_macros<Q, R>(fn, rn, ux, uy, uz);
+  const R wx = uface >= 0 ? (R)bc.u[uface][0] : ux;
+  const R wy = uface >= 0 ? (R)bc.u[uface][1] : uy;
+  const R wz = uface >= 0 ? (R)bc.u[uface][2] : uz;
+  const R uu = ux * ux + uy * uy + uz * uz, ww = wx * wx + wy * wy + wz * wz;
+#pragma unroll
+  for (int q = 0; q < Q; ++q)
+    f[q] = feq_c<Q, R>(q, rn, wx, wy, wz, ww) + fn[q] - feq_c<Q, R>(q, rn, ux, uy, uz, uu);
+}
+
+// Solid boundary on one dis cell (SinglePointInterpolation::treat,
+// ComplexBoundary_Manager.cpp:63-225): link interpolation (FH or distance
+// weighted) on wall links, pull or bounce-back elsewhere, density clamp,
+// eight multi-direct-forcing sweeps.  Only fcol and the density map are read,
+// so the result is independent of the post-stream f it replaces.
+// fc(q, dir) returns fcol_q at the neighbour along e_dir (dir 0 = the cell itself)
+// or NaN-free fallback through 'present'.
+template <int Q, typename R, typename FC>
+__device__ __forceinline__ void ib_cell(R* f, const double* drow, R rho_b, uint32_t pm, int method, FC fc,
+                                        double& fx, double& fy, double& fz) {
+  f[0] = fc(0, 0);
+  double sx = 0, sy = 0, sz = 0;
+#pragma unroll
+  for (int q = 1; q < Q; ++q) {
+    const int qi = qinv(q);
+    const double dist = drow[qi - 1];
+    if (dist != -1.0) {
+      const R finc = fc(qi, 0);
+      const R fs = ((pm >> qi) & 1) ? fc(q, qi) : finc;  // fcol_q(x - e_q)
+      R fnew;
+      const R d = (R)dist;
+      if (method == MT_IB_FH) {
+        if (d < (R)0.5) {
+          fnew = ((R)1 - (R)2 * d) * finc + (R)2 * d * fs;
+        } else {
+          const R fe = (R)wib(qi) * rho_b;
+          const R i2d = (R)1 / ((R)2 * d);
+          fnew = i2d * fs + ((R)1 - i2d) * fe;
+        }
+      } else {
+        fnew = ((R)1 - d) * finc + d * fs;
+      }
+      f[q] = fnew;
+      const double m = (double)finc + (double)fnew;
+      if (kEX[q]) sx -= m * kEX[q];
+      if (kEY[q]) sy -= m * kEY[q];
+      if (kEZ[q]) sz -= m * kEZ[q];
+    } else {
+      f[q] = ((pm >> q) & 1) ? fc(q, q) : fc(qi, 0);
+    }
+  }
+  fx += sx;
+  fy += sy;
+  fz += sz;
+  R ra = 0;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) ra += f[q];
+  if (ra < (R)0.3) f[0] += (R)1.0 * ((R)0.3 - ra);
+  else if (ra < (R)0.5) f[0] += (R)0.5 * ((R)0.5 - ra);
+  else if (ra > (R)3.0) f[0] += (R)1.0 * ((R)3.0 - ra);
+  else if (ra > (R)2.0) f[0] += (R)0.5 * ((R)2.0 - ra);
+  for (int it = 0; it < 8; ++it) {
+    R rho = 0, ux = 0, uy = 0, uz = 0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      rho += f[q];
+      if (kEX[q]) ux += (R)kEX[q] * f[q];
+      if (kEY[q]) uy += (R)kEY[q] * f[q];
+      if (kEZ[q]) uz += (R)kEZ[q] * f[q];
+    }
+    if (rho <= (R)0) continue;
+    ux /= rho;
+    uy /= rho;
+    uz /= rho;
+    const R um = sqrt(ux * ux + uy * uy + uz * uz);
+    if (um < (R)1e-10) continue;
+    const R mx = (R)-0.5 * rho * ux, my = (R)-0.5 * rho * uy, mz = (R)-0.5 * rho * uz;
+    R adj = 0;
+#pragma unroll
+    for (int q = 1; q < Q; ++q) {
+      if (drow[qinv(q) - 1] == -1.0) continue;
+      const R df = (R)wib(q) * (R)3.0 * ((R)kEX[q] * mx + (R)kEY[q] * my + (R)kEZ[q] * mz);
+      f[q] += df;
+      adj -= df;
+    }
+    f[0] += adj;
+  }
+}
+
+// collision of one cell in place (LBGK or cumulant, optional Smagorinsky)
+template <int Q, typename R>
+__device__ __forceinline__ void collide_cell(R* f, R rho, R ux, R uy, R uz, R tau, R lesC, int model) {
+  if (lesC > (R)0) tau = smag_tau<Q, R>(f, rho, ux, uy, uz, tau, lesC);
+  if (Q == 27 && model == MT_COLLIDE_CUMULANT)
+    collide_cumulant<R>(f, rho, ux, uy, uz, tau);
+  else
+    collide_lbgk<Q, R>(f, rho, ux, uy, uz, tau);
+}
+
+__device__ __forceinline__ void block_reduce3(double& x, double& y, double& z, double* lds) {
+  // wave reduction then cross-wave in LDS (256 threads = 4 waves)
+  for (int off = 32; off > 0; off >>= 1) {
+    x += __shfl_down(x, off, 64);
+    y += __shfl_down(y, off, 64);
+    z += __shfl_down(z, off, 64);
+  }
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) {
+    lds[3 * w] = x;
+    lds[3 * w + 1] = y;
+    lds[3 * w + 2] = z;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    x = lds[0] + lds[3] + lds[6] + lds[9];
+    y = lds[1] + lds[4] + lds[7] + lds[10];
+    z = lds[2] + lds[5] + lds[8] + lds[11];
+  }
+}
+
+// ------------------------------------------------------------------ fused step
+struct FusedArgs {
+  LvArgs v;
+  const int32_t* __restrict__ active;
+  int64_t n_active;
+  double tau, lesC;
+  int model;
+  int write_f;
+  int do_bc;
+  BcCfg bc;
+  int do_ib;
+  int ib_method;
+  const int32_t* __restrict__ dis_row;
+  const double* __restrict__ dis;
+  int Qm1;
+  double* force_part;
+};
+
+template <int Q, typename S>
+__global__ __launch_bounds__(256) void k_fused(FusedArgs a) {
+  typedef typename RealOf<S>::type R;
+  __shared__ double red[12];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t bi = (int64_t)blockIdx.x * 4 + wave;
+  double fx = 0, fy = 0, fz = 0;
+  const S* fin = (const S*)a.v.fcol_in;
+  S* fout = (S*)a.v.fcol_out;
+  S* fst = (S*)a.v.f;
+  const int64_t ns = a.v.ns;
+  if (bi < a.n_active) {
+    const int b = __builtin_amdgcn_readfirstlane(a.active[bi]);
+    int N[27];
+#pragma unroll
+    for (int o = 0; o < 27; ++o) N[o] = __builtin_amdgcn_readfirstlane(a.v.bnbr[(int64_t)b * 27 + o]);
+    const int s = (b << 6) | lane;
+    const uint16_t k = a.v.kind[s];
+    if (k & (MT_KIND_FLUID | MT_KIND_C2F)) {
+      int lx, ly, lz;
+      brick_coords(lane, lx, ly, lz);
+      const uint32_t pm = a.v.pmask[s];
+      R f[Q];
+      f[0] = load_pop<Q, S, R>(fin, ns, 0, s);
+#pragma unroll
+      for (int q = 1; q < Q; ++q) {
+        if ((pm >> q) & 1) {
+          const int t = nbr_slot(N, lx, ly, lz, kEX[q], kEY[q], kEZ[q]);
+          f[q] = load_pop<Q, S, R>(fin, ns, q, t);
+        } else {
+          f[q] = load_pop<Q, S, R>(fst, ns, q, s);
+        }
+      }
+      if (a.do_bc && (k & 0x3F00)) {
+        const int32_t* bn = a.v.bnbr;
+        const uint32_t* pmk = a.v.pmask;
+        auto nf = [&](int dx, int dy, int dz, int q) -> R {
+          const int t = slot_off(bn, s, dx, dy, dz);
+          if (t < 0) return f[q];  // no such cell: keep the cell's own value
+          if (q == 0) return load_pop<Q, S, R>(fin, ns, 0, t);
+          if ((pmk[t] >> q) & 1) {
+            const int u = slot_off(bn, t, kEX[q], kEY[q], kEZ[q]);
+            return load_pop<Q, S, R>(fin, ns, q, u);
+          }
+          return load_pop<Q, S, R>(fst, ns, q, t);
+        };
+        wet_bc<Q, R>(f, k, a.bc, nf);
+      }
+      const int row = (a.do_ib && (k & MT_KIND_DIS)) ? a.dis_row[s] : -1;
+      if (row >= 0 && (k & MT_KIND_FLUID)) {
+        const double* drow = a.dis + (int64_t)row * a.Qm1;
+        const R rho_b = (R)ld_real<R>(a.v.rho_in, s);
+        auto fc = [&](int q, int dir) -> R {
+          int t = s;
+          if (dir) t = nbr_slot(N, lx, ly, lz, kEX[dir], kEY[dir], kEZ[dir]);
+          return load_pop<Q, S, R>(fin, ns, q, t);
+        };
+        ib_cell<Q, R>(f, drow, rho_b, pm, a.ib_method, fc, fx, fy, fz);
+      }
+      if ((k & MT_KIND_C2F) || a.write_f) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) store_pop<Q, S, R>(fst, ns, q, s, f[q]);
+      }
+      if (k & MT_KIND_FLUID) {
+        R rho, ux, uy, uz;
+        cell_macros<Q, R>(f, rho, ux, uy, uz);
+        st_real<R>(a.v.rho_out, s, rho);
+        st_real<R>(a.v.u_out, s, ux);
+        st_real<R>(a.v.u_out, ns + s, uy);
+        st_real<R>(a.v.u_out, 2 * ns + s, uz);
+        collide_cell<Q, R>(f, rho, ux, uy, uz, (R)a.tau, (R)a.lesC, a.model);
+      }
+#pragma unroll
+      for (int q = 0; q < Q; ++q) store_pop<Q, S, R>(fout, ns, q, s, f[q]);
+    }
+  }
+  if (a.do_ib) {
+    block_reduce3(fx, fy, fz, red);
+    if (threadIdx.x == 0) {
+      a.force_part[3 * blockIdx.x] = fx;
+      a.force_part[3 * blockIdx.x + 1] = fy;
+      a.force_part[3 * blockIdx.x + 2] = fz;
+    }
+  }
+}
+
+// fixed-order sum of the per-block IB force partials into the accumulator
+// (LBM_Manager::accumIBForce, LBM_Manager.cpp:846-852); last[3] gets this call
+__global__ void k_force_reduce(const double* part, int64_t nb, double* acc) {
+  __shared__ double lds[3 * 256];
+  double x = 0, y = 0, z = 0;
+  for (int64_t i = threadIdx.x; i < nb; i += 256) {
+    x += part[3 * i];
+    y += part[3 * i + 1];
+    z += part[3 * i + 2];
+  }
+  lds[3 * threadIdx.x] = x;
+  lds[3 * threadIdx.x + 1] = y;
+  lds[3 * threadIdx.x + 2] = z;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sx = 0, sy = 0, sz = 0;
+    for (int i = 0; i < 256; ++i) {
+      sx += lds[3 * i];
+      sy += lds[3 * i + 1];
+      sz += lds[3 * i + 2];
+    }
+    acc[0] += sx;
+    acc[1] += sy;
+    acc[2] += sz;
+    acc[3] += 1.0;
+    acc[4] = sx;
+    acc[5] = sy;
+    acc[6] = sz;
+  }
+}
+
+// ------------------------------------------------------------------ seams
+struct SeamArgs {
+  LvArgs v;
+  double tau, lesC;
+  int model;
+};
+
+// LBM_Manager::calculateMacros(level) (LBM_Manager.cpp:1003-1050)
+template <int Q, typename S>
+__global__ void k_macros(SeamArgs a) {
+  typedef typename RealOf<S>::type R;
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.v.ns) return;
+  if (!(a.v.kind[s] & (MT_KIND_FLUID | MT_KIND_F2C))) return;
+  R f[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) f[q] = load_pop<Q, S, R>((const S*)a.v.f, a.v.ns, q, s);
+  R rho, ux, uy, uz;
+  cell_macros<Q, R>(f, rho, ux, uy, uz);
+  st_real<R>(a.v.rho_out, s, rho);
+  st_real<R>(a.v.u_out, s, ux);
+  st_real<R>(a.v.u_out, a.v.ns + s, uy);
+  st_real<R>(a.v.u_out, 2 * a.v.ns + s, uz);
+}
+
+// Collision_Model::collide(level): lat_f and F2C collide with the map moments,
+// C2F copy f -> fcol (Collision_Manager.cpp:41-52)
+template <int Q, typename S>
+__global__ void k_collide(SeamArgs a) {
+  typedef typename RealOf<S>::type R;
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.v.ns) return;
+  const uint16_t k = a.v.kind[s];
+  if (!(k & (MT_KIND_FLUID | MT_KIND_F2C | MT_KIND_C2F))) return;
+  R f[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) f[q] = load_pop<Q, S, R>((const S*)a.v.f, a.v.ns, q, s);
+  if (k & (MT_KIND_FLUID | MT_KIND_F2C)) {
+    const R rho = ld_real<R>(a.v.rho_in, s);
+    const R ux = ld_real<R>(a.v.u_in, s), uy = ld_real<R>(a.v.u_in, a.v.ns + s),
+            uz = ld_real<R>(a.v.u_in, 2 * a.v.ns + s);
+    collide_cell<Q, R>(f, rho, ux, uy, uz, (R)a.tau, (R)a.lesC, a.model);
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)a.v.fcol_out, a.v.ns, q, s, f[q]);
+}
+
+// ABPatternPull::stream(level) (Stream_Manager.cpp:43-76)
+template <int Q, typename S>
+__global__ void k_stream(SeamArgs a) {
+  typedef typename RealOf<S>::type R;
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.v.ns) return;
+  if (!(a.v.kind[s] & (MT_KIND_FLUID | MT_KIND_C2F))) return;
+  const S* fin = (const S*)a.v.fcol_in;
+  S* fst = (S*)a.v.f;
+  const uint32_t pm = a.v.pmask[s];
+  store_pop<Q, S, R>(fst, a.v.ns, 0, (int)s, load_pop<Q, S, R>(fin, a.v.ns, 0, (int)s));
+#pragma unroll
+  for (int q = 1; q < Q; ++q)
+    if ((pm >> q) & 1) {
+      const int t = slot_off(a.v.bnbr, (int)s, kEX[q], kEY[q], kEZ[q]);
+      store_pop<Q, S, R>(fst, a.v.ns, q, (int)s, load_pop<Q, S, R>(fin, a.v.ns, q, t));
+    }
+}
+
+// level-0 wet-node boundaries applied to f after stream
+template <int Q, typename S>
+__global__ void k_bc_wet(SeamArgs a, BcCfg bc) {
+  typedef typename RealOf<S>::type R;
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.v.ns) return;
+  const uint16_t k = a.v.kind[s];
+  if (!(k & MT_KIND_FLUID) || !(((k >> 8) & 63) & bc.wet_mask)) return;
+  S* fst = (S*)a.v.f;
+  R f[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) f[q] = load_pop<Q, S, R>(fst, a.v.ns, q, (int)s);
+  const int32_t* bn = a.v.bnbr;
+  const int64_t ns = a.v.ns;
+  auto nf = [&](int dx, int dy, int dz, int q) -> R {
+    const int t = slot_off(bn, (int)s, dx, dy, dz);
+    return t < 0 ? f[q] : load_pop<Q, S, R>(fst, ns, q, t);
+  };
+  wet_bc<Q, R>(f, k, bc, nf);
+  // every cell this writes is read by no other boundary cell: in place is Jacobi
+#pragma unroll
+  for (int q = 0; q < Q; ++q) store_pop<Q, S, R>(fst, a.v.ns, q, (int)s, f[q]);
+}
+
+// Non-equilibrium extrapolation on the reservoir cells
+// (BC/NonEquilibrumExtrapolation.cpp:50-66), Jacobi form: every read of f is
+// of the pre-pass state; results staged in tmp[Q][n] then scattered.
+template <int Q, typename S>
+__global__ void k_nee(SeamArgs a, BcCfg bc, const int32_t* list, int64_t n, double* tmp) {
+  typedef typename RealOf<S>::type R;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = list[i];
+  const uint16_t k = a.v.kind[s];
+  const int fm = (k >> 8) & 63;
+  const S* fst = (const S*)a.v.f;
+  const R rb = ld_real<R>(a.v.rho_in, s);
+  // the face strategy that owns this cell: West/East own their edges and corners,
+  // South/North the remaining edges, Bot/Top their face
+  // (header/BC/NonEquilibrumExtrapolation.hpp:24-109)
+  int owner = (fm & 1) ? 0 : (fm & 2) ? 1 : (fm & 4) ? 2 : (fm & 8) ? 3 : (fm & 16) ? 4 : 5;
+  const R bx = (R)bc.u[owner][0], by = (R)bc.u[owner][1], bz = (R)bc.u[owner][2];
+  const R bb = bx * bx + by * by + bz * bz;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    bool ok = true;
+    for (int ff = 0; ff < 6; ++ff)
+      if (fm & (1 << ff)) {
+        const int ax = ff >> 1, sg = (ff & 1) ? 1 : -1;
+        const int ea = ax == 0 ? kEX[q] : (ax == 1 ? kEY[q] : kEZ[q]);
+        if (ea * sg > 0) ok = false;
+      }
+    double out = (double)load_pop<Q, S, R>(fst, a.v.ns, q, s);
+    const int t = ok ? slot_off(a.v.bnbr, s, kEX[q], kEY[q], kEZ[q]) : -1;
+    if (t >= 0) {
+      const R rn = ld_real<R>(a.v.rho_in, t);
+      const R ux = ld_real<R>(a.v.u_in, t), uy = ld_real<R>(a.v.u_in, a.v.ns + t),
+              uz = ld_real<R>(a.v.u_in, 2 * a.v.ns + t);
+      const R fw = feq_c<Q, R>(q, rb, bx, by, bz, bb);
+      const R fn = feq_c<Q, R>(q, rn, ux, uy, uz, ux * ux + uy * uy + uz * uz);
+      out = (double)(fw + load_pop<Q, S, R>(fst, a.v.ns, q, t) - fn);
+    }
+    tmp[(int64_t)q * n + i] = out;
+  }
+}
+template <int Q, typename S>
+__global__ void k_nee_scatter(SeamArgs a, const int32_t* list, int64_t n, const double* tmp) {
+  typedef typename RealOf<S>::type R;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)a.v.f, a.v.ns, q, list[i], (R)tmp[(int64_t)q * n + i]);
+}
+
+// SinglePointInterpolation::treat as a seam: f of dis cells from fcol + rho map
+template <int Q, typename S>
+__global__ __launch_bounds__(256) void k_ib(SeamArgs a, const int32_t* dis_row, const double* dis, int method,
+                                            double* force_part) {
+  typedef typename RealOf<S>::type R;
+  __shared__ double red[12];
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double fx = 0, fy = 0, fz = 0;
+  if (s < a.v.ns) {
+    const uint16_t k = a.v.kind[s];
+    if ((k & MT_KIND_DIS) && (k & MT_KIND_FLUID) && dis_row[s] >= 0) {
+      const S* fin = (const S*)a.v.fcol_in;
+      const int64_t ns = a.v.ns;
+      const int32_t* bn = a.v.bnbr;
+      const uint32_t pm = a.v.pmask[s];
+      const double* drow = dis + (int64_t)dis_row[s] * (Q - 1);
+      const R rho_b = ld_real<R>(a.v.rho_in, s);
+      auto fc = [&](int q, int dir) -> R {
+        int t = (int)s;
+        if (dir) t = slot_off(bn, (int)s, kEX[dir], kEY[dir], kEZ[dir]);
+        return load_pop<Q, S, R>(fin, ns, q, t);
+      };
+      R f[Q];
+      ib_cell<Q, R>(f, drow, rho_b, pm, method, fc, fx, fy, fz);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)a.v.f, ns, q, (int)s, f[q]);
+    }
+  }
+  block_reduce3(fx, fy, fz, red);
+  if (threadIdx.x == 0) {
+    force_part[3 * blockIdx.x] = fx;
+    force_part[3 * blockIdx.x + 1] = fy;
+    force_part[3 * blockIdx.x + 2] = fz;
+  }
+}
+
+// ------------------------------------------------------------------ transfers
+struct XferArgs {
+  int64_t ns_c, ns_f;
+  const void* src;     // coarse fcol (explosion) / fine f (coalescence)
+  void* dst;           // fine fcol (explosion) / coarse f (coalescence)
+  void* dst2;          // explosion: second fine fcol buffer for ghost_C2F targets
+  const int32_t* list; // parent slots
+  int64_t n;
+  const int32_t* child;  // [n][8]
+  double scale, rho0;
+  // coalescence + collision of the F2C cell
+  int collide;
+  void* fcol_out;
+  void* rho_out;
+  void* u_out;
+  double tau, lesC;
+  int model;
+};
+
+// AMR_transDDF_explosion (LBM_Manager.cpp:1059-1129)
+template <int Q, typename S>
+__global__ void k_explode(XferArgs a, int ghost) {
+  typedef typename RealOf<S>::type R;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int p = a.list[i];
+  R fc[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) fc[q] = load_pop<Q, S, R>((const S*)a.src, a.ns_c, q, p);
+  R rho, ux, uy, uz;
+  cell_macros<Q, R>(fc, rho, ux, uy, uz);
+  const R uu = ux * ux + uy * uy + uz * uz;
+  const R sc = (R)a.scale;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)a.rho0);
+    fc[q] = fe + sc * (fc[q] - fe);
+  }
+  for (int j = 0; j < 8; ++j) {
+    const int t = a.child[i * 8 + j];
+    if (t < 0) continue;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      store_pop<Q, S, R>((S*)a.dst, a.ns_f, q, t, fc[q]);
+      if (ghost && a.dst2) store_pop<Q, S, R>((S*)a.dst2, a.ns_f, q, t, fc[q]);
+    }
+  }
+}
+
+// AMR_transDDF_coalescence (LBM_Manager.cpp:1131-1169); with collide set, the
+// F2C cell's next calculateMacros + collide follow in the same thread.
+template <int Q, typename S>
+__global__ __launch_bounds__(256) void k_coalesce(XferArgs a) {
+  typedef typename RealOf<S>::type R;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int p = a.list[i];
+  int ch[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) ch[j] = a.child[i * 8 + j];
+  R f[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    R s = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s += load_pop<Q, S, R>((const S*)a.src, a.ns_f, q, ch[j]);
+    f[q] = s / (R)8.0;
+  }
+  R rho, ux, uy, uz;
+  cell_macros<Q, R>(f, rho, ux, uy, uz);
+  const R uu = ux * ux + uy * uy + uz * uz;
+  const R sc = (R)a.scale;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)a.rho0);
+    f[q] = fe + sc * (f[q] - fe);
+    store_pop<Q, S, R>((S*)a.dst, a.ns_c, q, p, f[q]);
+  }
+  if (a.collide) {
+    cell_macros<Q, R>(f, rho, ux, uy, uz);
+    st_real<R>(a.rho_out, p, rho);
+    st_real<R>(a.u_out, p, ux);
+    st_real<R>(a.u_out, a.ns_c + p, uy);
+    st_real<R>(a.u_out, 2 * a.ns_c + p, uz);
+    collide_cell<Q, R>(f, rho, ux, uy, uz, (R)a.tau, (R)a.lesC, a.model);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)a.fcol_out, a.ns_c, q, p, f[q]);
+  }
+}
+
+// ------------------------------------------------------------------ state I/O
+// allocMacrosAndDDF (LBM_Manager.cpp:482-592): f = fcol = feq(rho0,u0); solid
+// cells zero with rho = 1, u = 0
+template <int Q, typename S>
+__global__ void k_init(LvArgs v, void* fcol2, void* rho2, void* u2, double rho0, double u0x, double u0y,
+                       double u0z) {
+  typedef typename RealOf<S>::type R;
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= v.ns) return;
+  const uint16_t k = v.kind[s];
+  const bool sol = k & MT_KIND_SOLID;
+  const R uu = (R)(u0x * u0x + u0y * u0y + u0z * u0z);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    R fe = (k && !sol) ? feq_c<Q, R>(q, (R)rho0, (R)u0x, (R)u0y, (R)u0z, uu) : (R)0;
+    store_pop<Q, S, R>((S*)v.f, v.ns, q, (int)s, fe);
+    store_pop<Q, S, R>((S*)v.fcol_out, v.ns, q, (int)s, fe);
+    store_pop<Q, S, R>((S*)fcol2, v.ns, q, (int)s, fe);
+  }
+  R r = 0, a = 0, b = 0, c = 0;
+  if (sol) r = 1;
+  else if (k & (MT_KIND_FLUID | MT_KIND_F2C | MT_KIND_C2F | MT_KIND_BCRES)) {
+    r = (R)rho0; a = (R)u0x; b = (R)u0y; c = (R)u0z;
+  }
+  void* rr[2] = {v.rho_out, rho2};
+  void* uuv[2] = {v.u_out, u2};
+  for (int j = 0; j < 2; ++j) {
+    st_real<R>(rr[j], s, r);
+    st_real<R>(uuv[j], s, a);
+    st_real<R>(uuv[j], v.ns + s, b);
+    st_real<R>(uuv[j], 2 * v.ns + s, c);
+  }
+}
+
+template <int Q, typename S>
+__global__ void k_gather(LvArgs v, int field, const int32_t* slots, int64_t n, double* out, const void* src) {
+  typedef typename RealOf<S>::type R;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = slots[i];
+  if (field <= MT_FIELD_FCOL) {
+    for (int q = 0; q < Q; ++q) out[(int64_t)q * n + i] = (double)load_pop<Q, S, R>((const S*)src, v.ns, q, s);
+  } else if (field == MT_FIELD_RHO) {
+    out[i] = (double)ld_real<R>(src, s);
+  } else {
+    for (int c = 0; c < 3; ++c) out[(int64_t)c * n + i] = (double)ld_real<R>(src, (int64_t)c * v.ns + s);
+  }
+}
+template <int Q, typename S>
+__global__ void k_scatter(LvArgs v, int field, const int32_t* slots, int64_t n, const double* in, void* dst) {
+  typedef typename RealOf<S>::type R;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = slots[i];
+  if (field <= MT_FIELD_FCOL) {
+    for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)dst, v.ns, q, s, (R)in[(int64_t)q * n + i]);
+  } else if (field == MT_FIELD_RHO) {
+    st_real<R>(dst, s, (R)in[i]);
+  } else {
+    for (int c = 0; c < 3; ++c) st_real<R>(dst, (int64_t)c * v.ns + s, (R)in[(int64_t)c * n + i]);
+  }
+}
+
+// mass, momentum and NaN count over lat_f and F2C from f (FP64, fixed order)
+template <int Q, typename S>
+__global__ __launch_bounds__(256) void k_reduce(LvArgs v, double* part) {
+  typedef typename RealOf<S>::type R;
+  __shared__ double lds[5 * 256];
+  double m = 0, px = 0, py = 0, pz = 0, nn = 0;
+  for (int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x; s < v.ns; s += (int64_t)gridDim.x * 256) {
+    if (!(v.kind[s] & (MT_KIND_FLUID | MT_KIND_F2C))) continue;
+    for (int q = 0; q < Q; ++q) {
+      const double fq = (double)load_pop<Q, S, R>((const S*)v.f, v.ns, q, (int)s);
+      if (fq != fq) nn += 1;
+      m += fq;
+      px += kEX[q] * fq;
+      py += kEY[q] * fq;
+      pz += kEZ[q] * fq;
+    }
+  }
+  double vals[5] = {m, px, py, pz, nn};
+  for (int j = 0; j < 5; ++j) lds[j * 256 + threadIdx.x] = vals[j];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int j = 0; j < 5; ++j) {
+      double t = 0;
+      for (int i = 0; i < 256; ++i) t += lds[j * 256 + i];
+      part[(int64_t)blockIdx.x * 5 + j] = t;
+    }
+}
+
+// ================================================================== launchers
+static inline int64_t nblk(int64_t n, int b = 256) { return n <= 0 ? 1 : (n + b - 1) / b; }
+
+static LvArgs lvargs(Case& c, int L, int fbuf_in, int fbuf_out, int rbuf_in, int rbuf_out) {
+  DevLevel& l = c.lv[L];
+  LvArgs v;
+  v.ns = l.n_slots;
+  v.bnbr = l.bnbr;
+  v.kind = l.kind;
+  v.pmask = l.pmask;
+  v.f = l.f;
+  v.fcol_in = l.fcol[fbuf_in];
+  v.fcol_out = l.fcol[fbuf_out];
+  v.rho_in = l.rho[rbuf_in];
+  v.u_in = l.u[rbuf_in];
+  v.rho_out = l.rho[rbuf_out];
+  v.u_out = l.u[rbuf_out];
+  return v;
+}
+static BcCfg bccfg(const Case& c) {
+  BcCfg b;
+  b.wet_mask = 0;
+  for (int f = 0; f < 6; ++f) {
+    b.kind[f] = c.d.face_kind[f];
+    for (int a = 0; a < 3; ++a) b.u[f][a] = c.d.face_u[f][a];
+    if (b.kind[f] == MT_BC_ZOUHE || b.kind[f] == MT_BC_OUTFLOW) b.wet_mask |= 1 << f;
+  }
+  return b;
+}
+
+// dispatch on (Q, storage)
+#define MT_DISPATCH(c, KERNEL_CALL)                                                         \
+  do {                                                                                      \
+    if ((c).Q == 19) {                                                                      \
+      if ((c).prec == MT_FP64) { constexpr int Q_ = 19; typedef double S_; KERNEL_CALL; }   \
+      else if ((c).prec == MT_FP32_SHIFTED) { constexpr int Q_ = 19; typedef float S_; KERNEL_CALL; } \
+      else { constexpr int Q_ = 19; typedef _Float16 S_; KERNEL_CALL; }                     \
+    } else {                                                                                \
+      if ((c).prec == MT_FP64) { constexpr int Q_ = 27; typedef double S_; KERNEL_CALL; }   \
+      else if ((c).prec == MT_FP32_SHIFTED) { constexpr int Q_ = 27; typedef float S_; KERNEL_CALL; } \
+      else { constexpr int Q_ = 27; typedef _Float16 S_; KERNEL_CALL; }                     \
+    }                                                                                       \
+  } while (0)
+
+int init_state(Case& c) {
+  for (int L = 0; L <= c.Lmax; ++L) {
+    DevLevel& l = c.lv[L];
+    LvArgs v = lvargs(c, L, 0, 0, 0, 0);
+    MT_DISPATCH(c, (k_init<Q_, S_><<<nblk(l.n_slots), 256, 0, c.stream>>>(v, l.fcol[1], l.rho[1], l.u[1], c.d.rho0,
+                                                                          c.d.u0[0], c.d.u0[1], c.d.u0[2])));
+    l.pf = 0;
+    l.pr = 0;
+  }
+  MT_HIP(hipGetLastError());
+  return MT_OK;
+}
+
+static SeamArgs seam(Case& c, int L) {
+  DevLevel& l = c.lv[L];
+  SeamArgs a;
+  a.v = lvargs(c, L, l.pf, l.pf, l.pr, l.pr);
+  a.tau = c.tau[L];
+  a.lesC = c.d.les_C;
+  a.model = c.d.collision;
+  return a;
+}
+
+int launch_macros(Case& c, int L) {
+  SeamArgs a = seam(c, L);
+  MT_DISPATCH(c, (k_macros<Q_, S_><<<nblk(a.v.ns), 256, 0, c.stream>>>(a)));
+  MT_HIP(hipGetLastError());
+  return MT_OK;
+}
+int launch_collide(Case& c, int L) {
+  SeamArgs a = seam(c, L);
+  MT_DISPATCH(c, (k_collide<Q_, S_><<<nblk(a.v.ns), 256, 0, c.stream>>>(a)));
+  MT_HIP(hipGetLastError());
+  return MT_OK;
+}
+int launch_stream(Case& c, int L) {
+  SeamArgs a = seam(c, L);
+  MT_DISPATCH(c, (k_stream<Q_, S_><<<nblk(a.v.ns), 256, 0, c.stream>>>(a)));
+  MT_HIP(hipGetLastError());
+  return MT_OK;
+}
+static int launch_nee(Case& c) {
+  DevLevel& l = c.lv[0];
+  if (!l.n_bcres) return MT_OK;
+  SeamArgs a = seam(c, 0);
+  BcCfg bc = bccfg(c);
+  MT_DISPATCH(c, (k_nee<Q_, S_><<<nblk(l.n_bcres), 256, 0, c.stream>>>(a, bc, l.bcres, l.n_bcres, c.nee_tmp)));
+  MT_DISPATCH(c, (k_nee_scatter<Q_, S_><<<nblk(l.n_bcres), 256, 0, c.stream>>>(a, l.bcres, l.n_bcres, c.nee_tmp)));
+  MT_HIP(hipGetLastError());
+  return MT_OK;
+}
+int launch_bc(Case& c) {
+  BcCfg bc = bccfg(c);
+  bool res = false;
+  for (int f = 0; f < 6; ++f) res |= c.d.face_kind[f] == MT_BC_RESERVOIR;
+  if (res) {
+    int r = launch_nee(c);
+    if (r) return r;
+  }
+  if (bc.wet_mask) {
+    SeamArgs a = seam(c, 0);
+    MT_DISPATCH(c, (k_bc_wet<Q_, S_><<<nblk(a.v.ns), 256, 0, c.stream>>>(a, bc)));
+  }
+  MT_HIP(hipGetLastError());
+  return MT_OK;
+}
+int launch_ib(Case& c, bool acc) {
+  int L = c.Lmax;
+  DevLevel& l = c.lv[L];
+  SeamArgs a = seam(c, L);
+  int64_t nb = nblk(a.v.ns);
+  if (nb > c.force_part_cap) {
+    set_error("force partial buffer too small");
+    return MT_ERR_STATE;
+  }
+  MT_DISPATCH(c, (k_ib<Q_, S_><<<nb, 256, 0, c.stream>>>(a, l.dis_row, l.dis, c.d.ib, c.force_part)));
+  k_force_reduce<<<1, 256, 0, c.stream>>>(c.force_part, nb, c.force_acc);
+  MT_HIP(hipGetLastError());
+  return MT_OK;
+}
+
+int launch_explode(Case& c, int L) {
+  if (L <= 0) return MT_OK;
+  DevLevel& lc = c.lv[L - 1];
+  DevLevel& lf = c.lv[L];
+  XferArgs a;
+  a.ns_c = lc.n_slots;
+  a.ns_f = lf.n_slots;
+  a.src = lc.fcol[lc.pf];
+  a.dst = lf.fcol[lf.pf];
+  a.dst2 = lf.fcol[1 - lf.pf];
+  a.scale = (2.0 * c.tau[L]) / c.tau[L - 1];
+  a.rho0 = c.d.rho0;
+  a.collide = 0;
+  if (lc.n_f2c) {
+    a.list = lc.f2c;
+    a.n = lc.n_f2c;
+    a.child = lc.f2c_child_write;
+    MT_DISPATCH(c, (k_explode<Q_, S_><<<nblk(a.n), 256, 0, c.stream>>>(a, 0)));
+  }
+  if (lc.n_gf2c) {
+    a.list = lc.gf2c;
+    a.n = lc.n_gf2c;
+    a.child = lc.gf2c_child;
+    MT_DISPATCH(c, (k_explode<Q_, S_><<<nblk(a.n), 256, 0, c.stream>>>(a, 1)));
+  }
+  MT_HIP(hipGetLastError());
+  return MT_OK;
+}
+
+int launch_coalesce(Case& c, int L, bool collide_after) {
+  if (L >= c.Lmax) return MT_OK;
+  DevLevel& lc = c.lv[L];
+  DevLevel& lf = c.lv[L + 1];
+  if (!lc.n_f2c) return MT_OK;
+  XferArgs a;
+  a.ns_c = lc.n_slots;
+  a.ns_f = lf.n_slots;
+  a.src = lf.f;
+  a.dst = lc.f;
+  a.dst2 = nullptr;
+  a.list = lc.f2c;
+  a.n = lc.n_f2c;
+  a.child = lc.f2c_child;
+  a.scale = c.tau[L] / (2.0 * c.tau[L + 1]);
+  a.rho0 = c.d.rho0;
+  a.collide = collide_after ? 1 : 0;
+  a.fcol_out = lc.fcol[1 - lc.pf];
+  a.rho_out = lc.rho[1 - lc.pr];
+  a.u_out = lc.u[1 - lc.pr];
+  a.tau = c.tau[L];
+  a.lesC = c.d.les_C;
+  a.model = c.d.collision;
+  MT_DISPATCH(c, (k_coalesce<Q_, S_><<<nblk(a.n), 256, 0, c.stream>>>(a)));
+  MT_HIP(hipGetLastError());
+  return MT_OK;
+}
+
+int launch_fused(Case& c, int L, bool write_f, bool nee_after) {
+  DevLevel& l = c.lv[L];
+  FusedArgs a;
+  a.v = lvargs(c, L, l.pf, 1 - l.pf, l.pr, 1 - l.pr);
+  a.active = l.active;
+  a.n_active = l.n_active;
+  a.tau = c.tau[L];
+  a.lesC = c.d.les_C;
+  a.model = c.d.collision;
+  a.write_f = write_f ? 1 : 0;
+  a.bc = bccfg(c);
+  a.do_bc = (L == 0 && a.bc.wet_mask) ? 1 : 0;
+  a.do_ib = (c.d.ib != MT_IB_NONE && L == c.Lmax && c.Lmax > 0 && l.n_dis > 0) ? 1 : 0;
+  a.ib_method = c.d.ib;
+  a.dis_row = l.dis_row;
+  a.dis = l.dis;
+  a.Qm1 = c.Q - 1;
+  a.force_part = c.force_part;
+  int64_t nb = (l.n_active + 3) / 4;
+  if (nb < 1) nb = 1;
+  if (a.do_ib && nb > c.force_part_cap) {
+    set_error("force partial buffer too small");
+    return MT_ERR_STATE;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c.timing) {
+    if (c.ev_pool.size() < 2) {
+      for (int i = 0; i < 64; ++i) {
+        hipEvent_t e;
+        MT_HIP(hipEventCreate(&e));
+        c.ev_pool.push_back(e);
+      }
+    }
+    e0 = c.ev_pool.back();
+    c.ev_pool.pop_back();
+    e1 = c.ev_pool.back();
+    c.ev_pool.pop_back();
+    MT_HIP(hipEventRecord(e0, c.stream));
+  }
+  MT_DISPATCH(c, (k_fused<Q_, S_><<<nb, 256, 0, c.stream>>>(a)));
+  MT_HIP(hipGetLastError());
+  if (c.timing) {
+    MT_HIP(hipEventRecord(e1, c.stream));
+    c.ev_pending.push_back({L, {e0, e1}});
+  }
+  if (a.do_ib) k_force_reduce<<<1, 256, 0, c.stream>>>(c.force_part, nb, c.force_acc);
+  if (nee_after) {
+    // reservoir NEE reads the pre-sub-step maps (rho/u[pr]) and the f just written
+    int r = launch_nee(c);
+    if (r) return r;
+  }
+  return MT_OK;
+}
+
+int launch_gather(Case& c, int L, int field, const int32_t* d_slots, int64_t n, double* d_out, int buf) {
+  DevLevel& l = c.lv[L];
+  LvArgs v = lvargs(c, L, 0, 0, 0, 0);
+  const void* src = field == MT_FIELD_F ? l.f
+                    : field == MT_FIELD_FCOL ? l.fcol[buf]
+                    : field == MT_FIELD_RHO ? l.rho[buf] : l.u[buf];
+  MT_DISPATCH(c, (k_gather<Q_, S_><<<nblk(n), 256, 0, c.stream>>>(v, field, d_slots, n, d_out, src)));
+  MT_HIP(hipGetLastError());
+  return MT_OK;
+}
+int launch_scatter(Case& c, int L, int field, const int32_t* d_slots, int64_t n, const double* d_in, int buf) {
+  DevLevel& l = c.lv[L];
+  LvArgs v = lvargs(c, L, 0, 0, 0, 0);
+  void* dst = field == MT_FIELD_F ? l.f
+              : field == MT_FIELD_FCOL ? l.fcol[buf]
+              : field == MT_FIELD_RHO ? l.rho[buf] : l.u[buf];
+  MT_DISPATCH(c, (k_scatter<Q_, S_><<<nblk(n), 256, 0, c.stream>>>(v, field, d_slots, n, d_in, dst)));
+  MT_HIP(hipGetLastError());
+  return MT_OK;
+}
+int launch_reduce(Case& c, int L, double* h_out) {
+  DevLevel& l = c.lv[L];
+  LvArgs v = lvargs(c, L, 0, 0, 0, 0);
+  int64_t nb = std::min<int64_t>(nblk(l.n_slots), 1024);
+  if (nb * 5 > c.red_cap) {
+    set_error("reduction buffer too small");
+    return MT_ERR_STATE;
+  }
+  MT_DISPATCH(c, (k_reduce<Q_, S_><<<nb, 256, 0, c.stream>>>(v, c.red_tmp)));
+  std::vector<double> part(nb * 5);
+  MT_HIP(hipMemcpyAsync(part.data(), c.red_tmp, sizeof(double) * nb * 5, hipMemcpyDeviceToHost, c.stream));
+  MT_HIP(hipStreamSynchronize(c.stream));
+  for (int j = 0; j < 5; ++j) h_out[j] = 0;
+  for (int64_t i = 0; i < nb; ++i)
+    for (int j = 0; j < 5; ++j) h_out[j] += part[i * 5 + j];
+  return MT_OK;
+}
+
+}  // namespace mt

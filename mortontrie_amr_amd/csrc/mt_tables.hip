@@ -1,0 +1,527 @@
+// mt_tables.hip -- device construction of the per-level cell tables.
+//
+// The reference keeps each level's cells in associative containers keyed by
+// Morton code (header/Grid_Class.h:116-137, mortonTrieMap.cpp) and resolves
+// every neighbour with find() (lbm_kernel/Stream_Manager.cpp:53-62).  Here the
+// uploaded key sets are sorted on the GPU (LSD radix sort, 8-bit digits, stable
+// tile ranking in LDS), grouped into 4x4x4 Morton bricks, and every neighbour /
+// parent / child relation is resolved once into index tables by a binary
+// descent over the sorted brick codes.  find() semantics are preserved: a key
+// is present iff its brick exists and its slot is occupied.
+#include <algorithm>
+#include <cstring>
+
+#include "mt_case.h"
+#include "mt_device.h"
+
+namespace mt {
+
+__host__ __device__ inline uint64_t spread3(uint64_t v) {
+  v &= 0x1FFFFF;
+  v = (v | (v << 32)) & 0x1F00000000FFFFull;
+  v = (v | (v << 16)) & 0x1F0000FF0000FFull;
+  v = (v | (v << 8)) & 0x100F00F00F00F00Full;
+  v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
+__host__ __device__ inline uint64_t compact3(uint64_t v) {
+  v &= 0x1249249249249249ull;
+  v = (v ^ (v >> 2)) & 0x10C30C30C30C30C3ull;
+  v = (v ^ (v >> 4)) & 0x100F00F00F00F00Full;
+  v = (v ^ (v >> 8)) & 0x1F0000FF0000FFull;
+  v = (v ^ (v >> 16)) & 0x1F00000000FFFFull;
+  v = (v ^ (v >> 32)) & 0x1FFFFFull;
+  return v;
+}
+__host__ __device__ inline uint64_t morton3(uint64_t x, uint64_t y, uint64_t z) {
+  return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
+}
+
+// ------------------------------------------------------------------ scan
+constexpr int kScanTile = 1024;  // 256 threads x 4
+
+__device__ int block_exclusive_scan(int v, int* lds, int& total) {
+  int t = threadIdx.x;
+  lds[t] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    int x = (t >= off) ? lds[t - off] : 0;
+    __syncthreads();
+    lds[t] += x;
+    __syncthreads();
+  }
+  int incl = lds[t];
+  total = lds[255];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ void k_scan_blocksum(const int* in, int64_t n, int* bsum) {
+  __shared__ int lds[256];
+  int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * 4;
+  int s = 0;
+  for (int j = 0; j < 4; ++j)
+    if (base + j < n) s += in[base + j];
+  int tot;
+  block_exclusive_scan(s, lds, tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+// single block: exclusive scan of nb block sums, chunked
+__global__ void k_scan_bsum(int* bsum, int64_t nb, int* total) {
+  __shared__ int lds[256];
+  int64_t chunk = (nb + 255) / 256;
+  int64_t lo = threadIdx.x * chunk, hi = lo + chunk < nb ? lo + chunk : nb;
+  int s = 0;
+  for (int64_t i = lo; i < hi; ++i) s += bsum[i];
+  int tot;
+  int ex = block_exclusive_scan(s, lds, tot);
+  int run = ex;
+  for (int64_t i = lo; i < hi; ++i) {
+    int v = bsum[i];
+    bsum[i] = run;
+    run += v;
+  }
+  if (threadIdx.x == 0 && total) *total = tot;
+}
+__global__ void k_scan_apply(const int* in, int64_t n, const int* bsum, int* out) {
+  __shared__ int lds[256];
+  int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * 4;
+  int v[4], s = 0;
+  for (int j = 0; j < 4; ++j) {
+    v[j] = (base + j < n) ? in[base + j] : 0;
+    s += v[j];
+  }
+  int tot;
+  int ex = block_exclusive_scan(s, lds, tot) + bsum[blockIdx.x];
+  for (int j = 0; j < 4; ++j) {
+    if (base + j < n) out[base + j] = ex;
+    ex += v[j];
+  }
+}
+
+// exclusive scan of n ints; *h_total receives the sum
+static int scan_exclusive(Case& c, const int* d_in, int* d_out, int64_t n, int64_t* h_total) {
+  int64_t nb = (n + kScanTile - 1) / kScanTile;
+  if (nb < 1) nb = 1;
+  int* bsum = nullptr;
+  int* tot = nullptr;
+  MT_HIP(hipMalloc((void**)&bsum, sizeof(int) * (nb + 1)));
+  tot = bsum + nb;
+  k_scan_blocksum<<<nb, 256, 0, c.stream>>>(d_in, n, bsum);
+  k_scan_bsum<<<1, 256, 0, c.stream>>>(bsum, nb, tot);
+  k_scan_apply<<<nb, 256, 0, c.stream>>>(d_in, n, bsum, d_out);
+  int t = 0;
+  MT_HIP(hipMemcpyAsync(&t, tot, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  MT_HIP(hipStreamSynchronize(c.stream));
+  MT_HIP(hipFree(bsum));
+  *h_total = t;
+  return MT_OK;
+}
+
+// ------------------------------------------------------------------ radix sort
+constexpr int kRsTile = 256;
+
+__global__ void k_rs_hist(const uint64_t* keys, int64_t n, int shift, int64_t n_tiles, int* hist) {
+  __shared__ int h[256];
+  int t = threadIdx.x;
+  h[t] = 0;
+  __syncthreads();
+  int64_t i = (int64_t)blockIdx.x * kRsTile + t;
+  if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1);
+  __syncthreads();
+  hist[(int64_t)t * n_tiles + blockIdx.x] = h[t];
+}
+__global__ void k_rs_scatter(const uint64_t* kin, const int32_t* vin, uint64_t* kout, int32_t* vout, int64_t n,
+                             int shift, int64_t n_tiles, const int* offs) {
+  __shared__ int dg[kRsTile];
+  int t = threadIdx.x;
+  int64_t i = (int64_t)blockIdx.x * kRsTile + t;
+  uint64_t k = 0;
+  int d = 256;
+  if (i < n) {
+    k = kin[i];
+    d = (int)((k >> shift) & 255);
+  }
+  dg[t] = d;
+  __syncthreads();
+  if (i < n) {
+    int rank = 0;
+    for (int j = 0; j < t; ++j) rank += (dg[j] == d);
+    int64_t pos = offs[(int64_t)d * n_tiles + blockIdx.x] + rank;
+    kout[pos] = k;
+    vout[pos] = vin[i];
+  }
+}
+__global__ void k_iota(int32_t* v, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (int32_t)i;
+}
+
+// sorts (keys, payload = original index) ascending; results in *ks, *vs
+static int radix_sort(Case& c, const uint64_t* d_keys, int64_t n, uint64_t max_key, uint64_t** ks, int32_t** vs) {
+  uint64_t *ka = nullptr, *kb = nullptr;
+  int32_t *va = nullptr, *vb = nullptr;
+  int* hist = nullptr;
+  int64_t n_tiles = (n + kRsTile - 1) / kRsTile;
+  if (n_tiles < 1) n_tiles = 1;
+  MT_HIP(hipMalloc((void**)&ka, 8 * (n + 1)));
+  MT_HIP(hipMalloc((void**)&kb, 8 * (n + 1)));
+  MT_HIP(hipMalloc((void**)&va, 4 * (n + 1)));
+  MT_HIP(hipMalloc((void**)&vb, 4 * (n + 1)));
+  MT_HIP(hipMalloc((void**)&hist, sizeof(int) * 256 * n_tiles));
+  MT_HIP(hipMemcpyAsync(ka, d_keys, 8 * n, hipMemcpyDeviceToDevice, c.stream));
+  k_iota<<<(n + 255) / 256 + 1, 256, 0, c.stream>>>(va, n);
+  int bits = 0;
+  while (bits < 64 && (max_key >> bits)) ++bits;
+  int passes = (bits + 7) / 8;
+  for (int p = 0; p < passes; ++p) {
+    int shift = 8 * p;
+    k_rs_hist<<<n_tiles, kRsTile, 0, c.stream>>>(ka, n, shift, n_tiles, hist);
+    int64_t tot;
+    int r = scan_exclusive(c, hist, hist, 256 * n_tiles, &tot);
+    if (r) return r;
+    k_rs_scatter<<<n_tiles, kRsTile, 0, c.stream>>>(ka, va, kb, vb, n, shift, n_tiles, hist);
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  MT_HIP(hipStreamSynchronize(c.stream));
+  MT_HIP(hipFree(kb));
+  MT_HIP(hipFree(vb));
+  MT_HIP(hipFree(hist));
+  *ks = ka;
+  *vs = va;
+  return MT_OK;
+}
+
+// ------------------------------------------------------------------ build
+__device__ inline int64_t bsearch_u64(const uint64_t* a, int64_t n, uint64_t x) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return (lo < n && a[lo] == x) ? lo : -1;
+}
+
+struct LevelView {
+  const uint64_t* bcode;
+  const uint16_t* kind;
+  int64_t n_bricks;
+  int shift;  // 3*(Lmax-L)
+};
+// find(): slot of a reference key on a level, -1 if absent
+__device__ inline int32_t find_slot(const LevelView& v, uint64_t key) {
+  if (v.shift && (key & ((1ull << v.shift) - 1))) return -1;
+  uint64_t code = key >> v.shift;
+  int64_t b = bsearch_u64(v.bcode, v.n_bricks, code >> 6);
+  if (b < 0) return -1;
+  int32_t s = (int32_t)((b << 6) | (code & 63));
+  return v.kind[s] ? s : -1;
+}
+
+__global__ void k_codes(const uint64_t* ks, int64_t n, int shift, int* head, int* err) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t k = ks[i];
+  if (shift && (k & ((1ull << shift) - 1))) atomicOr(err, 1);  // not a cell of this level
+  if (i > 0 && ks[i - 1] == k) atomicOr(err, 2);                // duplicate key
+  uint64_t code = k >> shift;
+  head[i] = (i == 0) || ((ks[i - 1] >> shift) >> 6) != (code >> 6);
+}
+__global__ void k_fill_empty(uint64_t* key, uint16_t* kind, int64_t ns) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  key[s] = ~0ull;
+  kind[s] = 0;
+}
+__global__ void k_place(const uint64_t* ks, const int32_t* vs, const uint16_t* kinds_in, const int* head,
+                        const int* bidx, int64_t n, int shift, uint64_t* bcode, uint64_t* key, uint16_t* kind) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t k = ks[i];
+  uint64_t code = k >> shift;
+  int64_t b = bidx[i] + head[i] - 1;
+  int64_t s = (b << 6) | (int64_t)(code & 63);
+  key[s] = k;
+  kind[s] = kinds_in[vs[i]];
+  if (head[i]) bcode[b] = code >> 6;
+}
+__global__ void k_bnbr(const uint64_t* bcode, int64_t nb, int bx_n, int by_n, int bz_n, int px, int py, int pz,
+                       int32_t* bnbr) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  uint64_t c = bcode[b];
+  int64_t x = (int64_t)compact3(c), y = (int64_t)compact3(c >> 1), z = (int64_t)compact3(c >> 2);
+  for (int o = 0; o < 27; ++o) {
+    int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
+    int64_t X = x + dx, Y = y + dy, Z = z + dz;
+    if (px) X = (X + bx_n) % bx_n;
+    if (py) Y = (Y + by_n) % by_n;
+    if (pz) Z = (Z + bz_n) % bz_n;
+    int32_t r = -1;
+    if (X >= 0 && Y >= 0 && Z >= 0) {
+      int64_t f = bsearch_u64(bcode, nb, morton3(X, Y, Z));
+      r = (int32_t)f;
+    }
+    bnbr[b * 27 + o] = r;
+  }
+}
+__global__ void k_pmask(const int32_t* bnbr, const uint16_t* kind, int64_t ns, int Q, uint32_t* pmask) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  if (!kind[s]) {
+    pmask[s] = 0;
+    return;
+  }
+  int b = (int)(s >> 6);
+  int lx, ly, lz;
+  brick_coords((int)(s & 63), lx, ly, lz);
+  const int* nb27 = bnbr + (int64_t)b * 27;
+  uint32_t m = 0;
+  for (int q = 1; q < Q; ++q) {
+    int t = slot_at(nb27, lx, ly, lz, kEX[q], kEY[q], kEZ[q]);
+    if (t >= 0 && kind[t]) m |= 1u << q;
+  }
+  pmask[s] = m;
+}
+__global__ void k_flag_kind(const uint16_t* kind, int64_t ns, uint16_t mask, int* flag) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  flag[s] = (kind[s] & mask) != 0;
+}
+__global__ void k_flag_brick(const uint16_t* kind, int64_t nb, uint16_t mask, int* flag) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  int any = 0;
+  for (int l = 0; l < 64; ++l) any |= (kind[b * 64 + l] & mask) != 0;
+  flag[b] = any;
+}
+__global__ void k_emit(const int* flag, const int* pos, int64_t n, int32_t* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (flag[i]) out[pos[i]] = (int32_t)i;
+}
+__global__ void k_find(LevelView v, const uint64_t* keys, int64_t n, int32_t* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = find_slot(v, keys[i]);
+}
+// children of F2C / ghost_F2C parents (child order of LBM_Manager.cpp:1074-1082)
+__global__ void k_children(const int32_t* parents, int64_t n, const uint64_t* pkey, int pshift, LevelView fine,
+                           const uint16_t* fkind, const uint64_t* dis_keys, int64_t n_dis, int ghost_only,
+                           int32_t* child_all, int32_t* child_write, int* err) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t k = pkey[parents[i]];
+  uint64_t px = compact3(k) >> pshift, py = compact3(k >> 1) >> pshift, pz = compact3(k >> 2) >> pshift;
+  for (int j = 0; j < 8; ++j) {
+    uint64_t cx = 2 * px + (j & 1), cy = 2 * py + ((j >> 1) & 1), cz = 2 * pz + ((j >> 2) & 1);
+    uint64_t ck = morton3(cx << fine.shift / 3, cy << fine.shift / 3, cz << fine.shift / 3);
+    int32_t s = find_slot(fine, ck);
+    bool in_dis = n_dis > 0 && bsearch_u64(dis_keys, n_dis, ck) >= 0;
+    if (!ghost_only) {
+      if (s < 0) atomicOr(err, 4);  // coalescence reads every child (LBM_Manager.cpp:1155)
+      child_all[i * 8 + j] = s;
+      child_write[i * 8 + j] = (s >= 0 && !in_dis) ? s : -1;
+    } else {
+      bool g = s >= 0 && (fkind[s] & MT_KIND_GHOST_C2F);
+      child_write[i * 8 + j] = (g && !in_dis) ? s : -1;
+    }
+  }
+}
+__global__ void k_dis_rows(const uint64_t* ks, const int32_t* vs, int64_t n, LevelView v, const double* vals_in,
+                           int m, int32_t* dis_row, double* vals_out, uint16_t* kind) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t s = find_slot(v, ks[i]);
+  for (int j = 0; j < m; ++j) vals_out[i * m + j] = vals_in[(int64_t)vs[i] * m + j];
+  if (s >= 0) {
+    dis_row[s] = (int32_t)i;
+    kind[s] |= MT_KIND_DIS;
+  }
+}
+__global__ void k_fill_i32(int32_t* a, int64_t n, int32_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+static inline int64_t nblk(int64_t n, int b = 256) { return n <= 0 ? 1 : (n + b - 1) / b; }
+
+static int compact_list(Case& c, const int* d_flag, int64_t n, int32_t** out, int64_t* count) {
+  int* pos = nullptr;
+  MT_HIP(hipMalloc((void**)&pos, sizeof(int) * (n + 1)));
+  int64_t tot;
+  int r = scan_exclusive(c, d_flag, pos, n, &tot);
+  if (r) return r;
+  *out = c.alloc<int32_t>(tot);
+  if (!*out) { set_error("hipMalloc failed"); return MT_ERR_HIP; }
+  k_emit<<<nblk(n), 256, 0, c.stream>>>(d_flag, pos, n, *out);
+  MT_HIP(hipStreamSynchronize(c.stream));
+  MT_HIP(hipFree(pos));
+  *count = tot;
+  return MT_OK;
+}
+
+static LevelView view(const Case& c, int L) {
+  LevelView v;
+  v.bcode = c.lv[L].bcode;
+  v.kind = c.lv[L].kind;
+  v.n_bricks = c.lv[L].n_bricks;
+  v.shift = 3 * (c.Lmax - L);
+  return v;
+}
+
+int find_slots(Case& c, int L, const uint64_t* d_keys, int64_t n, int32_t* d_out) {
+  if (n <= 0) return MT_OK;
+  k_find<<<nblk(n), 256, 0, c.stream>>>(view(c, L), d_keys, n, d_out);
+  MT_HIP(hipGetLastError());
+  return MT_OK;
+}
+
+int build_tables(Case& c) {
+  int* d_err = c.alloc<int>(1);
+  MT_HIP(hipMemsetAsync(d_err, 0, sizeof(int), c.stream));
+  for (int L = 0; L <= c.Lmax; ++L) {
+    DevLevel& l = c.lv[L];
+    int64_t n = (int64_t)l.h_keys.size();
+    if (n == 0) { set_error("level " + std::to_string(L) + " has no cells"); return MT_ERR_ARG; }
+    for (int a = 0; a < 3; ++a) l.dims[a] = c.d.dims0[a] << L;
+    uint64_t *d_keys = nullptr, *ks = nullptr;
+    uint16_t* d_kinds = nullptr;
+    int32_t* vs = nullptr;
+    MT_HIP(hipMalloc((void**)&d_keys, 8 * n));
+    MT_HIP(hipMalloc((void**)&d_kinds, 2 * n));
+    MT_HIP(hipMemcpyAsync(d_keys, l.h_keys.data(), 8 * n, hipMemcpyHostToDevice, c.stream));
+    MT_HIP(hipMemcpyAsync(d_kinds, l.h_kinds.data(), 2 * n, hipMemcpyHostToDevice, c.stream));
+    uint64_t mx = *std::max_element(l.h_keys.begin(), l.h_keys.end());
+    int r = radix_sort(c, d_keys, n, mx, &ks, &vs);
+    if (r) return r;
+    int shift = 3 * (c.Lmax - L);
+    int *head = nullptr, *bidx = nullptr;
+    MT_HIP(hipMalloc((void**)&head, 4 * n));
+    MT_HIP(hipMalloc((void**)&bidx, 4 * n));
+    k_codes<<<nblk(n), 256, 0, c.stream>>>(ks, n, shift, head, d_err);
+    int64_t nb;
+    r = scan_exclusive(c, head, bidx, n, &nb);
+    if (r) return r;
+    int herr = 0;
+    MT_HIP(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, c.stream));
+    MT_HIP(hipStreamSynchronize(c.stream));
+    if (herr) {
+      set_error("level " + std::to_string(L) + ": " +
+                ((herr & 2) ? "duplicate key in the uploaded set" : "key is not a cell of this level"));
+      return MT_ERR_ARG;
+    }
+    l.n_cells = n;
+    l.n_bricks = nb;
+    l.n_slots = nb * 64;
+    l.bcode = c.alloc<uint64_t>(nb);
+    l.bnbr = c.alloc<int32_t>(nb * 27);
+    l.key = c.alloc<uint64_t>(l.n_slots);
+    l.kind = c.alloc<uint16_t>(l.n_slots);
+    l.pmask = c.alloc<uint32_t>(l.n_slots);
+    l.dis_row = c.alloc<int32_t>(l.n_slots);
+    if (!l.bcode || !l.bnbr || !l.key || !l.kind || !l.pmask || !l.dis_row) {
+      set_error("hipMalloc failed for level tables");
+      return MT_ERR_HIP;
+    }
+    k_fill_empty<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.key, l.kind, l.n_slots);
+    k_fill_i32<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.dis_row, l.n_slots, -1);
+    k_place<<<nblk(n), 256, 0, c.stream>>>(ks, vs, d_kinds, head, bidx, n, shift, l.bcode, l.key, l.kind);
+    int per[3] = {c.d.face_kind[0] == MT_BC_PERIODIC, c.d.face_kind[2] == MT_BC_PERIODIC,
+                  c.d.face_kind[4] == MT_BC_PERIODIC};
+    for (int a = 0; a < 3; ++a)
+      if (per[a] && (l.dims[a] % 4)) {
+        set_error("periodic axis extent must be a multiple of 4 on every level");
+        return MT_ERR_ARG;
+      }
+    k_bnbr<<<nblk(nb), 256, 0, c.stream>>>(l.bcode, nb, (l.dims[0] + 3) / 4, (l.dims[1] + 3) / 4,
+                                           (l.dims[2] + 3) / 4, per[0], per[1], per[2], l.bnbr);
+    k_pmask<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.bnbr, l.kind, l.n_slots, c.Q, l.pmask);
+    MT_HIP(hipGetLastError());
+    MT_HIP(hipStreamSynchronize(c.stream));
+    MT_HIP(hipFree(d_keys));
+    MT_HIP(hipFree(d_kinds));
+    MT_HIP(hipFree(ks));
+    MT_HIP(hipFree(vs));
+    MT_HIP(hipFree(head));
+    MT_HIP(hipFree(bidx));
+  }
+  // dis (finest level)
+  int Lm = c.Lmax;
+  DevLevel& lm = c.lv[Lm];
+  uint64_t* d_dis_sorted = nullptr;
+  int64_t nd = (int64_t)c.h_dis_keys.size();
+  if (nd > 0) {
+    int m = c.Q - 1;
+    uint64_t *d_k = nullptr, *ks = nullptr;
+    int32_t* vs = nullptr;
+    double* d_v = nullptr;
+    MT_HIP(hipMalloc((void**)&d_k, 8 * nd));
+    MT_HIP(hipMalloc((void**)&d_v, 8 * nd * m));
+    MT_HIP(hipMemcpyAsync(d_k, c.h_dis_keys.data(), 8 * nd, hipMemcpyHostToDevice, c.stream));
+    MT_HIP(hipMemcpyAsync(d_v, c.h_dis_vals.data(), 8 * nd * m, hipMemcpyHostToDevice, c.stream));
+    uint64_t mx = *std::max_element(c.h_dis_keys.begin(), c.h_dis_keys.end());
+    int r = radix_sort(c, d_k, nd, mx, &ks, &vs);
+    if (r) return r;
+    lm.dis = c.alloc<double>(nd * m);
+    lm.n_dis = nd;
+    k_dis_rows<<<nblk(nd), 256, 0, c.stream>>>(ks, vs, nd, view(c, Lm), d_v, m, lm.dis_row, lm.dis, lm.kind);
+    d_dis_sorted = c.alloc<uint64_t>(nd);
+    MT_HIP(hipMemcpyAsync(d_dis_sorted, ks, 8 * nd, hipMemcpyDeviceToDevice, c.stream));
+    MT_HIP(hipStreamSynchronize(c.stream));
+    MT_HIP(hipFree(d_k));
+    MT_HIP(hipFree(d_v));
+    MT_HIP(hipFree(ks));
+    MT_HIP(hipFree(vs));
+  }
+  // lists
+  for (int L = 0; L <= c.Lmax; ++L) {
+    DevLevel& l = c.lv[L];
+    int* flag = nullptr;
+    int64_t nmax = std::max(l.n_slots, l.n_bricks);
+    MT_HIP(hipMalloc((void**)&flag, 4 * (nmax + 1)));
+    k_flag_brick<<<nblk(l.n_bricks), 256, 0, c.stream>>>(l.kind, l.n_bricks, MT_KIND_FLUID | MT_KIND_C2F, flag);
+    int r = compact_list(c, flag, l.n_bricks, &l.active, &l.n_active);
+    if (r) return r;
+    k_flag_kind<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.kind, l.n_slots, MT_KIND_F2C, flag);
+    r = compact_list(c, flag, l.n_slots, &l.f2c, &l.n_f2c);
+    if (r) return r;
+    k_flag_kind<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.kind, l.n_slots, MT_KIND_GHOST_F2C, flag);
+    r = compact_list(c, flag, l.n_slots, &l.gf2c, &l.n_gf2c);
+    if (r) return r;
+    k_flag_kind<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.kind, l.n_slots, MT_KIND_BCRES, flag);
+    r = compact_list(c, flag, l.n_slots, &l.bcres, &l.n_bcres);
+    if (r) return r;
+    MT_HIP(hipStreamSynchronize(c.stream));
+    MT_HIP(hipFree(flag));
+    if (L < c.Lmax && (l.n_f2c || l.n_gf2c)) {
+      l.f2c_child = c.alloc<int32_t>(l.n_f2c * 8);
+      l.f2c_child_write = c.alloc<int32_t>(l.n_f2c * 8);
+      l.gf2c_child = c.alloc<int32_t>(l.n_gf2c * 8);
+      LevelView fine = view(c, L + 1);
+      int pshift = c.Lmax - L;
+      if (l.n_f2c)
+        k_children<<<nblk(l.n_f2c), 256, 0, c.stream>>>(l.f2c, l.n_f2c, l.key, pshift, fine, c.lv[L + 1].kind,
+                                                        d_dis_sorted, nd, 0, l.f2c_child, l.f2c_child_write, d_err);
+      if (l.n_gf2c)
+        k_children<<<nblk(l.n_gf2c), 256, 0, c.stream>>>(l.gf2c, l.n_gf2c, l.key, pshift, fine,
+                                                         c.lv[L + 1].kind, d_dis_sorted, nd, 1, nullptr,
+                                                         l.gf2c_child, d_err);
+    } else if (L == c.Lmax && l.n_f2c) {
+      set_error("F2C cells on the finest level");
+      return MT_ERR_ARG;
+    }
+  }
+  int herr = 0;
+  MT_HIP(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, c.stream));
+  MT_HIP(hipStreamSynchronize(c.stream));
+  if (herr & 4) {
+    set_error("an F2C cell has a child missing on the finer level");
+    return MT_ERR_LOOKUP;
+  }
+  return MT_OK;
+}
+
+}  // namespace mt
