@@ -1,0 +1,149 @@
+"""Benchmark of the per-time-step AMR LBM update on one MI355X.
+
+Workload: flow past a sphere on a 3-level AMR grid, D3Q27 cumulant collision,
+FP64 (coarse 64x32x32, sphere radius 8 coarse cells at (16,16,16), refinement
+shells within 6 and 3 coarse cells of the surface, Zou-He inflow u_x=0.04,
+outflow at x-max, periodic y/z, Re=100, link IB with Filippova-Haenel
+interpolation on the finest level).  One step is one coarse time step of the
+recursive sub-cycling schedule (level L updated 2^L times).
+
+Prints one JSON line: MLUPS over the timed region (inputs resident in HBM),
+the HBM roofline of the dominant kernel (finest-level fused update) from HIP
+events, and the map-faithful CPU oracle timed on a bounded sample of the same
+workload as the baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "MLUPS & % HBM roofline (FP64 vs FP32-shifted); served cases/s + p50/p99 latency"
+
+
+def bytes_per_update(Q, precision):
+    b = {0: 8, 1: 4, 2: 2}[precision]
+    return 2 * Q * b
+
+
+def level_updates(case):
+    from mortontrie_amr_amd import cases as C
+    return [c["fluid"] + c["c2f"] for c in C.level_counts(case)]
+
+
+def run_gpu(case, steps, warmup, precision, timing_steps):
+    from mortontrie_amr_amd import amr
+    g = amr.AmrCase(case, precision=precision, graphs=True)
+    g.step(warmup)
+    g.sync()
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.sync()
+    t1 = time.perf_counter()
+    # kernel durations from HIP events around every fused launch, same stream
+    g.set_timing(True)
+    g.step(timing_steps)
+    ms, calls = g.timing()
+    g.set_timing(False)
+    m, p, nan = g.reduce(0)
+    g.close()
+    return (t1 - t0), ms, calls, nan
+
+
+def cpu_baseline(case, steps):
+    from oracle import pyoracle as O
+    o = O.OracleCase(case)
+    t0 = time.perf_counter()
+    o.step(steps)
+    t1 = time.perf_counter()
+    return t1 - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--timing-steps", type=int, default=50)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the update runs on a "
+              "single MI355X; multi-GPU sharding is a design note, not a build target)", file=sys.stderr)
+        sys.exit(2)
+
+    from mortontrie_amr_amd import cases as C, amr
+    t_build = time.perf_counter()
+    case = C.sphere_amr()
+    t_build = time.perf_counter() - t_build
+    per_level = level_updates(case)
+    upd_step = sum(n << L for L, n in enumerate(per_level))
+    Q = case["Q"]
+
+    secs, ms, calls, nan = run_gpu(case, args.steps, args.warmup, amr.FP64, args.timing_steps)
+    mlups = upd_step * args.steps / secs / 1e6
+    Lf = case["Lmax"]
+    avg_ms = ms[Lf] / max(calls[Lf], 1)
+    bytes_launch = per_level[Lf] * bytes_per_update(Q, amr.FP64)
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(secs / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "sphere_amr_3level_d3q27_cumulant_fp64",
+            "coarse": list(case["dims0"]), "sphere_radius": 8, "levels": Lf + 1,
+            "updates_per_level": per_level, "updates_per_coarse_step": upd_step,
+            "tau0": case["tau0"], "parallelism": "single_gpu",
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": f"k_fused<27,double> level {Lf}", "bytes_per_launch": bytes_launch,
+            "avg_launch_ms": round(avg_ms, 5), "launches_timed": int(calls[Lf]),
+            "per_level_avg_ms": [round(ms[L] / max(calls[L], 1), 5) for L in range(Lf + 1)],
+        },
+        "nan_count_level0": nan,
+    }
+    if not args.no_variants:
+        var = {}
+        for name, prec in (("fp32_shifted", amr.FP32_SHIFTED), ("fp16_storage", amr.FP16_STORAGE)):
+            s2, ms2, c2, _ = run_gpu(case, max(args.steps // 2, 10), args.warmup, prec, args.timing_steps)
+            a2 = ms2[Lf] / max(c2[Lf], 1)
+            bl = per_level[Lf] * bytes_per_update(Q, prec)
+            var[name] = {"MLUPS": round(upd_step * max(args.steps // 2, 10) / s2 / 1e6, 2),
+                         "finest_kernel_GBs": round(bl / (a2 * 1e-3) / 1e9, 1),
+                         "finest_kernel_ms": round(a2, 5)}
+        out["variants"] = var
+    if not args.no_cpu:
+        n = args.cpu_steps
+        cs = cpu_baseline(case, n)
+        out["cpu_baseline"] = {
+            "value": round(upd_step * n / cs / 1e6, 4), "unit": "MLUPS", "cores": 1, "kind": "port",
+            "sample": f"{n} coarse steps of the same workload, map-faithful oracle (one unordered_map per "
+                      f"population), single thread, {cs:.1f} s",
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

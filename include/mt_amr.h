@@ -79,7 +79,19 @@ typedef struct mt_case_desc {
   double les_C;       /* Smagorinsky constant, 0 = off                            */
   double rho0;        /* initial density and He-Luo reference density             */
   double u0[3];       /* initial velocity                                         */
+  int32_t options;    /* MT_OPT_* bit flags, 0 = reference semantics throughout   */
 } mt_case_desc;
+
+/* Optional departures from the reference semantics (off by default):
+ * MT_OPT_CONVECTIVE_XFER: rescale f_neq by tau_f/(2 tau_c) coarse->fine and
+ *   2 tau_c/tau_f fine->coarse (the convective-scaling ratio of the
+ *   non-equilibrium parts) instead of the reference's 2 tau_f/tau_c and
+ *   tau_c/(2 tau_f) (LBM_Manager.cpp:1065, 1137).                                */
+#define MT_OPT_CONVECTIVE_XFER 0x1
+/* MT_OPT_EXPLODE_BOTH: also refill the C2F / ghost_C2F layer from the coarse
+ *   level in sub-step B (the reference explodes in sub-step A only,
+ *   LBM_Simulator.cpp:47-51 vs 152-217).                                        */
+#define MT_OPT_EXPLODE_BOTH 0x2
 
 typedef struct mt_case mt_case;
 

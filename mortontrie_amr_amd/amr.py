@@ -33,7 +33,7 @@ class CaseDesc(C.Structure):
         ("Q", C.c_int32), ("Lmax", C.c_int32), ("collision", C.c_int32), ("ib", C.c_int32),
         ("precision", C.c_int32), ("dims0", C.c_int32 * 3), ("face_kind", C.c_int32 * 6),
         ("face_u", (C.c_double * 3) * 6), ("tau0", C.c_double), ("les_C", C.c_double),
-        ("rho0", C.c_double), ("u0", C.c_double * 3),
+        ("rho0", C.c_double), ("u0", C.c_double * 3), ("options", C.c_int32),
     ]
 
 
@@ -112,6 +112,7 @@ class AmrCase:
             for a in range(3):
                 d.face_u[f][a] = case["faces"][f][1 + a]
         d.tau0, d.les_C, d.rho0 = case["tau0"], case["les_C"], case["rho0"]
+        d.options = case.get("options", 0)
         h = C.c_void_p()
         _check(L.mt_case_create(C.byref(d), C.byref(h)))
         self.h = h

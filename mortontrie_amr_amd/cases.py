@@ -62,6 +62,17 @@ EZ27 = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1,
                  1, -1, 1, -1, 1, -1, 1, -1], dtype=np.int64)
 
 
+def reported(u_physical):
+    """Velocity in the reference's reported frame.
+
+    The reference streams by pulling f_q(x) <- fcol_q(x + e_q)
+    (lbm_kernel/Stream_Manager.cpp:53-62), so population q travels along -e_q and
+    the velocity its macros report, sum_q f_q e_q / rho (LBM_Manager.cpp:983-1001),
+    is minus the physical velocity.  Boundary and initial velocities of the
+    workloads are physical (flow enters at an inflow face); this converts them."""
+    return -u_physical
+
+
 def velocities(Q):
     if Q not in (19, 27):
         raise ValueError("Q must be 19 or 27")
@@ -297,14 +308,14 @@ def build_nested(dims0, Lmax, refine, solid=None, faces=None, Q=19, ib_dist=None
 
 
 def _base_case(Q, Lmax, tau0, collision, dims0, faces, levels, dis_keys, dis_vals,
-               u0=(0.0, 0.0, 0.0), rho0=1.0, ib=IB_NONE, les_C=0.0, cd=None, name=""):
+               u0=(0.0, 0.0, 0.0), rho0=1.0, ib=IB_NONE, les_C=0.0, cd=None, name="", options=0):
     return {
         "name": name, "Q": Q, "Lmax": Lmax, "tau0": float(tau0),
         "collision": collision, "les_C": float(les_C), "rho0": float(rho0),
         "u0": tuple(float(v) for v in u0), "dims0": tuple(int(d) for d in dims0),
         "faces": [tuple(f) for f in faces], "levels": levels,
         "dis_keys": dis_keys, "dis_vals": np.ascontiguousarray(dis_vals, dtype=np.float64),
-        "ib": ib, "cd": cd or {},
+        "ib": ib, "cd": cd or {}, "options": int(options),
     }
 
 
@@ -316,9 +327,9 @@ def taus(case):
 # ---- workloads ---------------------------------------------------------------
 def cavity(n=64, Q=19, tau=0.6, lid=0.05, collision=COLLIDE_LBGK, refine_box=None, name=None):
     """Lid-driven cavity: n^3 wet-node cells, Zou-He no-slip on five faces and a
-    Zou-He velocity lid (u_x=lid) on z-max; f0 = feq(1, 0).  With refine_box =
-    ((x0,y0,z0),(x1,y1,z1)) in coarse cells, that box is refined once."""
-    faces = [(BC_ZOUHE, 0.0, 0.0, 0.0)] * 5 + [(BC_ZOUHE, float(lid), 0.0, 0.0)]
+    Zou-He velocity lid (physical u_x=lid) on z-max; f0 = feq(1, 0).  With
+    refine_box = ((x0,y0,z0),(x1,y1,z1)) in coarse cells, that box is refined once."""
+    faces = [(BC_ZOUHE, 0.0, 0.0, 0.0)] * 5 + [(BC_ZOUHE, reported(float(lid)), 0.0, 0.0)]
     dims0 = (n, n, n) if np.isscalar(n) else tuple(n)
     Lmax = 0 if refine_box is None else 1
 
@@ -362,17 +373,20 @@ def sphere_ib_dist(center, radius):
 
 def sphere_amr(coarse=(64, 32, 32), radius=8.0, center=(16.0, 16.0, 16.0), shells=(6.0, 3.0),
                Q=27, collision=COLLIDE_CUMULANT, U=0.04, Re=100.0, ib=IB_FH, les_C=0.0,
-               periodic_yz=True, name=None):
+               periodic_yz=True, name=None, keep_solid=True, with_sphere=True, options=0, periodic_x=False):
     """Refined sphere in a channel: Zou-He inflow u_x=U at x=0, outflow at x-max,
     periodic y and z; level L+1 covers coarse cells whose centre lies within
     radius+shells[L] of the sphere centre; cells inside the sphere are solid on the
     finest level and the cut links carry analytic fractions.  Re = U*2R/nu0."""
     Lmax = len(shells)
-    nu0 = U * 2.0 * radius / Re
+    nu0 = abs(U) * 2.0 * radius / Re
     tau0 = 3.0 * nu0 + 0.5
     C = np.asarray(center, dtype=np.float64)
     pk = BC_PERIODIC if periodic_yz else BC_ZOUHE
-    faces = [(BC_ZOUHE, U, 0.0, 0.0), (BC_OUTFLOW, 0.0, 0.0, 0.0)] + [(pk, 0.0, 0.0, 0.0)] * 4
+    ur = reported(U)
+    faces = [(BC_ZOUHE, ur, 0.0, 0.0), (BC_OUTFLOW, 0.0, 0.0, 0.0)] + [(pk, 0.0, 0.0, 0.0)] * 4
+    if periodic_x:
+        faces[0] = faces[1] = (BC_PERIODIC, 0.0, 0.0, 0.0)
 
     def r_of(cx, cy, cz):
         return np.sqrt((cx - C[0]) ** 2 + (cy - C[1]) ** 2 + (cz - C[2]) ** 2)
@@ -383,12 +397,18 @@ def sphere_amr(coarse=(64, 32, 32), radius=8.0, center=(16.0, 16.0, 16.0), shell
     def solid(cx, cy, cz):
         return r_of(cx, cy, cz) < radius
 
-    levels, dk, dv = build_nested(coarse, Lmax, refine, solid=solid, faces=faces, Q=Q,
-                                  ib_dist=sphere_ib_dist(C, radius))
+    levels, dk, dv = build_nested(coarse, Lmax, refine, solid=solid if with_sphere else None, faces=faces, Q=Q,
+                                  ib_dist=sphere_ib_dist(C, radius) if with_sphere else None)
+    if not keep_solid:
+        top = levels[Lmax]
+        keep = (top["kinds"] & KIND_SOLID) == 0
+        top["keys"], top["kinds"] = top["keys"][keep], top["kinds"][keep]
+    if not with_sphere:
+        ib = IB_NONE
     d_lb = 2.0 * radius * (1 << Lmax)
     return _base_case(Q, Lmax, tau0, collision, coarse, faces, levels, dk, dv,
-                      u0=(U, 0.0, 0.0), ib=ib, les_C=les_C,
-                      cd={"U": U, "d": d_lb}, name=name or "sphere_amr")
+                      u0=(ur, 0.0, 0.0), ib=ib, les_C=les_C,
+                      cd={"U": ur, "d": d_lb}, name=name or "sphere_amr", options=options)
 
 
 def level_counts(case):

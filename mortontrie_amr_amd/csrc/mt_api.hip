@@ -70,6 +70,7 @@ static int run_level(Case& c, int L, bool last) {
   if (L == 0) return MT_OK;
   // sub-step B: no explosion (LBM_Simulator.cpp:152-217)
   if (L < c.Lmax) TRY(run_level(c, L + 1, last));
+  if (c.d.options & MT_OPT_EXPLODE_BOTH) TRY(launch_explode(c, L));
   TRY(launch_fused(c, L, last, false));
   if (L < c.Lmax) TRY(launch_coalesce(c, L, true));
   l.pf ^= 1;

@@ -895,7 +895,8 @@ int launch_explode(Case& c, int L) {
   a.src = lc.fcol[lc.pf];
   a.dst = lf.fcol[lf.pf];
   a.dst2 = lf.fcol[1 - lf.pf];
-  a.scale = (2.0 * c.tau[L]) / c.tau[L - 1];
+  a.scale = (c.d.options & MT_OPT_CONVECTIVE_XFER) ? c.tau[L] / (2.0 * c.tau[L - 1])
+                                                   : (2.0 * c.tau[L]) / c.tau[L - 1];
   a.rho0 = c.d.rho0;
   a.collide = 0;
   if (lc.n_f2c) {
@@ -928,7 +929,8 @@ int launch_coalesce(Case& c, int L, bool collide_after) {
   a.list = lc.f2c;
   a.n = lc.n_f2c;
   a.child = lc.f2c_child;
-  a.scale = c.tau[L] / (2.0 * c.tau[L + 1]);
+  a.scale = (c.d.options & MT_OPT_CONVECTIVE_XFER) ? (2.0 * c.tau[L]) / c.tau[L + 1]
+                                                   : c.tau[L] / (2.0 * c.tau[L + 1]);
   a.rho0 = c.d.rho0;
   a.collide = collide_after ? 1 : 0;
   a.fcol_out = lc.fcol[1 - lc.pf];

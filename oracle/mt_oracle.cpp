@@ -322,7 +322,7 @@ struct Level {
 };
 
 struct Case {
-  int Q = 19, Lmax = 0, collision = 0, ib = 0, nee_seq = 1;
+  int Q = 19, Lmax = 0, collision = 0, ib = 0, nee_seq = 1, options = 0;
   double tau0 = 0.6, les_C = 0, rho0 = 1;
   double u0[3] = {0, 0, 0};
   int dims0[3] = {0, 0, 0};
@@ -469,7 +469,7 @@ struct Case {
   // AMR_transDDF_explosion (LBM_Manager.cpp:1059-1129)
   void explode(int fine) {
     int coarse = fine - 1;
-    double s = (2.0 * tau[fine]) / tau[coarse];
+    double s = (options & 1) ? tau[fine] / (2.0 * tau[coarse]) : (2.0 * tau[fine]) / tau[coarse];
     Level& lc = lv[coarse];
     Level& lf = lv[fine];
     double dc[27], fe[27], u[3], rho;
@@ -503,7 +503,7 @@ struct Case {
   // AMR_transDDF_coalescence (LBM_Manager.cpp:1131-1169)
   void coalesce(int coarse) {
     int fine = coarse + 1;
-    double s = tau[coarse] / (2.0 * tau[fine]);
+    double s = (options & 1) ? (2.0 * tau[coarse]) / tau[fine] : tau[coarse] / (2.0 * tau[fine]);
     Level& lc = lv[coarse];
     Level& lf = lv[fine];
     double avg[27], fe[27], u[3], rho;
@@ -796,6 +796,7 @@ struct Case {
     calc_macros(L);
     collide(L);
     if (L < Lmax) kernel(L + 1);
+    if (options & 2) explode(L);
     stream(L);
     if (L == 0) bc_after_stream();
     else if (L == Lmax && ib) ib_treat(nullptr);
@@ -827,6 +828,7 @@ void* mto_create(int Q, int Lmax, double tau0, int collision, double les_C, doub
   return c;
 }
 void mto_destroy(void* h) { delete (Case*)h; }
+void mto_set_options(void* h, int options) { ((Case*)h)->options = options; }
 
 int mto_add_level(void* h, int L, const uint64_t* keys, const uint16_t* kinds, int64_t n) {
   Case* c = (Case*)h;

@@ -36,6 +36,7 @@ def lib():
         L.mto_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, _dp, _ip, _ip,
                                  _dp, C.c_int, C.c_int]
         L.mto_destroy.argtypes = [C.c_void_p]
+        L.mto_set_options.argtypes = [C.c_void_p, C.c_int]
         L.mto_add_level.argtypes = [C.c_void_p, C.c_int, _u64p, _u16p, C.c_int64]
         L.mto_set_dis.argtypes = [C.c_void_p, _u64p, _dp, C.c_int64]
         L.mto_init.argtypes = [C.c_void_p]
@@ -90,6 +91,7 @@ class OracleCase:
         self.h = L.mto_create(self.Q, case["Lmax"], case["tau0"], case["collision"], case["les_C"],
                               case["rho0"], pu0, dims.ctypes.data_as(_ip), fk.ctypes.data_as(_ip), pfu,
                               case["ib"], 1 if nee_sequential else 0)
+        L.mto_set_options(self.h, case.get("options", 0))
         for lvl, lv in enumerate(case["levels"]):
             keys, pk = _k(lv["keys"])
             kinds = np.ascontiguousarray(lv["kinds"], dtype=np.uint16)
