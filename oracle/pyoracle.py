@@ -224,9 +224,11 @@ def to_cumulants(k, rho):
     return C_.reshape(3, 3, 3)
 
 
-def from_cumulants(C_, rho):
+def from_cumulants(C_, rho, k_in):
+    """Backward cumulant transform in place on a copy of k_in (members the
+    reference never reassigns keep k_in's values)."""
     C_, pc = _d(C_)
-    k = np.zeros(27)
+    k = np.ascontiguousarray(np.array(k_in, dtype=np.float64).reshape(27))
     lib().mto_from_cumulants(pc, rho, k.ctypes.data_as(_dp))
     return k.reshape(3, 3, 3)
 

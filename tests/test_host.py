@@ -1,0 +1,154 @@
+"""Host-side checks that need no GPU: the case builders' set contract, the
+C-ABI library's exports and argument validation, the C++ driver header."""
+import ctypes
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from mortontrie_amr_amd import amr, cases as C
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# ---------------------------------------------------------------- case builders
+def parents(keys, L, Lmax):
+    x, y, z = C.decode_key(keys, L, Lmax)
+    return C.encode_key(x // 2, y // 2, z // 2, L - 1, Lmax)
+
+
+@pytest.mark.parametrize("mk", [
+    lambda: C.sphere_amr((32, 24, 24), radius=3.0, center=(12, 12, 12), shells=(5.0, 1.5)),
+    lambda: C.cavity(16, refine_box=((5, 5, 5), (11, 11, 11))),
+    lambda: C.reference_sphere((32, 24, 24), radius=3.0, center=(12, 12, 12), shells=(5.0, 1.5)),
+])
+def test_level_set_contract(mk):
+    """The Lat_Manager set relations (header/Lat_Manager.hpp:245-316,
+    Lat_Voxelize.cpp:525-733): unique keys; every C2F cell's parent is F2C and every
+    F2C cell has all 8 children as C2F; ghost_C2F cells are Q-neighbours of C2F
+    whose parents are ghost_F2C fluid cells; F2C is not in lat_f."""
+    case = mk()
+    Lmax = case["Lmax"]
+    for L, lv in enumerate(case["levels"]):
+        assert np.unique(lv["keys"]).size == lv["keys"].size
+        k = lv["kinds"]
+        assert not ((k & C.KIND_FLUID) & ((k & C.KIND_F2C) >> 1)).any()
+        if L == 0:
+            continue
+        prev = case["levels"][L - 1]
+        pk = dict(zip(prev["keys"].tolist(), prev["kinds"].tolist()))
+        c2f = lv["keys"][(k & C.KIND_C2F) != 0]
+        assert all(pk[int(p)] & C.KIND_F2C for p in parents(c2f, L, Lmax))
+        n_f2c = int(((prev["kinds"] & C.KIND_F2C) != 0).sum())
+        assert c2f.size == 8 * n_f2c
+        ghost = lv["keys"][(k & C.KIND_GHOST_C2F) != 0]
+        assert all(pk[int(p)] & C.KIND_GHOST_F2C for p in parents(ghost, L, Lmax))
+
+
+def test_sphere_workload_counts():
+    """The refined-sphere workload at full size: fluid cells per level
+    (coarse 64x32x32, r=8, shells 6 and 3)."""
+    case = C.sphere_amr()
+    cnt = C.level_counts(case)
+    assert cnt[0]["fluid"] == 54000
+    assert [c["fluid"] + c["c2f"] for c in cnt] == [54000, 47568, 220384]
+    assert C.updates_per_coarse_step(case) == 54000 + 2 * 47568 + 4 * 220384
+    assert case["Lmax"] == 2 and case["Q"] == 27
+    assert np.isclose(case["tau0"], 0.5192)
+
+
+def test_dis_fractions_analytic():
+    """dis: first crossing of the link with the sphere, in [0,1] (Lat_Voxelize.cpp:1252-1285)."""
+    case = C.sphere_amr((32, 24, 24), radius=3.0, center=(12, 12, 12), shells=(5.0, 1.5))
+    v = case["dis_vals"]
+    assert ((v == -1) | ((v >= 0) & (v <= 1))).all()
+    Lmax = case["Lmax"]
+    x, y, z = C.decode_key(case["dis_keys"], Lmax, Lmax)
+    h = 1.0 / (1 << Lmax)
+    ex, ey, ez = C.velocities(27)
+    for i in range(0, len(x), 97):
+        c = np.array([x[i] + 0.5, y[i] + 0.5, z[i] + 0.5]) * h
+        for q in range(1, 27):
+            t = v[i, q - 1]
+            if t < 0:
+                continue
+            p = c + t * h * np.array([ex[q], ey[q], ez[q]])
+            assert abs(np.linalg.norm(p - 12.0) - 3.0) < 1e-9
+
+
+def test_reported_velocity_convention():
+    """Inflow cases store -u_physical (the reference's pull reports minus the
+    physical velocity, lbm_kernel/Stream_Manager.cpp:53-62)."""
+    case = C.sphere_amr((32, 24, 24), radius=3.0, center=(12, 12, 12), shells=(5.0, 1.5), U=0.04)
+    assert case["faces"][0][1] == -0.04 and case["u0"][0] == -0.04
+
+
+# ---------------------------------------------------------------- C-ABI library
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "mt_amr.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*|int)\s+(mt_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(amr.LIB_PATH)
+    syms = header_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing
+    assert sorted(syms) == sorted(amr.EXPORTS)
+
+
+def _desc(**kw):
+    d = amr.CaseDesc()
+    d.Q, d.Lmax, d.collision, d.ib, d.precision = 19, 0, 0, 0, 0
+    d.dims0[:] = [8, 8, 8]
+    d.tau0, d.rho0 = 0.6, 1.0
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return d
+
+
+@pytest.mark.parametrize("kw,frag", [
+    (dict(Q=20), "Q must be 19 or 27"),
+    (dict(collision=1), "cumulant collision needs Q = 27"),
+    (dict(tau0=0.5), "tau0"),
+    (dict(Lmax=9), "Lmax"),
+    (dict(precision=7), "precision"),
+])
+def test_create_rejects_bad_descriptors(kw, frag):
+    """Argument validation happens before any device call (the reference's
+    log_error paths, e.g. Collision_Cumulant.cpp:24-26)."""
+    lib = amr.load()
+    h = ctypes.c_void_p()
+    rc = lib.mt_case_create(ctypes.byref(_desc(**kw)), ctypes.byref(h))
+    assert rc == -1
+    assert frag in lib.mt_last_error().decode()
+
+
+def test_periodic_faces_must_pair():
+    lib = amr.load()
+    d = _desc()
+    d.face_kind[0] = 4
+    h = ctypes.c_void_p()
+    assert lib.mt_case_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert "pairs" in lib.mt_last_error().decode()
+
+
+def test_null_handles_are_rejected():
+    lib = amr.load()
+    assert lib.mt_step(None, 1) == -1
+    assert lib.mt_macros(None, 0) == -1
+    assert lib.mt_case_destroy(None) == 0
+
+
+def test_cpp_driver_header_compiles():
+    """The LBM_Manager-shaped C++ driver (include/mt_lbm_manager.hpp) and the
+    driver check program compile against the C-ABI."""
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    src = os.path.join(ROOT, "tests", "cpp", "driver_check.cpp")
+    subprocess.check_call([gxx, "-std=c++17", "-fsyntax-only", "-Wall", "-I", os.path.join(ROOT, "include"), src])

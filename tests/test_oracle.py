@@ -1,0 +1,414 @@
+"""Pins of the CPU oracle against closed forms, invariants and brute force.
+
+Each test checks an oracle function against something computed another way
+(numpy over the velocity set, coordinates instead of Morton keys, exact
+identities), so that a dropped term, a wrong sign or a transposed index fails.
+"""
+import numpy as np
+import pytest
+
+from mortontrie_amr_amd import cases as C
+from oracle import pyoracle as O
+
+RNG = np.random.default_rng(1234)
+
+
+def lattice(Q):
+    ex, ey, ez = C.velocities(Q)
+    E = np.stack([ex, ey, ez], axis=1).astype(float)
+    n = np.abs(E).sum(1)
+    if Q == 19:
+        w = np.where(n == 0, 1 / 3, np.where(n == 1, 1 / 18, 1 / 36))
+    else:
+        w = np.where(n == 0, 8 / 27, np.where(n == 1, 2 / 27, np.where(n == 2, 1 / 54, 1 / 216)))
+    return E, w
+
+
+# ---------------------------------------------------------------- equilibria
+@pytest.mark.parametrize("Q", [19, 27])
+def test_feq_moments(Q):
+    """sum feq = rho, sum feq e = rho u, sum feq ee = rho (I/3 + uu) (LBM_Manager.cpp:137-169)."""
+    E, w = lattice(Q)
+    assert abs(w.sum() - 1) < 1e-15
+    for _ in range(5):
+        rho = 0.8 + 0.4 * RNG.random()
+        u = 0.1 * (RNG.random(3) - 0.5)
+        f = O.feq(Q, rho, u)
+        assert np.isclose(f.sum(), rho, rtol=0, atol=1e-14)
+        assert np.allclose(E.T @ f, rho * u, atol=1e-15)
+        P = np.einsum("q,qa,qb->ab", f, E, E)
+        assert np.allclose(P, rho * (np.eye(3) / 3 + np.outer(u, u)), atol=1e-14)
+
+
+@pytest.mark.parametrize("Q", [19, 27])
+def test_feq_heluo_moments(Q):
+    """He-Luo: sum = rho, sum e = rho0 u (LBM_Manager.cpp:171-203)."""
+    E, _ = lattice(Q)
+    rho, rho0, u = 0.93, 1.0, np.array([0.03, -0.02, 0.01])
+    f = O.feq_heluo(Q, rho, u, rho0)
+    assert np.isclose(f.sum(), rho, atol=1e-15)
+    assert np.allclose(E.T @ f, rho0 * u, atol=1e-15)
+
+
+def test_macros_floor():
+    """rho in (-1e-6, 1e-6) divides by +-1e-6 (LBM_Manager.cpp:994-997)."""
+    f = np.zeros(19)
+    f[1] = 2e-7
+    rho, u = O.macros_cell(f)
+    assert rho == 2e-7
+    assert np.isclose(u[0], 2e-7 / 1e-6)
+    f[1] = -2e-7
+    rho, u = O.macros_cell(f)
+    assert np.isclose(u[0], (-2e-7 * 1) / -1e-6)
+
+
+# ---------------------------------------------------------------- cumulant
+def central_moments(f, u):
+    E, _ = lattice(27)
+    c = E - u
+    k = np.zeros((3, 3, 3))
+    for a in range(3):
+        for b in range(3):
+            for cc in range(3):
+                k[a, b, cc] = np.sum(c[:, 0] ** a * c[:, 1] ** b * c[:, 2] ** cc * f)
+    return k
+
+
+def random_pop(Q=27):
+    E, w = lattice(Q)
+    rho = 1 + 0.05 * RNG.standard_normal()
+    u = 0.05 * RNG.standard_normal(3)
+    return O.feq(Q, rho, u) * (1 + 0.05 * RNG.standard_normal(Q))
+
+
+def test_central_moments_literal():
+    """The literal forward transform equals the moments computed over the velocity set."""
+    for _ in range(5):
+        f = random_pop()
+        u = 0.03 * RNG.standard_normal(3)
+        assert np.allclose(O.central_fwd(f, u), central_moments(f, u), atol=1e-15)
+
+
+def test_central_roundtrip():
+    """bwd_central_moment_trans(fwd(f)) = f (Collision_Cumulant.cpp:320-367, 476-652)."""
+    for _ in range(5):
+        f = random_pop()
+        u = 0.05 * RNG.standard_normal(3)
+        assert np.allclose(O.central_bwd(O.central_fwd(f, u), u), f, atol=1e-15)
+
+
+def test_cumulant_roundtrip():
+    """bwd_cum_trans(fwd_cum_trans(k)) = k with nothing relaxed (:374-398, :446-471)."""
+    for _ in range(5):
+        f = random_pop()
+        u = 0.05 * RNG.standard_normal(3)
+        k = O.central_fwd(f, u)
+        Cm = O.to_cumulants(k, f.sum())
+        k2 = O.from_cumulants(Cm, f.sum(), k.copy())
+        assert np.allclose(k2, k, atol=1e-16)
+
+
+def test_cumulant_fourth_order_closed_form():
+    """C211 = k211 - (k200 k011 + 2 k110 k101)/rho and C220 = k220 - (k200 k020 + 2 k110^2)/rho."""
+    f = random_pop()
+    u = f @ lattice(27)[0] / f.sum()
+    k = central_moments(f, u)
+    rho = f.sum()
+    Cm = O.to_cumulants(k, rho)
+    assert np.isclose(Cm[2, 1, 1], k[2, 1, 1] - (k[2, 0, 0] * k[0, 1, 1] + 2 * k[1, 1, 0] * k[1, 0, 1]) / rho,
+                      atol=1e-16)
+    assert np.isclose(Cm[2, 2, 0], k[2, 2, 0] - (k[2, 0, 0] * k[0, 2, 0] + 2 * k[1, 1, 0] ** 2) / rho, atol=1e-16)
+    # members the reference's forward transform never assigns stay zero
+    for i in [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (2, 1, 0), (0, 2, 1), (1, 0, 2)]:
+        assert Cm[i] == 0.0
+
+
+def test_cumulant_collision_properties():
+    """Mass and momentum conserved; shear moments relax by (1 - 1/tau); third-order
+    central moments k120, k012, k201, k111 vanish while k210, k021, k102 keep
+    their pre-collision values (the backward transform never assigns them,
+    Collision_Cumulant.cpp:451-454)."""
+    E, _ = lattice(27)
+    for tau in (0.55, 0.8, 1.0, 1.7):
+        f = random_pop()
+        rho = f.sum()
+        u = E.T @ f / rho
+        out = O.cumulant(f, rho, u, tau)
+        assert np.isclose(out.sum(), rho, atol=1e-14)
+        assert np.allclose(E.T @ out, E.T @ f, atol=1e-15)
+        kin = central_moments(f, u)
+        kout = central_moments(out, u)
+        for i in [(1, 1, 0), (1, 0, 1), (0, 1, 1)]:
+            assert np.isclose(kout[i], (1 - 1 / tau) * kin[i], atol=1e-15)
+        for i in [(1, 2, 0), (0, 1, 2), (2, 0, 1), (1, 1, 1)]:
+            assert abs(kout[i]) < 1e-15
+        for i in [(2, 1, 0), (0, 2, 1), (1, 0, 2)]:
+            assert np.isclose(kout[i], kin[i], atol=1e-15)
+        # bulk relaxes with omega2 = 1: trace of the second moments returns to k000 (c_s^2 = 1/3 each)
+        tr_out = kout[2, 0, 0] + kout[0, 2, 0] + kout[0, 0, 2]
+        assert abs(tr_out - rho) < 1e-3
+
+
+def test_cumulant_equilibrium_fixed_point_second_order():
+    """At equilibrium with the cell's own velocity the second-order moments are unchanged."""
+    E, _ = lattice(27)
+    rho, u = 1.02, np.array([0.02, -0.01, 0.015])
+    f = O.feq(27, rho, u)
+    out = O.cumulant(f, rho, E.T @ f / f.sum(), 0.6)
+    uu = E.T @ f / f.sum()
+    kin, kout = central_moments(f, uu), central_moments(out, uu)
+    for i in [(2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 1, 0)]:
+        assert np.isclose(kout[i], kin[i], atol=1e-6)
+
+
+# ---------------------------------------------------------------- LES
+def test_smagorinsky_closed_form():
+    """tau_eff = 3(nu + C^2 S) + 1/2 with S the positive root of 3 C^2 S^2 + nu S - 3 Q/2 = 0
+    (lbm_kernel/les.cpp:36-44); Q = 0 gives tau."""
+    for tau, Cs, qn in [(0.6, 0.1, 0.0), (0.52, 0.1, 1e-3), (0.9, 0.04, 0.02)]:
+        t = O.smag_tau(tau, Cs, qn)
+        nu = (tau - 0.5) / 3
+        S = ((t - 0.5) / 3 - nu) / Cs ** 2
+        assert np.isclose(3 * Cs ** 2 * S ** 2 + nu * S, 1.5 * qn, atol=1e-15)
+        if qn == 0:
+            assert np.isclose(t, tau, atol=1e-15)
+
+
+def test_stress_norm():
+    """|sum_q e e (f - feq)|_F (lbm_kernel/les.cpp:12-32)."""
+    E, _ = lattice(19)
+    f = random_pop(19)
+    fe = O.feq(19, 1.0, np.zeros(3))
+    P = np.einsum("q,qa,qb->ab", f - fe, E, E)
+    assert np.isclose(O.stress_norm(19, f, fe), np.sqrt((P ** 2).sum()), atol=1e-15)
+
+
+# ---------------------------------------------------------------- Morton
+@pytest.mark.parametrize("Lmax", [0, 2, 4])
+def test_find_neighbor_bruteforce(Lmax):
+    """find_neighbor (header/Morton_Assist.h:229-289) against decode -> +-1 -> encode,
+    including the wrap of the dilated field at 0 and at the top bit."""
+    E, _ = lattice(27)
+    for L in range(Lmax + 1):
+        s = Lmax - L
+        nbits = 21 - s
+        top = (1 << nbits) - 1
+        for trial in range(40):
+            if trial < 4:
+                x, y, z = [0, top][trial % 2], [0, top][(trial // 2) % 2], 5
+            else:
+                x, y, z = (int(v) for v in RNG.integers(0, 1 << min(nbits, 12), 3))
+            key = int(C.encode_key(x, y, z, L, Lmax))
+            for q in range(27):
+                dx, dy, dz = (int(v) for v in E[q])
+                want = int(C.encode_key((x + dx) % (1 << nbits), (y + dy) % (1 << nbits), (z + dz) % (1 << nbits),
+                                        L, Lmax))
+                assert O.find_neighbor(key, L, Lmax, dx, dy, dz) == want
+
+
+def test_encode_decode_roundtrip():
+    x, y, z = RNG.integers(0, 1 << 18, (3, 100))
+    k = C.encode_key(x, y, z, 1, 3)
+    x2, y2, z2 = C.decode_key(k, 1, 3)
+    assert (x2 == x).all() and (y2 == y).all() and (z2 == z).all()
+
+
+# ---------------------------------------------------------------- boundaries
+# the reference's per-set direction lists (header/BC/D3Q19_BC_Manager.hpp:62-89,94-121)
+BDRY_DIRS_19 = [
+    [0, 1, 3, 4, 5, 6, 7, 9, 11, 13, 15, 16, 17, 18], [0, 2, 3, 4, 5, 6, 8, 10, 12, 14, 15, 16, 17, 18],
+    [0, 1, 2, 3, 5, 6, 7, 10, 11, 12, 13, 14, 15, 17], [0, 1, 2, 4, 5, 6, 8, 9, 11, 12, 13, 14, 16, 18],
+    [0, 1, 2, 3, 4, 5, 7, 8, 9, 10, 11, 14, 15, 18], [0, 1, 2, 3, 4, 6, 7, 8, 9, 10, 12, 13, 16, 17],
+    [0, 2, 4, 5, 6, 8, 12, 14, 16, 18], [0, 1, 4, 5, 6, 9, 11, 13, 16, 18], [0, 1, 3, 5, 6, 7, 11, 13, 15, 17],
+    [0, 2, 3, 5, 6, 10, 12, 14, 15, 17], [0, 1, 2, 4, 6, 8, 9, 12, 13, 16], [0, 1, 2, 3, 6, 7, 10, 12, 13, 17],
+    [0, 1, 2, 4, 5, 8, 9, 11, 14, 18], [0, 1, 2, 3, 5, 7, 10, 11, 14, 15], [0, 2, 3, 4, 6, 8, 10, 12, 16, 17],
+    [0, 1, 3, 4, 6, 7, 9, 13, 16, 17], [0, 2, 3, 4, 5, 8, 10, 14, 15, 18], [0, 1, 3, 4, 5, 7, 9, 11, 15, 18],
+    [0, 2, 4, 6, 8, 12, 16], [0, 1, 4, 6, 9, 13, 16], [0, 1, 3, 6, 7, 13, 17], [0, 2, 3, 6, 10, 12, 17],
+    [0, 2, 4, 5, 8, 14, 18], [0, 1, 4, 5, 9, 11, 18], [0, 1, 3, 5, 7, 11, 15], [0, 2, 3, 5, 10, 14, 15],
+]
+BDRY_DIRS_27_TAIL = [  # the D3Q27 rows add these corner directions
+    [19, 21, 24, 26], [20, 22, 23, 25], [19, 22, 24, 25], [20, 21, 23, 26], [19, 21, 23, 25], [20, 22, 24, 26],
+    [20, 23], [21, 26], [19, 24], [22, 25], [20, 26], [22, 24], [21, 23], [19, 25], [20, 22], [24, 26],
+    [23, 25], [19, 21], [20], [26], [24], [22], [23], [21], [19], [25],
+]
+# faces of each of the 26 sets (W,E,S,N,B,T = 0..5), order of header/BC/D3Q19_BC_Manager.hpp:25-53
+SET_FACES = [[0], [1], [2], [3], [4], [5], [1, 3], [0, 3], [0, 2], [1, 2], [3, 5], [2, 5], [3, 4], [2, 4],
+             [1, 5], [0, 5], [1, 4], [0, 4], [1, 3, 5], [0, 3, 5], [0, 2, 5], [1, 2, 5], [1, 3, 4], [0, 3, 4],
+             [0, 2, 4], [1, 2, 4]]
+
+
+@pytest.mark.parametrize("Q", [19, 27])
+def test_boundary_direction_rule(Q):
+    """The NEE direction set of every face/edge/corner cell is 'the directions
+    whose neighbour is inside the box', the rule the oracle and the device use."""
+    E, _ = lattice(Q)
+    normals = [(-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)]
+    for i, faces in enumerate(SET_FACES):
+        got = [q for q in range(Q) if all(E[q] @ np.array(normals[f]) <= 0 for f in faces)]
+        want = BDRY_DIRS_19[i] + (BDRY_DIRS_27_TAIL[i] if Q == 27 else [])
+        assert sorted(got) == sorted(want), (i, faces)
+
+
+def test_zouhe_reconstructs_wall_velocity():
+    """After a step, every single-face wet cell carries exactly the wall velocity
+    and sum f equals the Zou-He density (definition; BC/ZouHe_Velocity.cpp:170-200 text)."""
+    case = C.cavity(8, lid=0.05)
+    o = O.OracleCase(case)
+    o.step(3)
+    lv = case["levels"][0]
+    E, _ = lattice(19)
+    f = o.f(0)
+    faces = (lv["kinds"] >> 8) & 63
+    single = np.isin(faces, [1, 2, 4, 8, 16, 32])
+    rho = f.sum(0)
+    u = (E.T @ f) / rho
+    for bit, face in zip([1, 2, 4, 8, 16, 32], range(6)):
+        m = faces == bit
+        want = np.array(case["faces"][face][1:4])
+        assert np.abs(u[:, m] - want[:, None]).max() < 1e-14
+    assert single.sum() == 6 * 36
+
+
+def test_outflow_copies_unknowns():
+    """Outflow (definition; BC/Outflow.cpp:39-52 text): the e.n = +1 populations of an
+    x-max cell equal those of its x-1 neighbour after the step."""
+    case = C.sphere_amr((16, 12, 12), radius=2.0, center=(6, 6, 6), shells=(), with_sphere=False, Q=19,
+                        collision=C.COLLIDE_LBGK)
+    o = O.OracleCase(case)
+    o.step(4)
+    lv = case["levels"][0]
+    x, y, z = C.decode_key(lv["keys"], 0, 0)
+    f = o.f(0)
+    E, _ = lattice(19)
+    east = np.where(x == 15)[0]
+    inner = {(a, b, c): i for i, (a, b, c) in enumerate(zip(x, y, z))}
+    unk = [q for q in range(19) if E[q, 0] == 1]
+    for i in east:
+        j = inner[(14, y[i], z[i])]
+        assert np.array_equal(f[unk, i], f[unk, j])
+
+
+def test_stream_is_reversed_pull():
+    """ABPatternPull: f_q(x) <- fcol_q(x + e_q) when that cell exists, else unchanged
+    (lbm_kernel/Stream_Manager.cpp:43-76), checked with coordinates, not keys."""
+    case = C.reservoir_box((6, 5, 4))
+    o = O.OracleCase(case)
+    lv = case["levels"][0]
+    n = lv["keys"].size
+    fc = RNG.random((19, n))
+    f0 = RNG.random((19, n))
+    o.set(0, 1, fc)
+    o.set(0, 0, f0)
+    o.stream(0)
+    f = o.f(0)
+    x, y, z = C.decode_key(lv["keys"], 0, 0)
+    idx = {(a, b, c): i for i, (a, b, c) in enumerate(zip(x, y, z))}
+    E, _ = lattice(19)
+    fluid = (lv["kinds"] & C.KIND_FLUID) != 0
+    for i in np.where(fluid)[0]:
+        for q in range(19):
+            src = (x[i] + E[q, 0], y[i] + E[q, 1], z[i] + E[q, 2])
+            want = fc[q, idx[src]] if src in idx else f0[q, i]
+            assert f[q, i] == want
+    # non-fluid cells are untouched
+    assert np.array_equal(f[:, ~fluid], f0[:, ~fluid])
+
+
+def test_nee_reservoir_order_independent():
+    """NEE writes only f of the lat_bc cells.  Its in-place chain along a face
+    (BC/NonEquilibrumExtrapolation.cpp:50-66, tangential directions reading
+    already-updated lat_bc neighbours) only ever reads lat_bc cells whose maps hold
+    the frozen rho0/u0, so the reference's sequential order and the device's Jacobi
+    form give identical f everywhere, bc cells included."""
+    case = C.reservoir_box((10, 8, 8), u0=(0.04, 0.0, 0.0))
+    a, b = O.OracleCase(case, nee_sequential=True), O.OracleCase(case, nee_sequential=False)
+    lv = case["levels"][0]
+    fl = (lv["kinds"] & C.KIND_FLUID) != 0
+    f0 = a.f(0)
+    f0[:, fl] *= 1 + 0.01 * RNG.standard_normal((19, int(fl.sum())))
+    a.set(0, 0, f0)
+    b.set(0, 0, f0)
+    a.step(6)
+    b.step(6)
+    assert np.array_equal(a.f(0)[:, fl], b.f(0)[:, fl])
+    assert np.array_equal(a.rho(0)[fl], b.rho(0)[fl])
+    assert np.array_equal(a.f(0)[:, ~fl], b.f(0)[:, ~fl])
+    # and the bc f did move away from the initial equilibrium (the pass is live)
+    assert not np.allclose(a.f(0)[:, ~fl], O.feq(19, 1.0, np.array([0.04, 0, 0]))[:, None], atol=1e-6)
+
+
+# ---------------------------------------------------------------- transfers
+def two_level_case():
+    return C.cavity(12, refine_box=((4, 4, 4), (8, 8, 8)))
+
+
+def test_coalescence_of_equilibrium_children():
+    """Eight identical rho = 1 equilibrium children coalesce to that equilibrium
+    (He-Luo feq = feq at rho = rho0, LBM_Manager.cpp:1131-1169)."""
+    case = two_level_case()
+    o = O.OracleCase(case)
+    u = np.array([0.02, -0.01, 0.005])
+    fe = O.feq(19, 1.0, u)
+    n1 = case["levels"][1]["keys"].size
+    o.set(1, 0, np.repeat(fe[:, None], n1, axis=1))
+    o.coalesce(0)
+    lv0 = case["levels"][0]
+    f2c = (lv0["kinds"] & C.KIND_F2C) != 0
+    got = o.f(0)[:, f2c]
+    assert np.abs(got - fe[:, None]).max() < 1e-16
+
+
+def test_explosion_mass_and_momentum():
+    """An exploded child carries the parent's mass; its momentum is
+    rho0 u + s (rho - rho0) u with s = 2 tau_f / tau_c (LBM_Manager.cpp:1059-1099)."""
+    case = two_level_case()
+    o = O.OracleCase(case)
+    lv0 = case["levels"][0]
+    n0 = lv0["keys"].size
+    E, _ = lattice(19)
+    fc = np.stack([O.feq(19, 1.0 + 0.05 * RNG.standard_normal(), 0.03 * RNG.standard_normal(3)) for _ in range(n0)],
+                  axis=1) * (1 + 0.01 * RNG.standard_normal((19, n0)))
+    o.set(0, 1, fc)
+    o.explode(1)
+    tau = C.taus(case)
+    s = 2 * tau[1] / tau[0]
+    f2c = np.where((lv0["kinds"] & C.KIND_F2C) != 0)[0]
+    lv1 = case["levels"][1]
+    x0, y0, z0 = C.decode_key(lv0["keys"], 0, 1)
+    fine = o.fcol(1)
+    fidx = {int(k): i for i, k in enumerate(lv1["keys"])}
+    for i in f2c[:20]:
+        rho = fc[:, i].sum()
+        u = E.T @ fc[:, i] / rho
+        for j in range(8):
+            ck = int(C.encode_key(2 * x0[i] + (j & 1), 2 * y0[i] + ((j >> 1) & 1), 2 * z0[i] + (j >> 2), 1, 1))
+            child = fine[:, fidx[ck]]
+            assert np.isclose(child.sum(), rho, atol=1e-14)
+            assert np.allclose(E.T @ child, u + s * (rho - 1.0) * u, atol=1e-14)
+
+
+def test_interface_options_change_only_the_transfer():
+    """The convective-scaling option changes the transfer factor and nothing else:
+    a single-level case is unaffected."""
+    a = C.cavity(10)
+    b = dict(a, options=1)
+    oa, ob = O.OracleCase(a), O.OracleCase(b)
+    oa.step(5)
+    ob.step(5)
+    assert np.array_equal(oa.f(0), ob.f(0))
+
+
+# ---------------------------------------------------------------- solid boundary
+def test_ib_density_clamp_and_force_log():
+    """After treat() every treated cell has rho >= 0.3 (the clamp of
+    ComplexBoundary_Manager.cpp:150-158; multi-direct forcing conserves rho), and
+    one force entry is logged per treat() call."""
+    case = C.sphere_amr((32, 24, 24), radius=3.0, center=(12, 12, 12), shells=(5.0, 1.5), Q=19,
+                        collision=C.COLLIDE_LBGK)
+    o = O.OracleCase(case)
+    o.step(1)
+    assert o.force_log().shape == (4, 3)
+    lv = case["levels"][2]
+    treated = ((lv["kinds"] & C.KIND_DIS) != 0) & ((lv["kinds"] & C.KIND_FLUID) != 0)
+    rho = o.f(2)[:, treated].sum(0)
+    assert rho.min() >= 0.3 - 1e-12
