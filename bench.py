@@ -5,7 +5,10 @@ FP64 (coarse 64x32x32, sphere radius 8 coarse cells at (16,16,16), refinement
 shells within 6 and 3 coarse cells of the surface, Zou-He inflow u_x=0.04,
 outflow at x-max, periodic y/z, Re=100, link IB with Filippova-Haenel
 interpolation on the finest level).  One step is one coarse time step of the
-recursive sub-cycling schedule (level L updated 2^L times).
+recursive sub-cycling schedule (level L updated 2^L times).  The interface
+refills the C2F layer in both fine sub-steps (MT_OPT_EXPLODE_BOTH): with the
+reference's sub-step-A-only explosion this configuration is linearly unstable
+at tau0 = 0.5192 (NaN within ~100 coarse steps on the device and in the oracle).
 
 Prints one JSON line: MLUPS over the timed region (inputs resident in HBM),
 the HBM roofline of the dominant kernel (finest-level fused update) from HIP
@@ -44,18 +47,20 @@ def run_gpu(case, steps, warmup, precision, timing_steps):
     g = amr.AmrCase(case, precision=precision, graphs=True)
     g.step(warmup)
     g.sync()
+    g.ib_force(reset=True)
     t0 = time.perf_counter()
     g.step(steps)
     g.sync()
     t1 = time.perf_counter()
+    cd = g.drag_coefficient(reset=True)
     # kernel durations from HIP events around every fused launch, same stream
     g.set_timing(True)
     g.step(timing_steps)
     ms, calls = g.timing()
     g.set_timing(False)
-    m, p, nan = g.reduce(0)
+    nan = sum(g.reduce(L)[2] for L in range(case["Lmax"] + 1))
     g.close()
-    return (t1 - t0), ms, calls, nan
+    return (t1 - t0), ms, calls, nan, cd
 
 
 def cpu_baseline(case, steps):
@@ -70,7 +75,7 @@ def cpu_baseline(case, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--timing-steps", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -84,13 +89,13 @@ def main():
 
     from mortontrie_amr_amd import cases as C, amr
     t_build = time.perf_counter()
-    case = C.sphere_amr()
+    case = C.sphere_amr(options=2)
     t_build = time.perf_counter() - t_build
     per_level = level_updates(case)
     upd_step = sum(n << L for L, n in enumerate(per_level))
     Q = case["Q"]
 
-    secs, ms, calls, nan = run_gpu(case, args.steps, args.warmup, amr.FP64, args.timing_steps)
+    secs, ms, calls, nan, cd = run_gpu(case, args.steps, args.warmup, amr.FP64, args.timing_steps)
     mlups = upd_step * args.steps / secs / 1e6
     Lf = case["Lmax"]
     avg_ms = ms[Lf] / max(calls[Lf], 1)
@@ -122,17 +127,20 @@ def main():
             "avg_launch_ms": round(avg_ms, 5), "launches_timed": int(calls[Lf]),
             "per_level_avg_ms": [round(ms[L] / max(calls[L], 1), 5) for L in range(Lf + 1)],
         },
-        "nan_count_level0": nan,
+        "nan_count": nan,
+        "drag_coefficient": round(cd, 5),
+        "interface_option": "explode_both",
     }
     if not args.no_variants:
         var = {}
         for name, prec in (("fp32_shifted", amr.FP32_SHIFTED), ("fp16_storage", amr.FP16_STORAGE)):
-            s2, ms2, c2, _ = run_gpu(case, max(args.steps // 2, 10), args.warmup, prec, args.timing_steps)
+            s2, ms2, c2, nan2, cd2 = run_gpu(case, max(args.steps // 2, 10), args.warmup, prec, args.timing_steps)
             a2 = ms2[Lf] / max(c2[Lf], 1)
             bl = per_level[Lf] * bytes_per_update(Q, prec)
             var[name] = {"MLUPS": round(upd_step * max(args.steps // 2, 10) / s2 / 1e6, 2),
                          "finest_kernel_GBs": round(bl / (a2 * 1e-3) / 1e9, 1),
-                         "finest_kernel_ms": round(a2, 5)}
+                         "finest_kernel_ms": round(a2, 5), "drag_coefficient": round(cd2, 5),
+                         "nan_count": nan2}
         out["variants"] = var
     if not args.no_cpu:
         n = args.cpu_steps

@@ -30,6 +30,8 @@ void set_error(const std::string& s) { g_err = s; }
     if (r_ != MT_OK) return r_;   \
   } while (0)
 
+static int run_level(Case& c, int L, bool last);
+
 static bool has_reservoir(const Case& c) {
   for (int f = 0; f < 6; ++f)
     if (c.d.face_kind[f] == MT_BC_RESERVOIR) return true;
@@ -57,25 +59,48 @@ static int prologue(Case& c) {
 
 // LBMkernel(L) (LBM_Simulator.cpp:20-229) over the fused kernels.  On entry
 // fcol[pf] of every level holds the collision of the sub-step about to stream.
-static int run_level(Case& c, int L, bool last) {
+//
+// Each level runs on its own stream.  Level L's explosion and stream+collide
+// read level L-1's current fcol and level L's current fcol and write level L's
+// C2F / ghost_C2F cells and the other fcol buffer; nothing in LBMkernel(L+1)
+// writes level-L data, and what it reads of level L (the current fcol of the
+// F2C and ghost_F2C parents) is neither of those cell sets (an interface cell is
+// never both, mt_case_upload_level rejects such a level).  So the finer levels'
+// sub-tree runs concurrently with level L's explosion and fused update; only
+// the coalescence, which reads the finer level's final f, joins them.  Issue
+// order is the reference's (finer levels, explosion, stream, coalescence).
+static int substep(Case& c, int L, bool child_last, bool write_f, bool nee, bool write_macros, bool explode) {
   DevLevel& l = c.lv[L];
-  const bool res0 = (L == 0) && has_reservoir(c);
-  // sub-step A
-  if (L < c.Lmax) TRY(run_level(c, L + 1, L == 0 ? last : false));
-  if (L > 0) TRY(launch_explode(c, L));
-  TRY(launch_fused(c, L, (L == 0 && last) || res0, res0));
-  if (L < c.Lmax) TRY(launch_coalesce(c, L, true));
-  l.pf ^= 1;
-  l.pr ^= 1;
-  if (L == 0) return MT_OK;
-  // sub-step B: no explosion (LBM_Simulator.cpp:152-217)
-  if (L < c.Lmax) TRY(run_level(c, L + 1, last));
-  if (c.d.options & MT_OPT_EXPLODE_BOTH) TRY(launch_explode(c, L));
-  TRY(launch_fused(c, L, last, false));
-  if (L < c.Lmax) TRY(launch_coalesce(c, L, true));
+  hipStream_t s = c.st[L];
+  if (L < c.Lmax) {
+    MT_HIP(hipEventRecord(c.ev_ready[L], s));
+    MT_HIP(hipStreamWaitEvent(c.st[L + 1], c.ev_ready[L], 0));
+    TRY(run_level(c, L + 1, child_last));
+  }
+  if (explode) TRY(launch_explode(c, L));
+  TRY(launch_fused(c, L, write_f, nee, write_macros));
+  if (L < c.Lmax) {
+    MT_HIP(hipEventRecord(c.ev_done[L + 1], c.st[L + 1]));
+    MT_HIP(hipStreamWaitEvent(s, c.ev_done[L + 1], 0));
+    TRY(launch_coalesce(c, L, true, write_macros));
+  }
   l.pf ^= 1;
   l.pr ^= 1;
   return MT_OK;
+}
+static int run_level(Case& c, int L, bool last) {
+  const bool res0 = (L == 0) && has_reservoir(c);
+  if (L == 0) {
+    // level 0 has one sub-step per coarse step
+    return substep(c, 0, last, last || res0, res0, last || res0, false);
+  }
+  TRY(substep(c, L, false, false, false, false, true));
+  // sub-step B: the reference does not explode here (LBM_Simulator.cpp:152-217)
+  return substep(c, L, last, last, false, last, (c.d.options & MT_OPT_EXPLODE_BOTH) != 0);
+}
+
+static void use_level_streams(Case& c, bool on) {
+  for (int L = 0; L <= c.Lmax; ++L) c.st[L] = (on && L > 0) ? c.lstream[L] : c.stream;
 }
 
 static int parity_key(const Case& c, bool last) {
@@ -165,11 +190,17 @@ int mt_case_create(const mt_case_desc* d, mt_case** out) {
   // tau_L = 2^L (tau0 - 1/2) + 1/2: nu_lb doubles per level (LBM_Manager.cpp:814-818)
   for (int L = 0; L <= c->Lmax; ++L) c->tau[L] = double(1 << L) * (d->tau0 - 0.5) + 0.5;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  for (int L = 1; L <= c->Lmax && e == hipSuccess; ++L) e = hipStreamCreateWithFlags(&c->lstream[L], hipStreamNonBlocking);
+  for (int L = 0; L <= c->Lmax && e == hipSuccess; ++L) {
+    e = hipEventCreateWithFlags(&c->ev_ready[L], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_done[L], hipEventDisableTiming);
+  }
   if (e != hipSuccess) {
     set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
-    delete c;
+    mt_case_destroy(c);
     return MT_ERR_HIP;
   }
+  use_level_streams(*c, false);
   *out = c;
   return MT_OK;
 }
@@ -184,6 +215,11 @@ int mt_case_destroy(mt_case* c) {
     hipEventDestroy(p.second.second);
   }
   for (void* p : c->allocs) hipFree(p);
+  for (int L = 0; L < kMaxLevels; ++L) {
+    if (c->ev_ready[L]) hipEventDestroy(c->ev_ready[L]);
+    if (c->ev_done[L]) hipEventDestroy(c->ev_done[L]);
+    if (c->lstream[L]) hipStreamDestroy(c->lstream[L]);
+  }
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return MT_OK;
@@ -195,6 +231,15 @@ int mt_case_upload_level(mt_case* c, int level, const uint64_t* keys, const uint
   if (c->finalized) { set_error("case already finalized"); return MT_ERR_STATE; }
   if (n < 0 || (n && (!keys || !kinds))) { set_error("bad key array"); return MT_ERR_ARG; }
   if (n > (int64_t)1 << 30) { set_error("too many cells on one level"); return MT_ERR_ARG; }
+  // an interface cell is either on the coarse side (F2C, ghost_F2C) or on the
+  // fine side (C2F, ghost_C2F) of a level boundary, never both
+  const uint16_t coarse_side = MT_KIND_F2C | MT_KIND_GHOST_F2C, fine_side = MT_KIND_C2F | MT_KIND_GHOST_C2F;
+  for (int64_t i = 0; i < n; ++i)
+    if ((kinds[i] & coarse_side) && (kinds[i] & fine_side)) {
+      set_error("level " + std::to_string(level) + ": cell " + std::to_string(i) +
+                " is on both sides of a level interface (refinement shells too close)");
+      return MT_ERR_ARG;
+    }
   DevLevel& l = c->lv[level];
   l.h_keys.assign(keys, keys + n);
   l.h_kinds.assign(kinds, kinds + n);
@@ -228,15 +273,18 @@ int mt_case_finalize(mt_case* c) {
     if (!l.f) { set_error("out of device memory"); return MT_ERR_HIP; }
     max_fp = std::max<int64_t>(max_fp, (ns + 255) / 256);
     max_fp = std::max<int64_t>(max_fp, (l.n_active + 3) / 4);
+    max_fp = std::max<int64_t>(max_fp, (l.n_ibl + 31) / 32);
   }
   c->force_part_cap = max_fp;
   c->force_part = c->alloc<double>(3 * max_fp);
   c->force_acc = c->alloc<double>(8);
+  c->force_slots = c->alloc<double>(3 * max_fp);
   c->nee_tmp = c->alloc<double>((int64_t)c->Q * std::max<int64_t>(1, c->lv[0].n_bcres));
   c->red_cap = 5 * 1024;
   c->red_tmp = c->alloc<double>(c->red_cap);
-  if (!c->force_part || !c->force_acc || !c->nee_tmp || !c->red_tmp) { set_error("out of device memory"); return MT_ERR_HIP; }
+  if (!c->force_part || !c->force_acc || !c->force_slots || !c->nee_tmp || !c->red_tmp) { set_error("out of device memory"); return MT_ERR_HIP; }
   MT_HIP(hipMemsetAsync(c->force_acc, 0, 8 * sizeof(double), c->stream));
+  MT_HIP(hipMemsetAsync(c->force_slots, 0, 3 * max_fp * sizeof(double), c->stream));
   TRY(init_state(*c));
   MT_HIP(hipStreamSynchronize(c->stream));
   for (int L = 0; L <= c->Lmax; ++L) {
@@ -409,7 +457,11 @@ int mt_step(mt_case* c, int n) {
   if (n < 0) { set_error("negative step count"); return MT_ERR_ARG; }
   if (n == 0) return MT_OK;
   if (!c->ahead) TRY(prologue(*c));
-  for (int i = 0; i < n; ++i) TRY(coarse_step(*c, i == n - 1));
+  use_level_streams(*c, true);
+  int r = MT_OK;
+  for (int i = 0; i < n && r == MT_OK; ++i) r = coarse_step(*c, i == n - 1);
+  use_level_streams(*c, false);
+  if (r != MT_OK) return r;
   if (c->timing) TRY(drain_timing(*c));
   return MT_OK;
 }
@@ -462,15 +514,23 @@ int mt_ib_force(mt_case* c, double* sum3, int64_t* count, int reset) {
   CHECK_FINAL(c);
   double acc[8];
   MT_HIP(hipMemcpyAsync(acc, c->force_acc, sizeof acc, hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> slots(3 * c->force_part_cap);
+  MT_HIP(hipMemcpyAsync(slots.data(), c->force_slots, sizeof(double) * slots.size(), hipMemcpyDeviceToHost,
+                        c->stream));
   MT_HIP(hipStreamSynchronize(c->stream));
+  double s[3] = {acc[0], acc[1], acc[2]};
+  for (int64_t b = 0; b < c->force_part_cap; ++b)
+    for (int a = 0; a < 3; ++a) s[a] += slots[3 * b + a];
   if (sum3) {
-    sum3[0] = acc[0];
-    sum3[1] = acc[1];
-    sum3[2] = acc[2];
+    sum3[0] = s[0];
+    sum3[1] = s[1];
+    sum3[2] = s[2];
   }
-  if (count) *count = (int64_t)acc[3];
+  if (count) *count = (int64_t)(acc[3] + acc[7]);
   if (reset) {
     MT_HIP(hipMemsetAsync(c->force_acc, 0, 4 * sizeof(double), c->stream));
+    MT_HIP(hipMemsetAsync(c->force_acc + 7, 0, sizeof(double), c->stream));
+    MT_HIP(hipMemsetAsync(c->force_slots, 0, sizeof(double) * slots.size(), c->stream));
     MT_HIP(hipStreamSynchronize(c->stream));
   }
   return MT_OK;

@@ -42,7 +42,13 @@ struct DevLevel {
   int32_t* gf2c = nullptr;             // ghost_F2C slots
   int64_t n_gf2c = 0;
   int32_t* gf2c_child = nullptr;       // [n_gf2c][8] ghost_C2F target or -1
+  // explosion work list: F2C parents then ghost_F2C parents, with their targets
+  int32_t* xp_list = nullptr;
+  int32_t* xp_child = nullptr;         // [n_xp][8]
+  int64_t n_xp = 0;
   int32_t* dis_row = nullptr;          // [n_slots] row in dis table or -1 (finest level)
+  int32_t* ibl = nullptr;              // treated dis slots (dis and lat_f), finest level
+  int64_t n_ibl = 0;
   double* dis = nullptr;               // [n_dis][Q-1]
   int64_t n_dis = 0;
   int32_t* bcres = nullptr;
@@ -61,11 +67,19 @@ struct Case {
   bool graphs = true;
   bool timing = false;
   hipStream_t stream = nullptr;
+  // per-level streams used by mt_step (level 0 uses `stream`); st[L] is the
+  // stream a level's kernels go to: `stream` for the seam entry points
+  hipStream_t lstream[kMaxLevels] = {nullptr};
+  hipStream_t st[kMaxLevels] = {nullptr};
+  hipEvent_t ev_ready[kMaxLevels] = {nullptr};  // level L's current fcol final for its children
+  hipEvent_t ev_done[kMaxLevels] = {nullptr};   // level L's sub-step tree finished
   std::vector<void*> allocs;
   // IB force
   double* force_part = nullptr;  // per-block partials of the finest fused kernel
   int64_t force_part_cap = 0;
-  double* force_acc = nullptr;   // [4]: sum x,y,z, count
+  double* force_acc = nullptr;   // [8]: seam-path sum x,y,z and count, last call x,y,z,
+                                 // [7] treat() calls folded into force_slots
+  double* force_slots = nullptr; // [force_part_cap][3] per-block running sums of the fused path
   double* nee_tmp = nullptr;     // NEE Jacobi staging
   double* red_tmp = nullptr;     // reduction partials
   int64_t red_cap = 0;
@@ -108,10 +122,10 @@ int launch_macros(Case& c, int L);
 int launch_collide(Case& c, int L);
 int launch_stream(Case& c, int L);
 int launch_explode(Case& c, int L);
-int launch_coalesce(Case& c, int L, bool collide_after);
+int launch_coalesce(Case& c, int L, bool collide_after, bool write_macros = true);
 int launch_bc(Case& c);
 int launch_ib(Case& c, bool reduce_into_acc);
-int launch_fused(Case& c, int L, bool write_f, bool bc_nee_after);
+int launch_fused(Case& c, int L, bool write_f, bool bc_nee_after, bool write_macros = true);
 int launch_gather(Case& c, int L, int field, const int32_t* d_slots, int64_t n, double* d_out, int buf);
 int launch_scatter(Case& c, int L, int field, const int32_t* d_slots, int64_t n, const double* d_in, int buf);
 int launch_reduce(Case& c, int L, double* h_out);

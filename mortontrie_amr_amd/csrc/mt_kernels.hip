@@ -58,6 +58,29 @@ __device__ __forceinline__ int nbr_slot(const int* N, int lx, int ly, int lz, in
   int nl = brick_local((lx + DX) & 3, (ly + DY) & 3, (lz + DZ) & 3);
   return nb < 0 ? -1 : ((nb << 6) | nl);
 }
+// compile-time direction form of nbr_slot
+template <int DX, int DY, int DZ>
+__device__ __forceinline__ int nbr_slot_c(const int* N, int lx, int ly, int lz) {
+  const bool cx = DX > 0 ? (lx == 3) : (DX < 0 ? (lx == 0) : false);
+  const bool cy = DY > 0 ? (ly == 3) : (DY < 0 ? (ly == 0) : false);
+  const bool cz = DZ > 0 ? (lz == 3) : (DZ < 0 ? (lz == 0) : false);
+  constexpr int OX = DX + 1, OY = DY + 1, OZ = DZ + 1;
+  const int n000 = N[13];
+  const int n100 = N[OX + 3 + 9], n010 = N[1 + 3 * OY + 9], n001 = N[1 + 3 + 9 * OZ];
+  const int n110 = N[OX + 3 * OY + 9], n101 = N[OX + 3 + 9 * OZ], n011 = N[1 + 3 * OY + 9 * OZ];
+  const int n111 = N[OX + 3 * OY + 9 * OZ];
+  const int nb = cz ? (cy ? (cx ? n111 : n011) : (cx ? n101 : n001)) : (cy ? (cx ? n110 : n010) : (cx ? n100 : n000));
+  const int nl = brick_local((lx + DX) & 3, (ly + DY) & 3, (lz + DZ) & 3);
+  return (nb << 6) | nl;  // callers only use it when the neighbour is present
+}
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& fn) {
+  if constexpr (B < E) {
+    fn(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(fn);
+  }
+}
+
 // general offset (|d| <= 4) through the table in memory (rare paths)
 __device__ __forceinline__ int slot_off(const int32_t* bnbr, int s, int dx, int dy, int dz) {
   int lx, ly, lz;
@@ -283,15 +306,20 @@ struct FusedArgs {
   const int32_t* __restrict__ dis_row;
   const double* __restrict__ dis;
   int Qm1;
-  double* force_part;
+  int write_macros;
 };
 
+// One 64-lane block per brick.  Workgroups are dealt round-robin over the 8
+// XCDs, each with its own L2; the block -> brick map below gives every XCD one
+// contiguous Morton range of bricks, so a brick's neighbour bricks (its halo)
+// are mostly served by the same L2.
 template <int Q, typename S>
-__global__ __launch_bounds__(256) void k_fused(FusedArgs a) {
+__global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
   typedef typename RealOf<S>::type R;
-  __shared__ double red[12];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t bi = (int64_t)blockIdx.x * 4 + wave;
+  const int lane = threadIdx.x;
+  const int64_t nb = gridDim.x;
+  const int64_t per_xcd = (nb + 7) >> 3;
+  const int64_t bi = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
   double fx = 0, fy = 0, fz = 0;
   const S* fin = (const S*)a.v.fcol_in;
   S* fout = (S*)a.v.fcol_out;
@@ -308,17 +336,24 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs a) {
       int lx, ly, lz;
       brick_coords(lane, lx, ly, lz);
       const uint32_t pm = a.v.pmask[s];
+      // a treated solid-boundary cell already holds its post-stream f (written by
+      // k_ib_links for this sub-step): read it like a cell with no sources
+      const int row = (a.do_ib && (k & MT_KIND_DIS) && (k & MT_KIND_FLUID)) ? a.dis_row[s] : -1;
+      const uint32_t pmx = row >= 0 ? 0u : pm;
       R f[Q];
-      f[0] = load_pop<Q, S, R>(fin, ns, 0, s);
-#pragma unroll
-      for (int q = 1; q < Q; ++q) {
-        if ((pm >> q) & 1) {
-          const int t = nbr_slot(N, lx, ly, lz, kEX[q], kEY[q], kEZ[q]);
-          f[q] = load_pop<Q, S, R>(fin, ns, q, t);
-        } else {
-          f[q] = load_pop<Q, S, R>(fst, ns, q, s);
-        }
-      }
+      f[0] = row >= 0 ? load_pop<Q, S, R>(fst, ns, 0, s) : load_pop<Q, S, R>(fin, ns, 0, s);
+      // pull f_q(x) <- fcol_q(x + e_q); a missing source keeps the cell's own f_q
+      // (Stream_Manager.cpp:53-62).  The source is selected, not branched on, so
+      // all Q loads are issued back to back.
+      static_for<1, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const int t = nbr_slot_c<kEX[q], kEY[q], kEZ[q]>(N, lx, ly, lz);
+        const bool p = (pmx >> q) & 1;
+        const S* src = p ? fin + (int64_t)q * ns + t : fst + (int64_t)q * ns + s;
+        R v = (R)Store<S>::ld(src);
+        if (Store<S>::shifted) v += (R)wq<Q>(q);
+        f[q] = v;
+      });
       if (a.do_bc && (k & 0x3F00)) {
         const int32_t* bn = a.v.bnbr;
         const uint32_t* pmk = a.v.pmask;
@@ -334,17 +369,6 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs a) {
         };
         wet_bc<Q, R>(f, k, a.bc, nf);
       }
-      const int row = (a.do_ib && (k & MT_KIND_DIS)) ? a.dis_row[s] : -1;
-      if (row >= 0 && (k & MT_KIND_FLUID)) {
-        const double* drow = a.dis + (int64_t)row * a.Qm1;
-        const R rho_b = (R)ld_real<R>(a.v.rho_in, s);
-        auto fc = [&](int q, int dir) -> R {
-          int t = s;
-          if (dir) t = nbr_slot(N, lx, ly, lz, kEX[dir], kEY[dir], kEZ[dir]);
-          return load_pop<Q, S, R>(fin, ns, q, t);
-        };
-        ib_cell<Q, R>(f, drow, rho_b, pm, a.ib_method, fc, fx, fy, fz);
-      }
       if ((k & MT_KIND_C2F) || a.write_f) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) store_pop<Q, S, R>(fst, ns, q, s, f[q]);
@@ -352,23 +376,199 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs a) {
       if (k & MT_KIND_FLUID) {
         R rho, ux, uy, uz;
         cell_macros<Q, R>(f, rho, ux, uy, uz);
-        st_real<R>(a.v.rho_out, s, rho);
-        st_real<R>(a.v.u_out, s, ux);
-        st_real<R>(a.v.u_out, ns + s, uy);
-        st_real<R>(a.v.u_out, 2 * ns + s, uz);
+        // the density/velocity maps are read only by the solid boundary (its own
+        // cells, next sub-step), the reservoir NEE and the caller at the end
+        if (a.write_macros || row >= 0) {
+          st_real<R>(a.v.rho_out, s, rho);
+          st_real<R>(a.v.u_out, s, ux);
+          st_real<R>(a.v.u_out, ns + s, uy);
+          st_real<R>(a.v.u_out, 2 * ns + s, uz);
+        }
         collide_cell<Q, R>(f, rho, ux, uy, uz, (R)a.tau, (R)a.lesC, a.model);
       }
 #pragma unroll
       for (int q = 0; q < Q; ++q) store_pop<Q, S, R>(fout, ns, q, s, f[q]);
     }
   }
-  if (a.do_ib) {
-    block_reduce3(fx, fy, fz, red);
-    if (threadIdx.x == 0) {
-      a.force_part[3 * blockIdx.x] = fx;
-      a.force_part[3 * blockIdx.x + 1] = fy;
-      a.force_part[3 * blockIdx.x + 2] = fz;
+}
+
+// per-lane lattice tables for kernels that index by a lane-owned direction
+__constant__ int cEX[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, -1, 1, -1, 1};
+__constant__ int cEY[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, -1, 1, 1, -1};
+__constant__ int cEZ[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1, 1, -1, 1, -1};
+
+// sum over the 8 lanes of a group (xor butterfly: every lane gets the same value)
+template <typename T>
+__device__ __forceinline__ T group_sum8(T v) {
+  v += __shfl_xor(v, 1, 64);
+  v += __shfl_xor(v, 2, 64);
+  v += __shfl_xor(v, 4, 64);
+  return v;
+}
+
+// SinglePointInterpolation::treat for the fused schedule
+// (ComplexBoundary_Manager.cpp:63-225).  Eight lanes per treated cell, lane j
+// owning links j, j+8, j+16, j+24: wall links interpolate (FH or distance
+// weighted) between fcol_inv(q)(x) and fcol_q(x - e_q), other links pull
+// fcol_q(x + e_q) or bounce back; then the density clamp on f_0 and eight
+// multi-direct-forcing sweeps whose moments are group sums.  The result is the
+// cell's post-stream f, which the fused kernel collides.  Each block adds this
+// call's momentum-exchange force to its own running slot (fixed block -> cell
+// map); mt_ib_force sums the slots in block order.
+template <int Q, typename S>
+__global__ __launch_bounds__(256) void k_ib_links(LvArgs v, const int32_t* __restrict__ list, int64_t n,
+                                                  const int32_t* __restrict__ dis_row,
+                                                  const double* __restrict__ dis, int method, double* slots,
+                                                  double* acc) {
+  typedef typename RealOf<S>::type R;
+  constexpr int K = 4;  // links per lane
+  __shared__ double red[12];
+  const int grp = threadIdx.x >> 3, j = threadIdx.x & 7;
+  const int64_t i = (int64_t)blockIdx.x * 32 + grp;
+  const bool live = i < n;
+  const S* fin = (const S*)v.fcol_in;
+  const int64_t ns = v.ns;
+  int s = 0;
+  R fq[K];
+  bool wall[K];
+  double fx = 0, fy = 0, fz = 0;
+  uint32_t pm = 0;
+  int64_t drow = 0;
+  R rho_b = 0;
+  if (live) {
+    s = list[i];
+    pm = v.pmask[s];
+    drow = (int64_t)dis_row[s] * (Q - 1);
+    rho_b = ld_real<R>(v.rho_in, s);
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int q = j + 8 * k;
+    fq[k] = 0;
+    wall[k] = false;
+    if (!live || q >= Q) continue;
+    if (q == 0) {
+      fq[k] = load_pop<Q, S, R>(fin, ns, 0, s);
+      continue;
     }
+    const int ex = cEX[q], ey = cEY[q], ez = cEZ[q];
+    const int qi = qinv(q);
+    const double dist = dis[drow + qi - 1];
+    const R finc = load_pop<Q, S, R>(fin, ns, qi, s);
+    if (dist != -1.0) {
+      wall[k] = true;
+      const R fs = ((pm >> qi) & 1) ? load_pop<Q, S, R>(fin, ns, q, slot_off(v.bnbr, s, -ex, -ey, -ez)) : finc;
+      const R d = (R)dist;
+      R fn;
+      if (method == MT_IB_FH) {
+        if (d < (R)0.5) {
+          fn = ((R)1 - (R)2 * d) * finc + (R)2 * d * fs;
+        } else {
+          const int nq = (ex != 0) + (ey != 0) + (ez != 0);  // |e_qi| = |e_q|; D3Q19 table
+          const R w = qi < 19 ? (nq == 1 ? (R)(1. / 18.) : (R)(1. / 36.)) : (R)0;
+          const R i2d = (R)1 / ((R)2 * d);
+          fn = i2d * fs + ((R)1 - i2d) * (w * rho_b);
+        }
+      } else {
+        fn = ((R)1 - d) * finc + d * fs;
+      }
+      fq[k] = fn;
+      const double m = (double)finc + (double)fn;
+      fx -= m * ex;
+      fy -= m * ey;
+      fz -= m * ez;
+    } else {
+      fq[k] = ((pm >> q) & 1) ? load_pop<Q, S, R>(fin, ns, q, slot_off(v.bnbr, s, ex, ey, ez)) : finc;
+    }
+  }
+  // density clamp through f_0 (ComplexBoundary_Manager.cpp:150-158); f_0 lives in lane 0, k = 0
+  R loc = 0, mx = 0, my = 0, mz = 0;
+  R wxx = 0, wyy = 0, wzz = 0, wxy = 0, wxz = 0, wyz = 0, vx = 0, vy = 0, vz = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int q = j + 8 * k;
+    if (q >= Q) continue;
+    const R ex = (R)cEX[q], ey = (R)cEY[q], ez = (R)cEZ[q];
+    loc += fq[k];
+    mx += ex * fq[k];
+    my += ey * fq[k];
+    mz += ez * fq[k];
+    if (wall[k] && q < 19) {  // D3Q19 weights: zero beyond q = 18
+      const int nq = (cEX[q] != 0) + (cEY[q] != 0) + (cEZ[q] != 0);
+      const R w = nq == 1 ? (R)(1. / 18.) : (R)(1. / 36.);
+      wxx += w * ex * ex;
+      wyy += w * ey * ey;
+      wzz += w * ez * ez;
+      wxy += w * ex * ey;
+      wxz += w * ex * ez;
+      wyz += w * ey * ez;
+      vx += w * ex;
+      vy += w * ey;
+      vz += w * ez;
+    }
+  }
+  const R ra = group_sum8(loc);
+  R dclamp = 0;
+  if (ra < (R)0.3) dclamp = (R)1.0 * ((R)0.3 - ra);
+  else if (ra < (R)0.5) dclamp = (R)0.5 * ((R)0.5 - ra);
+  else if (ra > (R)3.0) dclamp = (R)1.0 * ((R)3.0 - ra);
+  else if (ra > (R)2.0) dclamp = (R)0.5 * ((R)2.0 - ra);
+  if (j == 0) fq[0] += dclamp;
+  // multi-direct forcing, 8 sweeps, relaxation 1/2 (:180-225).  A sweep adds
+  // df_q = 3 w_q e_q.dM (dM = -m/2) to the wall links and -sum df_q to f_0, so
+  // rho is invariant and m <- m + 3 W dM with W = sum_wall w_q e_q e_q^T: the
+  // sweeps reduce to a 3-vector recurrence, and each wall link finally receives
+  // 3 w_q e_q.D with D the sum of the sweeps' dM (f_0: -3 V.D, V = sum_wall w_q e_q).
+  {
+    const R rho = ra + dclamp;
+    R m0 = group_sum8(mx), m1 = group_sum8(my), m2 = group_sum8(mz);
+    wxx = group_sum8(wxx);
+    wyy = group_sum8(wyy);
+    wzz = group_sum8(wzz);
+    wxy = group_sum8(wxy);
+    wxz = group_sum8(wxz);
+    wyz = group_sum8(wyz);
+    vx = group_sum8(vx);
+    vy = group_sum8(vy);
+    vz = group_sum8(vz);
+    R D0 = 0, D1 = 0, D2 = 0;
+    if (rho > (R)0) {
+      const R ir = (R)1 / rho;
+      for (int it = 0; it < 8; ++it) {
+        const R ux = m0 * ir, uy = m1 * ir, uz = m2 * ir;
+        if (ux * ux + uy * uy + uz * uz < (R)1e-20) break;  // |u| < 1e-10: m no longer changes
+        const R d0 = (R)-0.5 * rho * ux, d1 = (R)-0.5 * rho * uy, d2 = (R)-0.5 * rho * uz;
+        D0 += d0;
+        D1 += d1;
+        D2 += d2;
+        m0 += (R)3.0 * (wxx * d0 + wxy * d1 + wxz * d2);
+        m1 += (R)3.0 * (wxy * d0 + wyy * d1 + wyz * d2);
+        m2 += (R)3.0 * (wxz * d0 + wyz * d1 + wzz * d2);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int q = j + 8 * k;
+      if (!wall[k] || q >= 19) continue;
+      const int nq = (cEX[q] != 0) + (cEY[q] != 0) + (cEZ[q] != 0);
+      const R w = nq == 1 ? (R)(1. / 18.) : (R)(1. / 36.);
+      fq[k] += w * (R)3.0 * ((R)cEX[q] * D0 + (R)cEY[q] * D1 + (R)cEZ[q] * D2);
+    }
+    if (j == 0) fq[0] -= (R)3.0 * (vx * D0 + vy * D1 + vz * D2);
+  }
+  if (live) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int q = j + 8 * k;
+      if (q < Q) store_pop<Q, S, R>((S*)v.f, ns, q, s, fq[k]);
+    }
+  }
+  block_reduce3(fx, fy, fz, red);
+  if (threadIdx.x == 0) {
+    slots[3 * blockIdx.x] += fx;
+    slots[3 * blockIdx.x + 1] += fy;
+    slots[3 * blockIdx.x + 2] += fz;
+    if (blockIdx.x == 0) acc[7] += 1.0;  // treat() calls folded into the slots
   }
 }
 
@@ -412,7 +612,7 @@ struct SeamArgs {
 
 // LBM_Manager::calculateMacros(level) (LBM_Manager.cpp:1003-1050)
 template <int Q, typename S>
-__global__ void k_macros(SeamArgs a) {
+__global__ __launch_bounds__(256) void k_macros(SeamArgs a) {
   typedef typename RealOf<S>::type R;
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.v.ns) return;
@@ -431,7 +631,7 @@ __global__ void k_macros(SeamArgs a) {
 // Collision_Model::collide(level): lat_f and F2C collide with the map moments,
 // C2F copy f -> fcol (Collision_Manager.cpp:41-52)
 template <int Q, typename S>
-__global__ void k_collide(SeamArgs a) {
+__global__ __launch_bounds__(256) void k_collide(SeamArgs a) {
   typedef typename RealOf<S>::type R;
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.v.ns) return;
@@ -452,7 +652,7 @@ __global__ void k_collide(SeamArgs a) {
 
 // ABPatternPull::stream(level) (Stream_Manager.cpp:43-76)
 template <int Q, typename S>
-__global__ void k_stream(SeamArgs a) {
+__global__ __launch_bounds__(256) void k_stream(SeamArgs a) {
   typedef typename RealOf<S>::type R;
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.v.ns) return;
@@ -471,7 +671,7 @@ __global__ void k_stream(SeamArgs a) {
 
 // level-0 wet-node boundaries applied to f after stream
 template <int Q, typename S>
-__global__ void k_bc_wet(SeamArgs a, BcCfg bc) {
+__global__ __launch_bounds__(256) void k_bc_wet(SeamArgs a, BcCfg bc) {
   typedef typename RealOf<S>::type R;
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.v.ns) return;
@@ -596,76 +796,118 @@ struct XferArgs {
   void* u_out;
   double tau, lesC;
   int model;
+  int write_macros;
 };
 
-// AMR_transDDF_explosion (LBM_Manager.cpp:1059-1129)
+// Transfer kernels: one block of 256 threads serves 8 parents.  Per-parent
+// arithmetic runs with lane q of a 32-lane group owning population q (the
+// parent's moments come from an LDS copy, every lane reading the same words);
+// global memory is touched (parent, child)-wise: a wave's 64 lanes are 8 parents
+// x 8 children, so for one q they access 8 runs of 8 consecutive slots.
+
+// AMR_transDDF_explosion (LBM_Manager.cpp:1059-1129): the parent's fcol is
+// rescaled around its He-Luo equilibrium and written to its (up to 8) children;
+// parents at index >= n_plain are ghost_F2C parents whose ghost_C2F targets are
+// mirrored into the second fcol buffer (the layer is read by both fine sub-steps).
 template <int Q, typename S>
-__global__ void k_explode(XferArgs a, int ghost) {
+__global__ __launch_bounds__(256) void k_explode(XferArgs a, int64_t n_plain) {
   typedef typename RealOf<S>::type R;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
-  const int p = a.list[i];
-  R fc[Q];
+  __shared__ R lf[8][32];
+  __shared__ R lv[8][32];
+  const int pl = threadIdx.x >> 5, q = threadIdx.x & 31;
+  const int64_t i = (int64_t)blockIdx.x * 8 + pl;
+  if (i < a.n && q < Q) lf[pl][q] = load_pop<Q, S, R>((const S*)a.src, a.ns_c, q, a.list[i]);
+  __syncthreads();
+  if (i < a.n && q < Q) {
+    R fc[Q];
 #pragma unroll
-  for (int q = 0; q < Q; ++q) fc[q] = load_pop<Q, S, R>((const S*)a.src, a.ns_c, q, p);
-  R rho, ux, uy, uz;
-  cell_macros<Q, R>(fc, rho, ux, uy, uz);
-  const R uu = ux * ux + uy * uy + uz * uz;
-  const R sc = (R)a.scale;
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
+    for (int r = 0; r < Q; ++r) fc[r] = lf[pl][r];
+    R rho, ux, uy, uz;
+    cell_macros<Q, R>(fc, rho, ux, uy, uz);
+    const R uu = ux * ux + uy * uy + uz * uz;
     const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)a.rho0);
-    fc[q] = fe + sc * (fc[q] - fe);
+    lv[pl][q] = fe + (R)a.scale * (lf[pl][q] - fe);
   }
-  for (int j = 0; j < 8; ++j) {
-    const int t = a.child[i * 8 + j];
-    if (t < 0) continue;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      store_pop<Q, S, R>((S*)a.dst, a.ns_f, q, t, fc[q]);
-      if (ghost && a.dst2) store_pop<Q, S, R>((S*)a.dst2, a.ns_f, q, t, fc[q]);
-    }
+  __syncthreads();
+  const int pair = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int pp = pair >> 3, j = pair & 7;
+  const int64_t ip = (int64_t)blockIdx.x * 8 + pp;
+  if (ip >= a.n) return;
+  const int dst = a.child[ip * 8 + j];
+  if (dst < 0) return;
+  const bool ghost = ip >= n_plain;
+  for (int r = w; r < Q; r += 4) {
+    store_pop<Q, S, R>((S*)a.dst, a.ns_f, r, dst, lv[pp][r]);
+    if (ghost) store_pop<Q, S, R>((S*)a.dst2, a.ns_f, r, dst, lv[pp][r]);
   }
 }
 
-// AMR_transDDF_coalescence (LBM_Manager.cpp:1131-1169); with collide set, the
-// F2C cell's next calculateMacros + collide follow in the same thread.
+// AMR_transDDF_coalescence (LBM_Manager.cpp:1131-1169): the children's
+// post-stream f are staged in LDS, lane q sums its population over c0..c7 in
+// order and divides by 8, the parent's He-Luo equilibrium follows from the LDS
+// average.  With collide set the F2C cell's next calculateMacros + collide run
+// here too: LBGK per lane, the cumulant by lane 0 of the parent.
 template <int Q, typename S>
 __global__ __launch_bounds__(256) void k_coalesce(XferArgs a) {
   typedef typename RealOf<S>::type R;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  __shared__ R lc[8][8][Q];
+  __shared__ R lf[8][32];
+  {
+    const int pair = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int pp = pair >> 3, j = pair & 7;
+    const int64_t ip = (int64_t)blockIdx.x * 8 + pp;
+    if (ip < a.n) {
+      const int ch = a.child[ip * 8 + j];
+      for (int r = w; r < Q; r += 4) lc[pp][j][r] = load_pop<Q, S, R>((const S*)a.src, a.ns_f, r, ch);
+    }
+  }
+  __syncthreads();
+  const int pl = threadIdx.x >> 5, q = threadIdx.x & 31;
+  const int64_t i = (int64_t)blockIdx.x * 8 + pl;
+  const bool live = i < a.n && q < Q;
+  if (live) {
+    R s = 0;
+    for (int j = 0; j < 8; ++j) s += lc[pl][j][q];
+    lf[pl][q] = s / (R)8.0;
+  }
+  __syncthreads();
+  if (!live) return;
   const int p = a.list[i];
-  int ch[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) ch[j] = a.child[i * 8 + j];
   R f[Q];
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    R s = 0;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) s += load_pop<Q, S, R>((const S*)a.src, a.ns_f, q, ch[j]);
-    f[q] = s / (R)8.0;
-  }
+  for (int r = 0; r < Q; ++r) f[r] = lf[pl][r];
   R rho, ux, uy, uz;
   cell_macros<Q, R>(f, rho, ux, uy, uz);
-  const R uu = ux * ux + uy * uy + uz * uz;
+  R uu = ux * ux + uy * uy + uz * uz;
   const R sc = (R)a.scale;
+  // this lane's population (q is a lane index: no register-array indexing by it)
+  const R fe_q = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)a.rho0);
+  const R fq = fe_q + sc * (lf[pl][q] - fe_q);
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)a.rho0);
-    f[q] = fe + sc * (f[q] - fe);
-    store_pop<Q, S, R>((S*)a.dst, a.ns_c, q, p, f[q]);
+  for (int r = 0; r < Q; ++r) {
+    const R fe = feq_hl<Q, R>(r, rho, ux, uy, uz, uu, (R)a.rho0);
+    f[r] = fe + sc * (f[r] - fe);
   }
-  if (a.collide) {
-    cell_macros<Q, R>(f, rho, ux, uy, uz);
+  store_pop<Q, S, R>((S*)a.dst, a.ns_c, q, p, fq);
+  if (!a.collide) return;
+  cell_macros<Q, R>(f, rho, ux, uy, uz);
+  if (q == 0 && a.write_macros) {
     st_real<R>(a.rho_out, p, rho);
     st_real<R>(a.u_out, p, ux);
     st_real<R>(a.u_out, a.ns_c + p, uy);
     st_real<R>(a.u_out, 2 * a.ns_c + p, uz);
-    collide_cell<Q, R>(f, rho, ux, uy, uz, (R)a.tau, (R)a.lesC, a.model);
+  }
+  R tau = (R)a.tau;
+  if ((R)a.lesC > (R)0) tau = smag_tau<Q, R>(f, rho, ux, uy, uz, tau, (R)a.lesC);
+  if (Q == 27 && a.model == MT_COLLIDE_CUMULANT) {
+    if (q != 0) return;
+    collide_cumulant<R>(f, rho, ux, uy, uz, tau);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)a.fcol_out, a.ns_c, q, p, f[q]);
+    for (int r = 0; r < Q; ++r) store_pop<Q, S, R>((S*)a.fcol_out, a.ns_c, r, p, f[r]);
+  } else {
+    uu = ux * ux + uy * uy + uz * uz;
+    const R fe = feq_c<Q, R>(q, rho, ux, uy, uz, uu);
+    store_pop<Q, S, R>((S*)a.fcol_out, a.ns_c, q, p, fq - (fq - fe) * ((R)1 / tau));
   }
 }
 
@@ -829,13 +1071,13 @@ static SeamArgs seam(Case& c, int L) {
 
 int launch_macros(Case& c, int L) {
   SeamArgs a = seam(c, L);
-  MT_DISPATCH(c, (k_macros<Q_, S_><<<nblk(a.v.ns), 256, 0, c.stream>>>(a)));
+  MT_DISPATCH(c, (k_macros<Q_, S_><<<nblk(a.v.ns), 256, 0, c.st[L]>>>(a)));
   MT_HIP(hipGetLastError());
   return MT_OK;
 }
 int launch_collide(Case& c, int L) {
   SeamArgs a = seam(c, L);
-  MT_DISPATCH(c, (k_collide<Q_, S_><<<nblk(a.v.ns), 256, 0, c.stream>>>(a)));
+  MT_DISPATCH(c, (k_collide<Q_, S_><<<nblk(a.v.ns), 256, 0, c.st[L]>>>(a)));
   MT_HIP(hipGetLastError());
   return MT_OK;
 }
@@ -889,6 +1131,7 @@ int launch_explode(Case& c, int L) {
   if (L <= 0) return MT_OK;
   DevLevel& lc = c.lv[L - 1];
   DevLevel& lf = c.lv[L];
+  if (!lc.n_xp) return MT_OK;
   XferArgs a;
   a.ns_c = lc.n_slots;
   a.ns_f = lf.n_slots;
@@ -899,23 +1142,15 @@ int launch_explode(Case& c, int L) {
                                                    : (2.0 * c.tau[L]) / c.tau[L - 1];
   a.rho0 = c.d.rho0;
   a.collide = 0;
-  if (lc.n_f2c) {
-    a.list = lc.f2c;
-    a.n = lc.n_f2c;
-    a.child = lc.f2c_child_write;
-    MT_DISPATCH(c, (k_explode<Q_, S_><<<nblk(a.n), 256, 0, c.stream>>>(a, 0)));
-  }
-  if (lc.n_gf2c) {
-    a.list = lc.gf2c;
-    a.n = lc.n_gf2c;
-    a.child = lc.gf2c_child;
-    MT_DISPATCH(c, (k_explode<Q_, S_><<<nblk(a.n), 256, 0, c.stream>>>(a, 1)));
-  }
+  a.list = lc.xp_list;
+  a.n = lc.n_xp;
+  a.child = lc.xp_child;
+  MT_DISPATCH(c, (k_explode<Q_, S_><<<(a.n + 7) / 8, 256, 0, c.st[L]>>>(a, lc.n_f2c)));
   MT_HIP(hipGetLastError());
   return MT_OK;
 }
 
-int launch_coalesce(Case& c, int L, bool collide_after) {
+int launch_coalesce(Case& c, int L, bool collide_after, bool write_macros) {
   if (L >= c.Lmax) return MT_OK;
   DevLevel& lc = c.lv[L];
   DevLevel& lf = c.lv[L + 1];
@@ -933,18 +1168,19 @@ int launch_coalesce(Case& c, int L, bool collide_after) {
                                                    : c.tau[L] / (2.0 * c.tau[L + 1]);
   a.rho0 = c.d.rho0;
   a.collide = collide_after ? 1 : 0;
+  a.write_macros = write_macros ? 1 : 0;
   a.fcol_out = lc.fcol[1 - lc.pf];
   a.rho_out = lc.rho[1 - lc.pr];
   a.u_out = lc.u[1 - lc.pr];
   a.tau = c.tau[L];
   a.lesC = c.d.les_C;
   a.model = c.d.collision;
-  MT_DISPATCH(c, (k_coalesce<Q_, S_><<<nblk(a.n), 256, 0, c.stream>>>(a)));
+  MT_DISPATCH(c, (k_coalesce<Q_, S_><<<(a.n + 7) / 8, 256, 0, c.st[L]>>>(a)));
   MT_HIP(hipGetLastError());
   return MT_OK;
 }
 
-int launch_fused(Case& c, int L, bool write_f, bool nee_after) {
+int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros) {
   DevLevel& l = c.lv[L];
   FusedArgs a;
   a.v = lvargs(c, L, l.pf, 1 - l.pf, l.pr, 1 - l.pr);
@@ -961,12 +1197,19 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after) {
   a.dis_row = l.dis_row;
   a.dis = l.dis;
   a.Qm1 = c.Q - 1;
-  a.force_part = c.force_part;
-  int64_t nb = (l.n_active + 3) / 4;
-  if (nb < 1) nb = 1;
-  if (a.do_ib && nb > c.force_part_cap) {
-    set_error("force partial buffer too small");
-    return MT_ERR_STATE;
+  a.write_macros = write_macros ? 1 : 0;
+  // grid: a multiple of 8 blocks so the XCD remap covers every brick index
+  int64_t nb = ((l.n_active + 7) / 8) * 8;
+  if (nb < 8) nb = 8;
+  if (a.do_ib && l.n_ibl) {
+    // the solid boundary first: it reads only fcol / rho of this sub-step
+    const int64_t nbi = (l.n_ibl + 31) / 32;
+    if (nbi > c.force_part_cap) {
+      set_error("force slot buffer too small");
+      return MT_ERR_STATE;
+    }
+    MT_DISPATCH(c, (k_ib_links<Q_, S_><<<nbi, 256, 0, c.st[L]>>>(a.v, l.ibl, l.n_ibl, l.dis_row, l.dis, c.d.ib,
+                                                                   c.force_slots, c.force_acc)));
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c.timing) {
@@ -981,15 +1224,14 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after) {
     c.ev_pool.pop_back();
     e1 = c.ev_pool.back();
     c.ev_pool.pop_back();
-    MT_HIP(hipEventRecord(e0, c.stream));
+    MT_HIP(hipEventRecord(e0, c.st[L]));
   }
-  MT_DISPATCH(c, (k_fused<Q_, S_><<<nb, 256, 0, c.stream>>>(a)));
+  MT_DISPATCH(c, (k_fused<Q_, S_><<<nb, 64, 0, c.st[L]>>>(a)));
   MT_HIP(hipGetLastError());
   if (c.timing) {
-    MT_HIP(hipEventRecord(e1, c.stream));
+    MT_HIP(hipEventRecord(e1, c.st[L]));
     c.ev_pending.push_back({L, {e0, e1}});
   }
-  if (a.do_ib) k_force_reduce<<<1, 256, 0, c.stream>>>(c.force_part, nb, c.force_acc);
   if (nee_after) {
     // reservoir NEE reads the pre-sub-step maps (rho/u[pr]) and the f just written
     int r = launch_nee(c);

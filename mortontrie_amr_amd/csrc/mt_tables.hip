@@ -290,6 +290,11 @@ __global__ void k_flag_kind(const uint16_t* kind, int64_t ns, uint16_t mask, int
   if (s >= ns) return;
   flag[s] = (kind[s] & mask) != 0;
 }
+__global__ void k_flag_ib(const uint16_t* kind, const int32_t* dis_row, int64_t ns, int* flag) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  flag[s] = (kind[s] & MT_KIND_DIS) && (kind[s] & MT_KIND_FLUID) && dis_row[s] >= 0;
+}
 __global__ void k_flag_brick(const uint16_t* kind, int64_t nb, uint16_t mask, int* flag) {
   int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
@@ -494,6 +499,11 @@ int build_tables(Case& c) {
     k_flag_kind<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.kind, l.n_slots, MT_KIND_BCRES, flag);
     r = compact_list(c, flag, l.n_slots, &l.bcres, &l.n_bcres);
     if (r) return r;
+    if (L == c.Lmax) {
+      k_flag_ib<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.kind, l.dis_row, l.n_slots, flag);
+      r = compact_list(c, flag, l.n_slots, &l.ibl, &l.n_ibl);
+      if (r) return r;
+    }
     MT_HIP(hipStreamSynchronize(c.stream));
     MT_HIP(hipFree(flag));
     if (L < c.Lmax && (l.n_f2c || l.n_gf2c)) {
@@ -509,6 +519,18 @@ int build_tables(Case& c) {
         k_children<<<nblk(l.n_gf2c), 256, 0, c.stream>>>(l.gf2c, l.n_gf2c, l.key, pshift, fine,
                                                          c.lv[L + 1].kind, d_dis_sorted, nd, 1, nullptr,
                                                          l.gf2c_child, d_err);
+      l.n_xp = l.n_f2c + l.n_gf2c;
+      l.xp_list = c.alloc<int32_t>(l.n_xp);
+      l.xp_child = c.alloc<int32_t>(l.n_xp * 8);
+      if (l.n_f2c) {
+        MT_HIP(hipMemcpyAsync(l.xp_list, l.f2c, 4 * l.n_f2c, hipMemcpyDeviceToDevice, c.stream));
+        MT_HIP(hipMemcpyAsync(l.xp_child, l.f2c_child_write, 32 * l.n_f2c, hipMemcpyDeviceToDevice, c.stream));
+      }
+      if (l.n_gf2c) {
+        MT_HIP(hipMemcpyAsync(l.xp_list + l.n_f2c, l.gf2c, 4 * l.n_gf2c, hipMemcpyDeviceToDevice, c.stream));
+        MT_HIP(hipMemcpyAsync(l.xp_child + 8 * l.n_f2c, l.gf2c_child, 32 * l.n_gf2c, hipMemcpyDeviceToDevice,
+                              c.stream));
+      }
     } else if (L == c.Lmax && l.n_f2c) {
       set_error("F2C cells on the finest level");
       return MT_ERR_ARG;
