@@ -63,6 +63,41 @@ def run_gpu(case, steps, warmup, precision, timing_steps):
     return (t1 - t0), ms, calls, nan, cd
 
 
+def counter_run(case, steps, warmup, copy_mib):
+    """Short eager run for hardware-counter passes: `steps` coarse steps with one
+    dispatch per kernel (no graphs), then a plain device copy of `copy_mib` MiB
+    whose counter readings calibrate bytes per counter unit."""
+    import torch
+    from mortontrie_amr_amd import amr
+    g = amr.AmrCase(case, precision=amr.FP64, graphs=False)
+    g.step(warmup)
+    g.sync()
+    g.step(steps)
+    g.sync()
+    g.close()
+    n = copy_mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"counter_run": True, "steps": steps, "warmup": warmup, "copy_bytes": n * 8,
+                      "fused_grid_threads": [((nb + 7) // 8) * 8 * 64 for nb in active_bricks(case)]}), flush=True)
+
+
+def active_bricks(case):
+    """Bricks (4x4x4 Morton blocks) holding fluid or C2F cells, per level: the
+    fused kernel's grid is one 64-lane block per such brick, rounded up to 8."""
+    from mortontrie_amr_amd import cases as C
+    out = []
+    for L, lv in enumerate(case["levels"]):
+        k = lv["keys"][(lv["kinds"] & (C.KIND_FLUID | C.KIND_C2F)) != 0]
+        x, y, z = C.decode_key(k, L, case["Lmax"])
+        b = (np.asarray(x) // 4) | ((np.asarray(y) // 4) << 20) | ((np.asarray(z) // 4) << 40)
+        out.append(int(np.unique(b).size))
+    return out
+
+
 def cpu_baseline(case, steps):
     from oracle import pyoracle as O
     o = O.OracleCase(case)
@@ -81,6 +116,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--counters", action="store_true",
+                    help="short eager run plus a calibration copy, for rocprofv3 --pmc passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the update runs on a "
@@ -88,9 +125,10 @@ def main():
         sys.exit(2)
 
     from mortontrie_amr_amd import cases as C, amr
-    t_build = time.perf_counter()
     case = C.sphere_amr(options=2)
-    t_build = time.perf_counter() - t_build
+    if args.counters:
+        counter_run(case, 4, 5, 256)
+        return
     per_level = level_updates(case)
     upd_step = sum(n << L for L, n in enumerate(per_level))
     Q = case["Q"]

@@ -30,7 +30,7 @@ void set_error(const std::string& s) { g_err = s; }
     if (r_ != MT_OK) return r_;   \
   } while (0)
 
-static int run_level(Case& c, int L, bool last);
+static int run_level(Case& c, int L, bool last, const CoalReq* coal);
 
 static bool has_reservoir(const Case& c) {
   for (int f = 0; f < 6; ++f)
@@ -69,34 +69,49 @@ static int prologue(Case& c) {
 // sub-tree runs concurrently with level L's explosion and fused update; only
 // the coalescence, which reads the finer level's final f, joins them.  Issue
 // order is the reference's (finer levels, explosion, stream, coalescence).
-static int substep(Case& c, int L, bool child_last, bool write_f, bool nee, bool write_macros, bool explode) {
+// One sub-step of level L.  explode: 0 none, 1 the reference's explosion,
+// 2 explosion whose targets also fill the second buffer.  coal: the coalescence
+// into level L-1 that this sub-step's fused update performs (last sub-step of
+// the pair), or null.
+static int substep(Case& c, int L, bool child_last, bool write_f, bool nee, bool write_macros, int explode,
+                   bool c2f_fcol, const CoalReq* coal) {
   DevLevel& l = c.lv[L];
   hipStream_t s = c.st[L];
+  const bool ib = level_has_ib(c, L);
   if (L < c.Lmax) {
     MT_HIP(hipEventRecord(c.ev_ready[L], s));
     MT_HIP(hipStreamWaitEvent(c.st[L + 1], c.ev_ready[L], 0));
-    TRY(run_level(c, L + 1, child_last));
+    const CoalReq cr = {write_macros ? 1 : 0};
+    TRY(run_level(c, L + 1, child_last, l.coal_in_fused ? &cr : nullptr));
   }
-  if (explode) TRY(launch_explode(c, L));
-  TRY(launch_fused(c, L, write_f, nee, write_macros));
+  if (explode) TRY(launch_explode(c, L, explode == 2));
+  // the treated cells first (stream + links + collision): they read only this
+  // sub-step's fcol and rho, and the fused update skips them
+  if (ib) TRY(launch_ib_links(c, L, s));
+  TRY(launch_fused(c, L, write_f, nee, write_macros, c2f_fcol, coal));
   if (L < c.Lmax) {
     MT_HIP(hipEventRecord(c.ev_done[L + 1], c.st[L + 1]));
     MT_HIP(hipStreamWaitEvent(s, c.ev_done[L + 1], 0));
-    TRY(launch_coalesce(c, L, true, write_macros));
+    if (!l.coal_in_fused) TRY(launch_coalesce(c, L, true, write_macros));
   }
   l.pf ^= 1;
   l.pr ^= 1;
   return MT_OK;
 }
-static int run_level(Case& c, int L, bool last) {
+static int run_level(Case& c, int L, bool last, const CoalReq* coal) {
   const bool res0 = (L == 0) && has_reservoir(c);
   if (L == 0) {
     // level 0 has one sub-step per coarse step
-    return substep(c, 0, last, last || res0, res0, last || res0, false);
+    return substep(c, 0, last, last || res0, res0, last || res0, 0, true, nullptr);
   }
-  TRY(substep(c, L, false, false, false, false, true));
-  // sub-step B: the reference does not explode here (LBM_Simulator.cpp:152-217)
-  return substep(c, L, last, last, false, last, (c.d.options & MT_OPT_EXPLODE_BOTH) != 0);
+  // Sub-step B: the reference does not explode (LBM_Simulator.cpp:152-217).
+  // With MT_OPT_EXPLODE_BOTH it explodes again from the same parent fcol (the
+  // parent level streams only after both sub-steps), i.e. it rewrites the values
+  // of sub-step A's explosion: A's explosion fills both buffers instead, and the
+  // fused updates skip the C2F cells' fcol (always overwritten before a read).
+  const bool both = (c.d.options & MT_OPT_EXPLODE_BOTH) != 0;
+  TRY(substep(c, L, false, false, false, false, both ? 2 : 1, !both, nullptr));
+  return substep(c, L, last, last, false, last, 0, !both, coal);
 }
 
 static void use_level_streams(Case& c, bool on) {
@@ -110,7 +125,7 @@ static int parity_key(const Case& c, bool last) {
 }
 
 static int coarse_step(Case& c, bool last) {
-  if (!c.graphs || c.timing) return run_level(c, 0, last);
+  if (!c.graphs || c.timing) return run_level(c, 0, last, nullptr);
   int key = parity_key(c, last);
   auto it = c.graph_exec.find(key);
   int pf[kMaxLevels], pr[kMaxLevels];
@@ -118,7 +133,7 @@ static int coarse_step(Case& c, bool last) {
   if (it == c.graph_exec.end()) {
     hipGraph_t g;
     MT_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-    int r = run_level(c, 0, last);
+    int r = run_level(c, 0, last, nullptr);
     hipError_t e = hipStreamEndCapture(c.stream, &g);
     if (r) return r;
     if (e != hipSuccess) { set_error(std::string("graph capture: ") + hipGetErrorString(e)); return MT_ERR_HIP; }
@@ -195,6 +210,7 @@ int mt_case_create(const mt_case_desc* d, mt_case** out) {
     e = hipEventCreateWithFlags(&c->ev_ready[L], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_done[L], hipEventDisableTiming);
   }
+
   if (e != hipSuccess) {
     set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
     mt_case_destroy(c);

@@ -302,46 +302,65 @@ struct FusedArgs {
   int do_bc;
   BcCfg bc;
   int do_ib;
-  int ib_method;
   const int32_t* __restrict__ dis_row;
-  const double* __restrict__ dis;
-  int Qm1;
   int write_macros;
+  int c2f_fcol;  // store the C2F cells' fcol (0: an explosion overwrites it before any read)
+  int c2f_f;     // store the C2F cells' f (read by a separate coalescence kernel)
+  // coalescence into the next coarser level (k_fused<.., true>)
+  const int32_t* __restrict__ cpar;  // [n_slots / 8] parent slot of an aligned group of 8, -1 none
+  int64_t ns_c;
+  void* cf;         // coarse f
+  void* cfcol_out;  // coarse fcol after the F2C cell's collision
+  void* crho_out;
+  void* cu_out;
+  double ctau, cscale, rho0;
+  int c_write_macros;
 };
 
 // One 64-lane block per brick.  Workgroups are dealt round-robin over the 8
 // XCDs, each with its own L2; the block -> brick map below gives every XCD one
 // contiguous Morton range of bricks, so a brick's neighbour bricks (its halo)
 // are mostly served by the same L2.
-template <int Q, typename S>
+//
+// With COAL the block also coalesces (AMR_transDDF_coalescence,
+// LBM_Manager.cpp:1131-1169) into the F2C parents whose children it holds:
+// children of a parent are the 8 lanes of an aligned group, in the reference's
+// child order, so their post-stream f are summed in that order with lane
+// shuffles; the group's first lane rescales the average around its He-Luo
+// equilibrium and runs the F2C cell's calculateMacros + collide, which the
+// reference does at the start of the coarse level's next sub-step.
+template <int Q, typename S, bool COAL>
 __global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
   typedef typename RealOf<S>::type R;
   const int lane = threadIdx.x;
   const int64_t nb = gridDim.x;
   const int64_t per_xcd = (nb + 7) >> 3;
   const int64_t bi = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-  double fx = 0, fy = 0, fz = 0;
   const S* fin = (const S*)a.v.fcol_in;
   S* fout = (S*)a.v.fcol_out;
   S* fst = (S*)a.v.f;
   const int64_t ns = a.v.ns;
-  if (bi < a.n_active) {
-    const int b = __builtin_amdgcn_readfirstlane(a.active[bi]);
+  if (bi >= a.n_active) return;  // uniform over the block
+  const int b = __builtin_amdgcn_readfirstlane(a.active[bi]);
+  R f[Q];
+  if (COAL) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = 0;
+  }
+  {
     int N[27];
 #pragma unroll
     for (int o = 0; o < 27; ++o) N[o] = __builtin_amdgcn_readfirstlane(a.v.bnbr[(int64_t)b * 27 + o]);
     const int s = (b << 6) | lane;
     const uint16_t k = a.v.kind[s];
-    if (k & (MT_KIND_FLUID | MT_KIND_C2F)) {
+    // treated solid-boundary cells are streamed, treated and collided by
+    // k_ib_links
+    const bool treated = a.do_ib && (k & MT_KIND_DIS) && (k & MT_KIND_FLUID) && a.dis_row[s] >= 0;
+    if ((k & (MT_KIND_FLUID | MT_KIND_C2F)) && !treated) {
       int lx, ly, lz;
       brick_coords(lane, lx, ly, lz);
-      const uint32_t pm = a.v.pmask[s];
-      // a treated solid-boundary cell already holds its post-stream f (written by
-      // k_ib_links for this sub-step): read it like a cell with no sources
-      const int row = (a.do_ib && (k & MT_KIND_DIS) && (k & MT_KIND_FLUID)) ? a.dis_row[s] : -1;
-      const uint32_t pmx = row >= 0 ? 0u : pm;
-      R f[Q];
-      f[0] = row >= 0 ? load_pop<Q, S, R>(fst, ns, 0, s) : load_pop<Q, S, R>(fin, ns, 0, s);
+      const uint32_t pmx = a.v.pmask[s];
+      f[0] = load_pop<Q, S, R>(fin, ns, 0, s);
       // pull f_q(x) <- fcol_q(x + e_q); a missing source keeps the cell's own f_q
       // (Stream_Manager.cpp:53-62).  The source is selected, not branched on, so
       // all Q loads are issued back to back.
@@ -369,27 +388,71 @@ __global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
         };
         wet_bc<Q, R>(f, k, a.bc, nf);
       }
-      if ((k & MT_KIND_C2F) || a.write_f) {
+      // A C2F cell's stored f is read by a separate coalescence kernel and, for
+      // the directions with no source, by its own next pull; those entries never
+      // change, so with the coalescence done here only the final f is stored.
+      if (((k & MT_KIND_C2F) && a.c2f_f) || a.write_f) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) store_pop<Q, S, R>(fst, ns, q, s, f[q]);
       }
       if (k & MT_KIND_FLUID) {
+        // (in place: lat_f cells are never children in a coalescence group)
         R rho, ux, uy, uz;
         cell_macros<Q, R>(f, rho, ux, uy, uz);
         // the density/velocity maps are read only by the solid boundary (its own
         // cells, next sub-step), the reservoir NEE and the caller at the end
-        if (a.write_macros || row >= 0) {
+        if (a.write_macros) {
           st_real<R>(a.v.rho_out, s, rho);
           st_real<R>(a.v.u_out, s, ux);
           st_real<R>(a.v.u_out, ns + s, uy);
           st_real<R>(a.v.u_out, 2 * ns + s, uz);
         }
         collide_cell<Q, R>(f, rho, ux, uy, uz, (R)a.tau, (R)a.lesC, a.model);
-      }
 #pragma unroll
-      for (int q = 0; q < Q; ++q) store_pop<Q, S, R>(fout, ns, q, s, f[q]);
+        for (int q = 0; q < Q; ++q) store_pop<Q, S, R>(fout, ns, q, s, f[q]);
+      } else if (a.c2f_fcol) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) store_pop<Q, S, R>(fout, ns, q, s, f[q]);
+      }
     }
   }
+  if (!COAL) return;
+  const int par = a.cpar[(b << 3) | (lane >> 3)];
+  if (__ballot(par >= 0) == 0) return;  // uniform over the wave
+  // every lane of a group with a parent is a streamed C2F cell holding its
+  // post-stream f; the sums run over all 64 lanes
+  const int b0 = lane & ~7;
+  __shared__ R lavg[8][Q];
+  for (int q = 0; q < Q; ++q) {
+    R sum = __shfl(f[q], b0, 64);
+#pragma unroll
+    for (int t = 1; t < 8; ++t) sum += __shfl(f[q], b0 + t, 64);
+    if ((lane & 7) == 0) lavg[lane >> 3][q] = sum / (R)8.0;
+  }
+  if ((lane & 7) || par < 0) return;
+  R avg[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) avg[q] = lavg[lane >> 3][q];
+  R rho, ux, uy, uz;
+  cell_macros<Q, R>(avg, rho, ux, uy, uz);
+  const R uu = ux * ux + uy * uy + uz * uz;
+  const R sc = (R)a.cscale;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)a.rho0);
+    avg[q] = fe + sc * (avg[q] - fe);
+    store_pop<Q, S, R>((S*)a.cf, a.ns_c, q, par, avg[q]);
+  }
+  cell_macros<Q, R>(avg, rho, ux, uy, uz);
+  if (a.c_write_macros) {
+    st_real<R>(a.crho_out, par, rho);
+    st_real<R>(a.cu_out, par, ux);
+    st_real<R>(a.cu_out, a.ns_c + par, uy);
+    st_real<R>(a.cu_out, 2 * a.ns_c + par, uz);
+  }
+  collide_cell<Q, R>(avg, rho, ux, uy, uz, (R)a.ctau, (R)a.lesC, a.model);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)a.cfcol_out, a.ns_c, q, par, avg[q]);
 }
 
 // per-lane lattice tables for kernels that index by a lane-owned direction
@@ -412,14 +475,15 @@ __device__ __forceinline__ T group_sum8(T v) {
 // weighted) between fcol_inv(q)(x) and fcol_q(x - e_q), other links pull
 // fcol_q(x + e_q) or bounce back; then the density clamp on f_0 and eight
 // multi-direct-forcing sweeps whose moments are group sums.  The result is the
-// cell's post-stream f, which the fused kernel collides.  Each block adds this
-// call's momentum-exchange force to its own running slot (fixed block -> cell
-// map); mt_ib_force sums the slots in block order.
-template <int Q, typename S>
+// cell's post-stream f; with COLLIDE its calculateMacros + collide follow here
+// (the fused kernel skips treated cells).  Each block adds this call's
+// momentum-exchange force to its own running slot (fixed block -> cell map);
+// mt_ib_force sums the slots in block order.
+template <int Q, typename S, bool COLLIDE>
 __global__ __launch_bounds__(256) void k_ib_links(LvArgs v, const int32_t* __restrict__ list, int64_t n,
-                                                  const int32_t* __restrict__ dis_row,
-                                                  const double* __restrict__ dis, int method, double* slots,
-                                                  double* acc) {
+                                                  const int32_t* __restrict__ tsrc,
+                                                  const double* __restrict__ tdist, int method, double* slots,
+                                                  double* acc, double tau, double lesC, int model) {
   typedef typename RealOf<S>::type R;
   constexpr int K = 4;  // links per lane
   __shared__ double red[12];
@@ -432,13 +496,9 @@ __global__ __launch_bounds__(256) void k_ib_links(LvArgs v, const int32_t* __res
   R fq[K];
   bool wall[K];
   double fx = 0, fy = 0, fz = 0;
-  uint32_t pm = 0;
-  int64_t drow = 0;
   R rho_b = 0;
   if (live) {
     s = list[i];
-    pm = v.pmask[s];
-    drow = (int64_t)dis_row[s] * (Q - 1);
     rho_b = ld_real<R>(v.rho_in, s);
   }
 #pragma unroll
@@ -447,17 +507,21 @@ __global__ __launch_bounds__(256) void k_ib_links(LvArgs v, const int32_t* __res
     fq[k] = 0;
     wall[k] = false;
     if (!live || q >= Q) continue;
+    // the link's source element (table built once per case) and wall fraction
+    const int32_t e = tsrc[i * 32 + q];
+    R src = (R)Store<S>::ld(fin + e);
+    if (Store<S>::shifted) src += (R)wq<Q>(q);  // w_inv(q) = w_q
     if (q == 0) {
-      fq[k] = load_pop<Q, S, R>(fin, ns, 0, s);
+      fq[k] = src;
       continue;
     }
+    const double dist = tdist[i * 32 + q];
     const int ex = cEX[q], ey = cEY[q], ez = cEZ[q];
     const int qi = qinv(q);
-    const double dist = dis[drow + qi - 1];
-    const R finc = load_pop<Q, S, R>(fin, ns, qi, s);
     if (dist != -1.0) {
+      const R finc = load_pop<Q, S, R>(fin, ns, qi, s);
       wall[k] = true;
-      const R fs = ((pm >> qi) & 1) ? load_pop<Q, S, R>(fin, ns, q, slot_off(v.bnbr, s, -ex, -ey, -ez)) : finc;
+      const R fs = src;
       const R d = (R)dist;
       R fn;
       if (method == MT_IB_FH) {
@@ -478,7 +542,7 @@ __global__ __launch_bounds__(256) void k_ib_links(LvArgs v, const int32_t* __res
       fy -= m * ey;
       fz -= m * ez;
     } else {
-      fq[k] = ((pm >> q) & 1) ? load_pop<Q, S, R>(fin, ns, q, slot_off(v.bnbr, s, ex, ey, ez)) : finc;
+      fq[k] = src;
     }
   }
   // density clamp through f_0 (ComplexBoundary_Manager.cpp:150-158); f_0 lives in lane 0, k = 0
@@ -556,11 +620,15 @@ __global__ __launch_bounds__(256) void k_ib_links(LvArgs v, const int32_t* __res
     }
     if (j == 0) fq[0] -= (R)3.0 * (vx * D0 + vy * D1 + vz * D2);
   }
+  __shared__ R tf[COLLIDE ? 32 : 1][Q];
   if (live) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int q = j + 8 * k;
-      if (q < Q) store_pop<Q, S, R>((S*)v.f, ns, q, s, fq[k]);
+      if (q < Q) {
+        store_pop<Q, S, R>((S*)v.f, ns, q, s, fq[k]);
+        if (COLLIDE) tf[grp][q] = fq[k];
+      }
     }
   }
   block_reduce3(fx, fy, fz, red);
@@ -570,6 +638,26 @@ __global__ __launch_bounds__(256) void k_ib_links(LvArgs v, const int32_t* __res
     slots[3 * blockIdx.x + 2] += fz;
     if (blockIdx.x == 0) acc[7] += 1.0;  // treat() calls folded into the slots
   }
+  if (!COLLIDE) return;
+  // calculateMacros + collide of the block's 32 treated cells, one lane each;
+  // the density/velocity maps are written every sub-step (the next sub-step's
+  // link treatment reads rho)
+  __syncthreads();
+  const int64_t ic = (int64_t)blockIdx.x * 32 + threadIdx.x;
+  if (threadIdx.x >= 32 || ic >= n) return;
+  const int sc = list[ic];
+  R f[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) f[q] = tf[threadIdx.x][q];
+  R rho, ux, uy, uz;
+  cell_macros<Q, R>(f, rho, ux, uy, uz);
+  st_real<R>(v.rho_out, sc, rho);
+  st_real<R>(v.u_out, sc, ux);
+  st_real<R>(v.u_out, ns + sc, uy);
+  st_real<R>(v.u_out, 2 * ns + sc, uz);
+  collide_cell<Q, R>(f, rho, ux, uy, uz, (R)tau, (R)lesC, model);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)v.fcol_out, ns, q, sc, f[q]);
 }
 
 // fixed-order sum of the per-block IB force partials into the accumulator
@@ -799,11 +887,11 @@ struct XferArgs {
   int write_macros;
 };
 
-// Transfer kernels: one block of 256 threads serves 8 parents.  Per-parent
-// arithmetic runs with lane q of a 32-lane group owning population q (the
-// parent's moments come from an LDS copy, every lane reading the same words);
-// global memory is touched (parent, child)-wise: a wave's 64 lanes are 8 parents
-// x 8 children, so for one q they access 8 runs of 8 consecutive slots.
+// Explosion: one block of 256 threads serves 8 parents.  Per-parent arithmetic
+// runs with lane q of a 32-lane group owning population q (the parent's moments
+// come from an LDS copy, every lane reading the same words); global memory is
+// touched (parent, child)-wise: a wave's 64 lanes are 8 parents x 8 children,
+// so for one q they access 8 runs of 8 consecutive slots.
 
 // AMR_transDDF_explosion (LBM_Manager.cpp:1059-1129): the parent's fcol is
 // rescaled around its He-Luo equilibrium and written to its (up to 8) children;
@@ -842,73 +930,64 @@ __global__ __launch_bounds__(256) void k_explode(XferArgs a, int64_t n_plain) {
   }
 }
 
-// AMR_transDDF_coalescence (LBM_Manager.cpp:1131-1169): the children's
-// post-stream f are staged in LDS, lane q sums its population over c0..c7 in
-// order and divides by 8, the parent's He-Luo equilibrium follows from the LDS
-// average.  With collide set the F2C cell's next calculateMacros + collide run
-// here too: LBGK per lane, the cumulant by lane 0 of the parent.
+// AMR_transDDF_coalescence (LBM_Manager.cpp:1131-1169), 16 parents per 64-lane
+// block.  For a group of 8 parents the lanes are (parent, child): one load per
+// population reads all 64 children (8 runs of 8 consecutive slots), and the 8
+// children of a parent are summed in child order with lane shuffles and divided
+// by 8.  Lane p < 16 then rescales its parent around the He-Luo equilibrium of
+// the average and, with collide set, runs the F2C cell's next calculateMacros +
+// collide, so the collision uses all 64 lanes' worth of parents in one wave.
 template <int Q, typename S>
-__global__ __launch_bounds__(256) void k_coalesce(XferArgs a) {
+__global__ __launch_bounds__(64) void k_coalesce(XferArgs a) {
   typedef typename RealOf<S>::type R;
-  __shared__ R lc[8][8][Q];
-  __shared__ R lf[8][32];
-  {
-    const int pair = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int pp = pair >> 3, j = pair & 7;
-    const int64_t ip = (int64_t)blockIdx.x * 8 + pp;
-    if (ip < a.n) {
-      const int ch = a.child[ip * 8 + j];
-      for (int r = w; r < Q; r += 4) lc[pp][j][r] = load_pop<Q, S, R>((const S*)a.src, a.ns_f, r, ch);
+  constexpr int P = 16;
+  __shared__ R ls[P][Q];
+  const int lane = threadIdx.x, pp = lane >> 3, j = lane & 7, b0 = lane & ~7;
+  const int64_t base = (int64_t)blockIdx.x * P;
+#pragma unroll
+  for (int g = 0; g < P / 8; ++g) {
+    const int64_t ip = base + g * 8 + pp;
+    const bool ok = ip < a.n;
+    const int ch = ok ? a.child[ip * 8 + j] : 0;
+    R v[Q];
+#pragma unroll
+    for (int r = 0; r < Q; ++r) v[r] = ok ? load_pop<Q, S, R>((const S*)a.src, a.ns_f, r, ch) : (R)0;
+#pragma unroll
+    for (int r = 0; r < Q; ++r) {
+      R sum = __shfl(v[r], b0, 64);
+#pragma unroll
+      for (int t = 1; t < 8; ++t) sum += __shfl(v[r], b0 + t, 64);
+      if (j == 0) ls[g * 8 + pp][r] = sum / (R)8.0;
     }
   }
   __syncthreads();
-  const int pl = threadIdx.x >> 5, q = threadIdx.x & 31;
-  const int64_t i = (int64_t)blockIdx.x * 8 + pl;
-  const bool live = i < a.n && q < Q;
-  if (live) {
-    R s = 0;
-    for (int j = 0; j < 8; ++j) s += lc[pl][j][q];
-    lf[pl][q] = s / (R)8.0;
-  }
-  __syncthreads();
-  if (!live) return;
+  const int64_t i = base + lane;
+  if (lane >= P || i >= a.n) return;
   const int p = a.list[i];
   R f[Q];
 #pragma unroll
-  for (int r = 0; r < Q; ++r) f[r] = lf[pl][r];
+  for (int r = 0; r < Q; ++r) f[r] = ls[lane][r];
   R rho, ux, uy, uz;
   cell_macros<Q, R>(f, rho, ux, uy, uz);
-  R uu = ux * ux + uy * uy + uz * uz;
+  const R uu = ux * ux + uy * uy + uz * uz;
   const R sc = (R)a.scale;
-  // this lane's population (q is a lane index: no register-array indexing by it)
-  const R fe_q = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)a.rho0);
-  const R fq = fe_q + sc * (lf[pl][q] - fe_q);
 #pragma unroll
   for (int r = 0; r < Q; ++r) {
     const R fe = feq_hl<Q, R>(r, rho, ux, uy, uz, uu, (R)a.rho0);
     f[r] = fe + sc * (f[r] - fe);
+    store_pop<Q, S, R>((S*)a.dst, a.ns_c, r, p, f[r]);
   }
-  store_pop<Q, S, R>((S*)a.dst, a.ns_c, q, p, fq);
   if (!a.collide) return;
   cell_macros<Q, R>(f, rho, ux, uy, uz);
-  if (q == 0 && a.write_macros) {
+  if (a.write_macros) {
     st_real<R>(a.rho_out, p, rho);
     st_real<R>(a.u_out, p, ux);
     st_real<R>(a.u_out, a.ns_c + p, uy);
     st_real<R>(a.u_out, 2 * a.ns_c + p, uz);
   }
-  R tau = (R)a.tau;
-  if ((R)a.lesC > (R)0) tau = smag_tau<Q, R>(f, rho, ux, uy, uz, tau, (R)a.lesC);
-  if (Q == 27 && a.model == MT_COLLIDE_CUMULANT) {
-    if (q != 0) return;
-    collide_cumulant<R>(f, rho, ux, uy, uz, tau);
+  collide_cell<Q, R>(f, rho, ux, uy, uz, (R)a.tau, (R)a.lesC, a.model);
 #pragma unroll
-    for (int r = 0; r < Q; ++r) store_pop<Q, S, R>((S*)a.fcol_out, a.ns_c, r, p, f[r]);
-  } else {
-    uu = ux * ux + uy * uy + uz * uz;
-    const R fe = feq_c<Q, R>(q, rho, ux, uy, uz, uu);
-    store_pop<Q, S, R>((S*)a.fcol_out, a.ns_c, q, p, fq - (fq - fe) * ((R)1 / tau));
-  }
+  for (int r = 0; r < Q; ++r) store_pop<Q, S, R>((S*)a.fcol_out, a.ns_c, r, p, f[r]);
 }
 
 // ------------------------------------------------------------------ state I/O
@@ -1127,7 +1206,7 @@ int launch_ib(Case& c, bool acc) {
   return MT_OK;
 }
 
-int launch_explode(Case& c, int L) {
+int launch_explode(Case& c, int L, bool mirror_all) {
   if (L <= 0) return MT_OK;
   DevLevel& lc = c.lv[L - 1];
   DevLevel& lf = c.lv[L];
@@ -1145,7 +1224,9 @@ int launch_explode(Case& c, int L) {
   a.list = lc.xp_list;
   a.n = lc.n_xp;
   a.child = lc.xp_child;
-  MT_DISPATCH(c, (k_explode<Q_, S_><<<(a.n + 7) / 8, 256, 0, c.st[L]>>>(a, lc.n_f2c)));
+  // mirror_all: every target also goes to the second buffer, which then holds
+  // what the next sub-step's explosion would write
+  MT_DISPATCH(c, (k_explode<Q_, S_><<<(a.n + 7) / 8, 256, 0, c.st[L]>>>(a, mirror_all ? 0 : lc.n_f2c)));
   MT_HIP(hipGetLastError());
   return MT_OK;
 }
@@ -1175,12 +1256,36 @@ int launch_coalesce(Case& c, int L, bool collide_after, bool write_macros) {
   a.tau = c.tau[L];
   a.lesC = c.d.les_C;
   a.model = c.d.collision;
-  MT_DISPATCH(c, (k_coalesce<Q_, S_><<<(a.n + 7) / 8, 256, 0, c.st[L]>>>(a)));
+  MT_DISPATCH(c, (k_coalesce<Q_, S_><<<(a.n + 15) / 16, 64, 0, c.st[L]>>>(a)));
   MT_HIP(hipGetLastError());
   return MT_OK;
 }
 
-int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros) {
+bool level_has_ib(const Case& c, int L) {
+  return c.d.ib != MT_IB_NONE && L == c.Lmax && c.Lmax > 0 && c.lv[L].n_dis > 0 && c.lv[L].n_ibl > 0;
+}
+
+// The finest level's treated cells for one sub-step on stream st: k_ib_links
+// (stream + link treatment + clamp + forcing from this sub-step's fcol and rho,
+// then macros + collision into the other fcol buffer).
+int launch_ib_links(Case& c, int L, hipStream_t st) {
+  DevLevel& l = c.lv[L];
+  if (!level_has_ib(c, L)) return MT_OK;
+  const LvArgs v = lvargs(c, L, l.pf, 1 - l.pf, l.pr, 1 - l.pr);
+  const int64_t nbi = (l.n_ibl + 31) / 32;
+  if (nbi > c.force_part_cap) {
+    set_error("force slot buffer too small");
+    return MT_ERR_STATE;
+  }
+  MT_DISPATCH(c, (k_ib_links<Q_, S_, true><<<nbi, 256, 0, st>>>(v, l.ibl, l.n_ibl, l.ib_src, l.ib_dist, c.d.ib,
+                                                                  c.force_slots, c.force_acc, c.tau[L], c.d.les_C,
+                                                                  c.d.collision)));
+  MT_HIP(hipGetLastError());
+  return MT_OK;
+}
+
+int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros, bool c2f_fcol,
+                 const CoalReq* coal) {
   DevLevel& l = c.lv[L];
   FusedArgs a;
   a.v = lvargs(c, L, l.pf, 1 - l.pf, l.pr, 1 - l.pr);
@@ -1192,25 +1297,37 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros
   a.write_f = write_f ? 1 : 0;
   a.bc = bccfg(c);
   a.do_bc = (L == 0 && a.bc.wet_mask) ? 1 : 0;
-  a.do_ib = (c.d.ib != MT_IB_NONE && L == c.Lmax && c.Lmax > 0 && l.n_dis > 0) ? 1 : 0;
-  a.ib_method = c.d.ib;
+  a.do_ib = level_has_ib(c, L) ? 1 : 0;
   a.dis_row = l.dis_row;
-  a.dis = l.dis;
-  a.Qm1 = c.Q - 1;
   a.write_macros = write_macros ? 1 : 0;
+  a.c2f_fcol = c2f_fcol ? 1 : 0;
+  a.c2f_f = (L > 0 && !c.lv[L - 1].coal_in_fused) ? 1 : 0;
+  a.cpar = nullptr;
+  a.ns_c = 0;
+  a.cf = a.cfcol_out = a.crho_out = a.cu_out = nullptr;
+  a.ctau = a.cscale = 0;
+  a.rho0 = c.d.rho0;
+  a.c_write_macros = 0;
+  if (coal) {
+    if (L == 0 || !c.lv[L - 1].coal_in_fused || !l.cpar) {
+      set_error("coalescence in the fused update requested where it does not apply");
+      return MT_ERR_STATE;
+    }
+    DevLevel& lc = c.lv[L - 1];
+    a.cpar = l.cpar;
+    a.ns_c = lc.n_slots;
+    a.cf = lc.f;
+    a.cfcol_out = lc.fcol[1 - lc.pf];
+    a.crho_out = lc.rho[1 - lc.pr];
+    a.cu_out = lc.u[1 - lc.pr];
+    a.ctau = c.tau[L - 1];
+    a.cscale = (c.d.options & MT_OPT_CONVECTIVE_XFER) ? (2.0 * c.tau[L - 1]) / c.tau[L]
+                                                      : c.tau[L - 1] / (2.0 * c.tau[L]);
+    a.c_write_macros = coal->write_macros;
+  }
   // grid: a multiple of 8 blocks so the XCD remap covers every brick index
   int64_t nb = ((l.n_active + 7) / 8) * 8;
   if (nb < 8) nb = 8;
-  if (a.do_ib && l.n_ibl) {
-    // the solid boundary first: it reads only fcol / rho of this sub-step
-    const int64_t nbi = (l.n_ibl + 31) / 32;
-    if (nbi > c.force_part_cap) {
-      set_error("force slot buffer too small");
-      return MT_ERR_STATE;
-    }
-    MT_DISPATCH(c, (k_ib_links<Q_, S_><<<nbi, 256, 0, c.st[L]>>>(a.v, l.ibl, l.n_ibl, l.dis_row, l.dis, c.d.ib,
-                                                                   c.force_slots, c.force_acc)));
-  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c.timing) {
     if (c.ev_pool.size() < 2) {
@@ -1226,7 +1343,10 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros
     c.ev_pool.pop_back();
     MT_HIP(hipEventRecord(e0, c.st[L]));
   }
-  MT_DISPATCH(c, (k_fused<Q_, S_><<<nb, 64, 0, c.st[L]>>>(a)));
+  if (coal)
+    MT_DISPATCH(c, (k_fused<Q_, S_, true><<<nb, 64, 0, c.st[L]>>>(a)));
+  else
+    MT_DISPATCH(c, (k_fused<Q_, S_, false><<<nb, 64, 0, c.st[L]>>>(a)));
   MT_HIP(hipGetLastError());
   if (c.timing) {
     MT_HIP(hipEventRecord(e1, c.st[L]));

@@ -295,6 +295,61 @@ __global__ void k_flag_ib(const uint16_t* kind, const int32_t* dis_row, int64_t 
   if (s >= ns) return;
   flag[s] = (kind[s] & MT_KIND_DIS) && (kind[s] & MT_KIND_FLUID) && dis_row[s] >= 0;
 }
+// Link table of the treated cells (SinglePointInterpolation::treat,
+// ComplexBoundary_Manager.cpp:75-148): link q of cell s is a wall link when
+// dis[s][inv(q) - 1] != -1; it then interpolates with fcol_q(x - e_q), else it
+// pulls fcol_q(x + e_q); a missing source reads the cell's own fcol_inv(q).
+__global__ void k_ib_table(const int32_t* list, int64_t n, const int32_t* bnbr, const uint32_t* pmask,
+                           const int32_t* dis_row, const double* dis, int Q, int64_t ns, int32_t* src,
+                           double* dist) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = list[i];
+  int lx, ly, lz;
+  brick_coords(s & 63, lx, ly, lz);
+  const int* nb27 = bnbr + (int64_t)(s >> 6) * 27;
+  const uint32_t pm = pmask[s];
+  const double* row = dis + (int64_t)dis_row[s] * (Q - 1);
+  for (int q = 0; q < 32; ++q) {
+    int32_t e = -1;
+    double d = -1.0;
+    if (q == 0) {
+      e = (int32_t)s;
+    } else if (q < Q) {
+      const int qi = qinv(q);
+      d = row[qi - 1];
+      const bool wall = d != -1.0;
+      const int sg = wall ? -1 : 1;
+      const bool have = wall ? ((pm >> qi) & 1) : ((pm >> q) & 1);
+      const int t = have ? slot_at(nb27, lx, ly, lz, sg * kEX[q], sg * kEY[q], sg * kEZ[q]) : -1;
+      e = t >= 0 ? (int32_t)(q * ns + t) : (int32_t)(qi * ns + s);
+    }
+    src[i * 32 + q] = e;
+    dist[i * 32 + q] = d;
+  }
+}
+// Coalescence inside the fine level's fused update: the 8 children of an F2C
+// parent form one aligned group of 8 consecutive fine slots, child j at offset j
+// (the child order j = x + 2y + 4z of LBM_Manager.cpp:1074-1082 is the low
+// Morton bits), so the fine block holding them can average them.  A parent
+// qualifies when its children are C2F cells that the fused update streams
+// (not lat_f, not treated solid-boundary cells); cpar[group] = parent slot.
+__global__ void k_cpar(const int32_t* f2c, const int32_t* child, int64_t n, const uint16_t* fkind,
+                       const int32_t* fdis_row, int32_t* cpar, int* bad) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t c0 = child[i * 8];
+  bool ok = c0 >= 0 && (c0 & 7) == 0;
+  for (int j = 0; j < 8 && ok; ++j) {
+    const int32_t cj = child[i * 8 + j];
+    ok = cj == c0 + j;
+    if (!ok) break;
+    const uint16_t k = fkind[cj];
+    ok = (k & MT_KIND_C2F) && !(k & MT_KIND_FLUID) && !((k & MT_KIND_DIS) && fdis_row[cj] >= 0);
+  }
+  if (ok) cpar[c0 >> 3] = f2c[i];
+  else *bad = 1;
+}
 __global__ void k_flag_brick(const uint16_t* kind, int64_t nb, uint16_t mask, int* flag) {
   int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
@@ -503,6 +558,21 @@ int build_tables(Case& c) {
       k_flag_ib<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.kind, l.dis_row, l.n_slots, flag);
       r = compact_list(c, flag, l.n_slots, &l.ibl, &l.n_ibl);
       if (r) return r;
+      if (l.n_ibl) {
+        if ((int64_t)c.Q * l.n_slots >= ((int64_t)1 << 31)) {
+          set_error("finest level too large for 32-bit link table entries");
+          return MT_ERR_ARG;
+        }
+        l.ib_src = c.alloc<int32_t>(l.n_ibl * 32);
+        l.ib_dist = c.alloc<double>(l.n_ibl * 32);
+        if (!l.ib_src || !l.ib_dist) {
+          set_error("out of device memory");
+          return MT_ERR_HIP;
+        }
+        k_ib_table<<<nblk(l.n_ibl), 256, 0, c.stream>>>(l.ibl, l.n_ibl, l.bnbr, l.pmask, l.dis_row, l.dis, c.Q,
+                                                        l.n_slots, l.ib_src, l.ib_dist);
+        MT_HIP(hipGetLastError());
+      }
     }
     MT_HIP(hipStreamSynchronize(c.stream));
     MT_HIP(hipFree(flag));
@@ -539,6 +609,29 @@ int build_tables(Case& c) {
   int herr = 0;
   MT_HIP(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, c.stream));
   MT_HIP(hipStreamSynchronize(c.stream));
+  if (!herr) {
+    for (int L = 0; L < c.Lmax; ++L) {
+      DevLevel& l = c.lv[L];
+      DevLevel& lf = c.lv[L + 1];
+      if (!l.n_f2c) continue;
+      lf.cpar = c.alloc<int32_t>(lf.n_slots / 8);
+      if (!lf.cpar) {
+        set_error("out of device memory");
+        return MT_ERR_HIP;
+      }
+      int* d_bad = nullptr;
+      MT_HIP(hipMalloc((void**)&d_bad, 4));
+      MT_HIP(hipMemsetAsync(d_bad, 0, 4, c.stream));
+      k_fill_i32<<<nblk(lf.n_slots / 8), 256, 0, c.stream>>>(lf.cpar, lf.n_slots / 8, -1);
+      k_cpar<<<nblk(l.n_f2c), 256, 0, c.stream>>>(l.f2c, l.f2c_child, l.n_f2c, lf.kind, lf.dis_row, lf.cpar, d_bad);
+      MT_HIP(hipGetLastError());
+      int hbad = 1;
+      MT_HIP(hipMemcpyAsync(&hbad, d_bad, 4, hipMemcpyDeviceToHost, c.stream));
+      MT_HIP(hipStreamSynchronize(c.stream));
+      MT_HIP(hipFree(d_bad));
+      l.coal_in_fused = hbad == 0;
+    }
+  }
   if (herr & 4) {
     set_error("an F2C cell has a child missing on the finer level");
     return MT_ERR_LOOKUP;
