@@ -77,7 +77,6 @@ static int substep(Case& c, int L, bool child_last, bool write_f, bool nee, bool
                    bool c2f_fcol, const CoalReq* coal) {
   DevLevel& l = c.lv[L];
   hipStream_t s = c.st[L];
-  const bool ib = level_has_ib(c, L);
   if (L < c.Lmax) {
     MT_HIP(hipEventRecord(c.ev_ready[L], s));
     MT_HIP(hipStreamWaitEvent(c.st[L + 1], c.ev_ready[L], 0));
@@ -85,9 +84,6 @@ static int substep(Case& c, int L, bool child_last, bool write_f, bool nee, bool
     TRY(run_level(c, L + 1, child_last, l.coal_in_fused ? &cr : nullptr));
   }
   if (explode) TRY(launch_explode(c, L, explode == 2));
-  // the treated cells first (stream + links + collision): they read only this
-  // sub-step's fcol and rho, and the fused update skips them
-  if (ib) TRY(launch_ib_links(c, L, s));
   TRY(launch_fused(c, L, write_f, nee, write_macros, c2f_fcol, coal));
   if (L < c.Lmax) {
     MT_HIP(hipEventRecord(c.ev_done[L + 1], c.st[L + 1]));
@@ -288,7 +284,7 @@ int mt_case_finalize(mt_case* c) {
     }
     if (!l.f) { set_error("out of device memory"); return MT_ERR_HIP; }
     max_fp = std::max<int64_t>(max_fp, (ns + 255) / 256);
-    max_fp = std::max<int64_t>(max_fp, (l.n_active + 3) / 4);
+    max_fp = std::max<int64_t>(max_fp, l.n_active);
     max_fp = std::max<int64_t>(max_fp, (l.n_ibl + 31) / 32);
   }
   c->force_part_cap = max_fp;

@@ -49,11 +49,6 @@ struct DevLevel {
   int32_t* dis_row = nullptr;          // [n_slots] row in dis table or -1 (finest level)
   int32_t* ibl = nullptr;              // treated dis slots (dis and lat_f), finest level
   int64_t n_ibl = 0;
-  // per treated cell and link q (row stride 32): the fcol element the link
-  // reads (q*ns + source slot, or the cell's own inverse population when the
-  // source is missing) and the wall fraction of the link (-1: fluid link)
-  int32_t* ib_src = nullptr;
-  double* ib_dist = nullptr;
   double* dis = nullptr;               // [n_dis][Q-1]
   int64_t n_dis = 0;
   int32_t* bcres = nullptr;
@@ -142,7 +137,6 @@ int launch_ib(Case& c, bool reduce_into_acc);
 int launch_fused(Case& c, int L, bool write_f, bool bc_nee_after, bool write_macros = true, bool c2f_fcol = true,
                  const CoalReq* coal = nullptr);
 bool level_has_ib(const Case& c, int L);
-int launch_ib_links(Case& c, int L, hipStream_t s);
 int launch_gather(Case& c, int L, int field, const int32_t* d_slots, int64_t n, double* d_out, int buf);
 int launch_scatter(Case& c, int L, int field, const int32_t* d_slots, int64_t n, const double* d_in, int buf);
 int launch_reduce(Case& c, int L, double* h_out);

@@ -8,7 +8,8 @@
 //  * the fused kernel used by mt_step: for every lat_f / C2F cell of a level it
 //    pulls fcol from the neighbours, applies the level-0 wet-node boundary or the
 //    finest-level solid boundary, computes the moments and collides, writing
-//    the next sub-step's fcol into the other buffer.  The reference runs
+//    the next sub-step's fcol into the other buffer (and, on the last sub-step
+//    of a pair, coalesces into the coarser level's F2C cells).  The reference runs
 //    stream -> boundary -> (next sub-step) macros -> collide on the same f with
 //    no other writer in between (LBM_Simulator.cpp:20-229), so the fused order
 //    computes the same values while reading and writing each population once.
@@ -301,8 +302,12 @@ struct FusedArgs {
   int write_f;
   int do_bc;
   BcCfg bc;
-  int do_ib;
+  int do_ib;  // the finest level's solid-boundary links (SinglePointInterpolation::treat)
+  int ib_method;
   const int32_t* __restrict__ dis_row;
+  const double* __restrict__ dis;
+  double* force_slots;  // [n_active][3] per-brick running momentum-exchange sums
+  double* force_acc;
   int write_macros;
   int c2f_fcol;  // store the C2F cells' fcol (0: an explosion overwrites it before any read)
   int c2f_f;     // store the C2F cells' f (read by a separate coalescence kernel)
@@ -317,6 +322,108 @@ struct FusedArgs {
   int c_write_macros;
 };
 
+// SinglePointInterpolation::treat (ComplexBoundary_Manager.cpp:63-225) on one
+// treated cell, in the lane that streams it.  On entry f holds the pulled
+// populations (fluid links: fcol_q(x + e_q), or the bounce-back fcol_inv(q)(x)
+// when the source is missing).  Wall links (dis[inv(q) - 1] != -1) interpolate
+// between fcol_inv(q)(x) and fcol_q(x - e_q) (Filippova-Haenel or distance
+// weighted) and add their momentum exchange to the force; then the density
+// clamp on f_0 and eight multi-direct-forcing sweeps.  A sweep adds
+// df_q = 3 w_q e_q.dM (dM = -rho u / 2, D3Q19 weights, zero for q > 18) to the
+// wall links and -sum df_q to f_0, so rho is invariant and the momentum obeys
+// m <- m + 3 W dM with W = sum_wall w_q e_q e_q^T: the sweeps reduce to that
+// 3-vector recurrence, and each wall link finally receives 3 w_q e_q.D with D
+// the sum of the sweeps' dM (f_0: -3 V.D, V = sum_wall w_q e_q).
+template <int Q, typename S, typename R, typename Args>
+__device__ __forceinline__ void ib_links_cell(R* f, const Args& a, const int* N, int lx, int ly, int lz, int s,
+                                              int row, uint32_t pm, double& fx, double& fy, double& fz) {
+  const S* fin = (const S*)a.v.fcol_in;
+  const int64_t ns = a.v.ns;
+  const double* drow = a.dis + (int64_t)row * (Q - 1);
+  const R rho_b = ld_real<R>(a.v.rho_in, s);
+  uint32_t wm = 0;
+  R wxx = 0, wyy = 0, wzz = 0, wxy = 0, wxz = 0, wyz = 0, vx = 0, vy = 0, vz = 0;
+  static_for<1, Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int qi = qinv(q);
+    const double dist = drow[qi - 1];
+    if (dist != -1.0) {
+      wm |= 1u << q;
+      const R finc = load_pop<Q, S, R>(fin, ns, qi, s);
+      R fs = finc;
+      if ((pm >> qi) & 1) fs = load_pop<Q, S, R>(fin, ns, q, nbr_slot_c<-kEX[q], -kEY[q], -kEZ[q]>(N, lx, ly, lz));
+      const R d = (R)dist;
+      R fn;
+      if (a.ib_method == MT_IB_FH) {
+        if (d < (R)0.5) {
+          fn = ((R)1 - (R)2 * d) * finc + (R)2 * d * fs;
+        } else {
+          // D3Q19 weight of inv(q) (|e_inv(q)| = |e_q|), zero beyond q = 18
+          constexpr double w = qi < 19 ? (qnorm1(q) == 1 ? 1. / 18. : 1. / 36.) : 0.;
+          const R i2d = (R)1 / ((R)2 * d);
+          fn = i2d * fs + ((R)1 - i2d) * ((R)w * rho_b);
+        }
+      } else {
+        fn = ((R)1 - d) * finc + d * fs;
+      }
+      f[q] = fn;
+      const double m = (double)finc + (double)fn;
+      fx -= m * kEX[q];
+      fy -= m * kEY[q];
+      fz -= m * kEZ[q];
+      if (q < 19) {
+        constexpr double w = qnorm1(q) == 1 ? 1. / 18. : 1. / 36.;
+        wxx += (R)(w * kEX[q] * kEX[q]);
+        wyy += (R)(w * kEY[q] * kEY[q]);
+        wzz += (R)(w * kEZ[q] * kEZ[q]);
+        wxy += (R)(w * kEX[q] * kEY[q]);
+        wxz += (R)(w * kEX[q] * kEZ[q]);
+        wyz += (R)(w * kEY[q] * kEZ[q]);
+        vx += (R)(w * kEX[q]);
+        vy += (R)(w * kEY[q]);
+        vz += (R)(w * kEZ[q]);
+      }
+    }
+  });
+  // density clamp through f_0 (ComplexBoundary_Manager.cpp:150-158)
+  R ra = 0, m0 = 0, m1 = 0, m2 = 0;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    ra += f[q];
+    m0 += (R)kEX[q] * f[q];
+    m1 += (R)kEY[q] * f[q];
+    m2 += (R)kEZ[q] * f[q];
+  }
+  R dclamp = 0;
+  if (ra < (R)0.3) dclamp = (R)1.0 * ((R)0.3 - ra);
+  else if (ra < (R)0.5) dclamp = (R)0.5 * ((R)0.5 - ra);
+  else if (ra > (R)3.0) dclamp = (R)1.0 * ((R)3.0 - ra);
+  else if (ra > (R)2.0) dclamp = (R)0.5 * ((R)2.0 - ra);
+  f[0] += dclamp;
+  const R rho = ra + dclamp;
+  R D0 = 0, D1 = 0, D2 = 0;
+  if (rho > (R)0) {
+    const R ir = (R)1 / rho;
+    for (int it = 0; it < 8; ++it) {
+      const R ux = m0 * ir, uy = m1 * ir, uz = m2 * ir;
+      if (ux * ux + uy * uy + uz * uz < (R)1e-20) break;  // |u| < 1e-10: m no longer changes
+      const R d0 = (R)-0.5 * rho * ux, d1 = (R)-0.5 * rho * uy, d2 = (R)-0.5 * rho * uz;
+      D0 += d0;
+      D1 += d1;
+      D2 += d2;
+      m0 += (R)3.0 * (wxx * d0 + wxy * d1 + wxz * d2);
+      m1 += (R)3.0 * (wxy * d0 + wyy * d1 + wyz * d2);
+      m2 += (R)3.0 * (wxz * d0 + wyz * d1 + wzz * d2);
+    }
+  }
+  static_for<1, (Q < 19 ? Q : 19)>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr double w = qnorm1(q) == 1 ? 1. / 18. : 1. / 36.;
+    if ((wm >> q) & 1) f[q] += (R)w * (R)3.0 * ((R)kEX[q] * D0 + (R)kEY[q] * D1 + (R)kEZ[q] * D2);
+  });
+  f[0] -= (R)3.0 * (vx * D0 + vy * D1 + vz * D2);
+}
+
 // One 64-lane block per brick.  Workgroups are dealt round-robin over the 8
 // XCDs, each with its own L2; the block -> brick map below gives every XCD one
 // contiguous Morton range of bricks, so a brick's neighbour bricks (its halo)
@@ -330,7 +437,7 @@ struct FusedArgs {
 // equilibrium and runs the F2C cell's calculateMacros + collide, which the
 // reference does at the start of the coarse level's next sub-step.
 template <int Q, typename S, bool COAL>
-__global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_fused(FusedArgs a) {
   typedef typename RealOf<S>::type R;
   const int lane = threadIdx.x;
   const int64_t nb = gridDim.x;
@@ -347,32 +454,36 @@ __global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
 #pragma unroll
     for (int q = 0; q < Q; ++q) f[q] = 0;
   }
+  double fx = 0, fy = 0, fz = 0;  // momentum exchange of this lane's treated cell
   {
     int N[27];
 #pragma unroll
     for (int o = 0; o < 27; ++o) N[o] = __builtin_amdgcn_readfirstlane(a.v.bnbr[(int64_t)b * 27 + o]);
     const int s = (b << 6) | lane;
     const uint16_t k = a.v.kind[s];
-    // treated solid-boundary cells are streamed, treated and collided by
-    // k_ib_links
-    const bool treated = a.do_ib && (k & MT_KIND_DIS) && (k & MT_KIND_FLUID) && a.dis_row[s] >= 0;
-    if ((k & (MT_KIND_FLUID | MT_KIND_C2F)) && !treated) {
+    const int row = (a.do_ib && (k & MT_KIND_DIS) && (k & MT_KIND_FLUID)) ? a.dis_row[s] : -1;
+    const bool treated = row >= 0;
+    if (k & (MT_KIND_FLUID | MT_KIND_C2F)) {
       int lx, ly, lz;
       brick_coords(lane, lx, ly, lz);
       const uint32_t pmx = a.v.pmask[s];
       f[0] = load_pop<Q, S, R>(fin, ns, 0, s);
       // pull f_q(x) <- fcol_q(x + e_q); a missing source keeps the cell's own f_q
-      // (Stream_Manager.cpp:53-62).  The source is selected, not branched on, so
-      // all Q loads are issued back to back.
+      // (Stream_Manager.cpp:53-62), except on a treated solid-boundary cell,
+      // which bounces back its own fcol_inv(q) (ComplexBoundary_Manager.cpp:
+      // 140-146).  The source is selected, not branched on, so all Q loads are
+      // issued back to back.
       static_for<1, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const int t = nbr_slot_c<kEX[q], kEY[q], kEZ[q]>(N, lx, ly, lz);
         const bool p = (pmx >> q) & 1;
-        const S* src = p ? fin + (int64_t)q * ns + t : fst + (int64_t)q * ns + s;
+        const S* src = p ? fin + (int64_t)q * ns + t
+                         : (treated ? fin + (int64_t)qinv(q) * ns + s : fst + (int64_t)q * ns + s);
         R v = (R)Store<S>::ld(src);
         if (Store<S>::shifted) v += (R)wq<Q>(q);
         f[q] = v;
       });
+      if (treated) ib_links_cell<Q, S, R>(f, a, N, lx, ly, lz, s, row, pmx, fx, fy, fz);
       if (a.do_bc && (k & 0x3F00)) {
         const int32_t* bn = a.v.bnbr;
         const uint32_t* pmk = a.v.pmask;
@@ -401,7 +512,7 @@ __global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
         cell_macros<Q, R>(f, rho, ux, uy, uz);
         // the density/velocity maps are read only by the solid boundary (its own
         // cells, next sub-step), the reservoir NEE and the caller at the end
-        if (a.write_macros) {
+        if (a.write_macros || treated) {
           st_real<R>(a.v.rho_out, s, rho);
           st_real<R>(a.v.u_out, s, ux);
           st_real<R>(a.v.u_out, ns + s, uy);
@@ -414,6 +525,22 @@ __global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) store_pop<Q, S, R>(fout, ns, q, s, f[q]);
       }
+    }
+  }
+  if (a.do_ib) {
+    // this call's force: one running slot per brick, fixed block -> brick map
+    for (int off = 32; off > 0; off >>= 1) {
+      fx += __shfl_down(fx, off, 64);
+      fy += __shfl_down(fy, off, 64);
+      fz += __shfl_down(fz, off, 64);
+    }
+    if (lane == 0) {
+      if (fx != 0.0 || fy != 0.0 || fz != 0.0) {
+        a.force_slots[3 * bi] += fx;
+        a.force_slots[3 * bi + 1] += fy;
+        a.force_slots[3 * bi + 2] += fz;
+      }
+      if (bi == 0) a.force_acc[7] += 1.0;  // treat() calls folded into the slots
     }
   }
   if (!COAL) return;
@@ -453,211 +580,6 @@ __global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
   collide_cell<Q, R>(avg, rho, ux, uy, uz, (R)a.ctau, (R)a.lesC, a.model);
 #pragma unroll
   for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)a.cfcol_out, a.ns_c, q, par, avg[q]);
-}
-
-// per-lane lattice tables for kernels that index by a lane-owned direction
-__constant__ int cEX[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, -1, 1, -1, 1};
-__constant__ int cEY[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, -1, 1, 1, -1};
-__constant__ int cEZ[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1, 1, -1, 1, -1};
-
-// sum over the 8 lanes of a group (xor butterfly: every lane gets the same value)
-template <typename T>
-__device__ __forceinline__ T group_sum8(T v) {
-  v += __shfl_xor(v, 1, 64);
-  v += __shfl_xor(v, 2, 64);
-  v += __shfl_xor(v, 4, 64);
-  return v;
-}
-
-// SinglePointInterpolation::treat for the fused schedule
-// (ComplexBoundary_Manager.cpp:63-225).  Eight lanes per treated cell, lane j
-// owning links j, j+8, j+16, j+24: wall links interpolate (FH or distance
-// weighted) between fcol_inv(q)(x) and fcol_q(x - e_q), other links pull
-// fcol_q(x + e_q) or bounce back; then the density clamp on f_0 and eight
-// multi-direct-forcing sweeps whose moments are group sums.  The result is the
-// cell's post-stream f; with COLLIDE its calculateMacros + collide follow here
-// (the fused kernel skips treated cells).  Each block adds this call's
-// momentum-exchange force to its own running slot (fixed block -> cell map);
-// mt_ib_force sums the slots in block order.
-template <int Q, typename S, bool COLLIDE>
-__global__ __launch_bounds__(256) void k_ib_links(LvArgs v, const int32_t* __restrict__ list, int64_t n,
-                                                  const int32_t* __restrict__ tsrc,
-                                                  const double* __restrict__ tdist, int method, double* slots,
-                                                  double* acc, double tau, double lesC, int model) {
-  typedef typename RealOf<S>::type R;
-  constexpr int K = 4;  // links per lane
-  __shared__ double red[12];
-  const int grp = threadIdx.x >> 3, j = threadIdx.x & 7;
-  const int64_t i = (int64_t)blockIdx.x * 32 + grp;
-  const bool live = i < n;
-  const S* fin = (const S*)v.fcol_in;
-  const int64_t ns = v.ns;
-  int s = 0;
-  R fq[K];
-  bool wall[K];
-  double fx = 0, fy = 0, fz = 0;
-  R rho_b = 0;
-  if (live) {
-    s = list[i];
-    rho_b = ld_real<R>(v.rho_in, s);
-  }
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int q = j + 8 * k;
-    fq[k] = 0;
-    wall[k] = false;
-    if (!live || q >= Q) continue;
-    // the link's source element (table built once per case) and wall fraction
-    const int32_t e = tsrc[i * 32 + q];
-    R src = (R)Store<S>::ld(fin + e);
-    if (Store<S>::shifted) src += (R)wq<Q>(q);  // w_inv(q) = w_q
-    if (q == 0) {
-      fq[k] = src;
-      continue;
-    }
-    const double dist = tdist[i * 32 + q];
-    const int ex = cEX[q], ey = cEY[q], ez = cEZ[q];
-    const int qi = qinv(q);
-    if (dist != -1.0) {
-      const R finc = load_pop<Q, S, R>(fin, ns, qi, s);
-      wall[k] = true;
-      const R fs = src;
-      const R d = (R)dist;
-      R fn;
-      if (method == MT_IB_FH) {
-        if (d < (R)0.5) {
-          fn = ((R)1 - (R)2 * d) * finc + (R)2 * d * fs;
-        } else {
-          const int nq = (ex != 0) + (ey != 0) + (ez != 0);  // |e_qi| = |e_q|; D3Q19 table
-          const R w = qi < 19 ? (nq == 1 ? (R)(1. / 18.) : (R)(1. / 36.)) : (R)0;
-          const R i2d = (R)1 / ((R)2 * d);
-          fn = i2d * fs + ((R)1 - i2d) * (w * rho_b);
-        }
-      } else {
-        fn = ((R)1 - d) * finc + d * fs;
-      }
-      fq[k] = fn;
-      const double m = (double)finc + (double)fn;
-      fx -= m * ex;
-      fy -= m * ey;
-      fz -= m * ez;
-    } else {
-      fq[k] = src;
-    }
-  }
-  // density clamp through f_0 (ComplexBoundary_Manager.cpp:150-158); f_0 lives in lane 0, k = 0
-  R loc = 0, mx = 0, my = 0, mz = 0;
-  R wxx = 0, wyy = 0, wzz = 0, wxy = 0, wxz = 0, wyz = 0, vx = 0, vy = 0, vz = 0;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int q = j + 8 * k;
-    if (q >= Q) continue;
-    const R ex = (R)cEX[q], ey = (R)cEY[q], ez = (R)cEZ[q];
-    loc += fq[k];
-    mx += ex * fq[k];
-    my += ey * fq[k];
-    mz += ez * fq[k];
-    if (wall[k] && q < 19) {  // D3Q19 weights: zero beyond q = 18
-      const int nq = (cEX[q] != 0) + (cEY[q] != 0) + (cEZ[q] != 0);
-      const R w = nq == 1 ? (R)(1. / 18.) : (R)(1. / 36.);
-      wxx += w * ex * ex;
-      wyy += w * ey * ey;
-      wzz += w * ez * ez;
-      wxy += w * ex * ey;
-      wxz += w * ex * ez;
-      wyz += w * ey * ez;
-      vx += w * ex;
-      vy += w * ey;
-      vz += w * ez;
-    }
-  }
-  const R ra = group_sum8(loc);
-  R dclamp = 0;
-  if (ra < (R)0.3) dclamp = (R)1.0 * ((R)0.3 - ra);
-  else if (ra < (R)0.5) dclamp = (R)0.5 * ((R)0.5 - ra);
-  else if (ra > (R)3.0) dclamp = (R)1.0 * ((R)3.0 - ra);
-  else if (ra > (R)2.0) dclamp = (R)0.5 * ((R)2.0 - ra);
-  if (j == 0) fq[0] += dclamp;
-  // multi-direct forcing, 8 sweeps, relaxation 1/2 (:180-225).  A sweep adds
-  // df_q = 3 w_q e_q.dM (dM = -m/2) to the wall links and -sum df_q to f_0, so
-  // rho is invariant and m <- m + 3 W dM with W = sum_wall w_q e_q e_q^T: the
-  // sweeps reduce to a 3-vector recurrence, and each wall link finally receives
-  // 3 w_q e_q.D with D the sum of the sweeps' dM (f_0: -3 V.D, V = sum_wall w_q e_q).
-  {
-    const R rho = ra + dclamp;
-    R m0 = group_sum8(mx), m1 = group_sum8(my), m2 = group_sum8(mz);
-    wxx = group_sum8(wxx);
-    wyy = group_sum8(wyy);
-    wzz = group_sum8(wzz);
-    wxy = group_sum8(wxy);
-    wxz = group_sum8(wxz);
-    wyz = group_sum8(wyz);
-    vx = group_sum8(vx);
-    vy = group_sum8(vy);
-    vz = group_sum8(vz);
-    R D0 = 0, D1 = 0, D2 = 0;
-    if (rho > (R)0) {
-      const R ir = (R)1 / rho;
-      for (int it = 0; it < 8; ++it) {
-        const R ux = m0 * ir, uy = m1 * ir, uz = m2 * ir;
-        if (ux * ux + uy * uy + uz * uz < (R)1e-20) break;  // |u| < 1e-10: m no longer changes
-        const R d0 = (R)-0.5 * rho * ux, d1 = (R)-0.5 * rho * uy, d2 = (R)-0.5 * rho * uz;
-        D0 += d0;
-        D1 += d1;
-        D2 += d2;
-        m0 += (R)3.0 * (wxx * d0 + wxy * d1 + wxz * d2);
-        m1 += (R)3.0 * (wxy * d0 + wyy * d1 + wyz * d2);
-        m2 += (R)3.0 * (wxz * d0 + wyz * d1 + wzz * d2);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int q = j + 8 * k;
-      if (!wall[k] || q >= 19) continue;
-      const int nq = (cEX[q] != 0) + (cEY[q] != 0) + (cEZ[q] != 0);
-      const R w = nq == 1 ? (R)(1. / 18.) : (R)(1. / 36.);
-      fq[k] += w * (R)3.0 * ((R)cEX[q] * D0 + (R)cEY[q] * D1 + (R)cEZ[q] * D2);
-    }
-    if (j == 0) fq[0] -= (R)3.0 * (vx * D0 + vy * D1 + vz * D2);
-  }
-  __shared__ R tf[COLLIDE ? 32 : 1][Q];
-  if (live) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int q = j + 8 * k;
-      if (q < Q) {
-        store_pop<Q, S, R>((S*)v.f, ns, q, s, fq[k]);
-        if (COLLIDE) tf[grp][q] = fq[k];
-      }
-    }
-  }
-  block_reduce3(fx, fy, fz, red);
-  if (threadIdx.x == 0) {
-    slots[3 * blockIdx.x] += fx;
-    slots[3 * blockIdx.x + 1] += fy;
-    slots[3 * blockIdx.x + 2] += fz;
-    if (blockIdx.x == 0) acc[7] += 1.0;  // treat() calls folded into the slots
-  }
-  if (!COLLIDE) return;
-  // calculateMacros + collide of the block's 32 treated cells, one lane each;
-  // the density/velocity maps are written every sub-step (the next sub-step's
-  // link treatment reads rho)
-  __syncthreads();
-  const int64_t ic = (int64_t)blockIdx.x * 32 + threadIdx.x;
-  if (threadIdx.x >= 32 || ic >= n) return;
-  const int sc = list[ic];
-  R f[Q];
-#pragma unroll
-  for (int q = 0; q < Q; ++q) f[q] = tf[threadIdx.x][q];
-  R rho, ux, uy, uz;
-  cell_macros<Q, R>(f, rho, ux, uy, uz);
-  st_real<R>(v.rho_out, sc, rho);
-  st_real<R>(v.u_out, sc, ux);
-  st_real<R>(v.u_out, ns + sc, uy);
-  st_real<R>(v.u_out, 2 * ns + sc, uz);
-  collide_cell<Q, R>(f, rho, ux, uy, uz, (R)tau, (R)lesC, model);
-#pragma unroll
-  for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)v.fcol_out, ns, q, sc, f[q]);
 }
 
 // fixed-order sum of the per-block IB force partials into the accumulator
@@ -1265,25 +1187,6 @@ bool level_has_ib(const Case& c, int L) {
   return c.d.ib != MT_IB_NONE && L == c.Lmax && c.Lmax > 0 && c.lv[L].n_dis > 0 && c.lv[L].n_ibl > 0;
 }
 
-// The finest level's treated cells for one sub-step on stream st: k_ib_links
-// (stream + link treatment + clamp + forcing from this sub-step's fcol and rho,
-// then macros + collision into the other fcol buffer).
-int launch_ib_links(Case& c, int L, hipStream_t st) {
-  DevLevel& l = c.lv[L];
-  if (!level_has_ib(c, L)) return MT_OK;
-  const LvArgs v = lvargs(c, L, l.pf, 1 - l.pf, l.pr, 1 - l.pr);
-  const int64_t nbi = (l.n_ibl + 31) / 32;
-  if (nbi > c.force_part_cap) {
-    set_error("force slot buffer too small");
-    return MT_ERR_STATE;
-  }
-  MT_DISPATCH(c, (k_ib_links<Q_, S_, true><<<nbi, 256, 0, st>>>(v, l.ibl, l.n_ibl, l.ib_src, l.ib_dist, c.d.ib,
-                                                                  c.force_slots, c.force_acc, c.tau[L], c.d.les_C,
-                                                                  c.d.collision)));
-  MT_HIP(hipGetLastError());
-  return MT_OK;
-}
-
 int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros, bool c2f_fcol,
                  const CoalReq* coal) {
   DevLevel& l = c.lv[L];
@@ -1298,7 +1201,15 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros
   a.bc = bccfg(c);
   a.do_bc = (L == 0 && a.bc.wet_mask) ? 1 : 0;
   a.do_ib = level_has_ib(c, L) ? 1 : 0;
+  a.ib_method = c.d.ib;
   a.dis_row = l.dis_row;
+  a.dis = l.dis;
+  a.force_slots = c.force_slots;
+  a.force_acc = c.force_acc;
+  if (a.do_ib && l.n_active > c.force_part_cap) {
+    set_error("force slot buffer too small");
+    return MT_ERR_STATE;
+  }
   a.write_macros = write_macros ? 1 : 0;
   a.c2f_fcol = c2f_fcol ? 1 : 0;
   a.c2f_f = (L > 0 && !c.lv[L - 1].coal_in_fused) ? 1 : 0;

@@ -295,39 +295,6 @@ __global__ void k_flag_ib(const uint16_t* kind, const int32_t* dis_row, int64_t 
   if (s >= ns) return;
   flag[s] = (kind[s] & MT_KIND_DIS) && (kind[s] & MT_KIND_FLUID) && dis_row[s] >= 0;
 }
-// Link table of the treated cells (SinglePointInterpolation::treat,
-// ComplexBoundary_Manager.cpp:75-148): link q of cell s is a wall link when
-// dis[s][inv(q) - 1] != -1; it then interpolates with fcol_q(x - e_q), else it
-// pulls fcol_q(x + e_q); a missing source reads the cell's own fcol_inv(q).
-__global__ void k_ib_table(const int32_t* list, int64_t n, const int32_t* bnbr, const uint32_t* pmask,
-                           const int32_t* dis_row, const double* dis, int Q, int64_t ns, int32_t* src,
-                           double* dist) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int s = list[i];
-  int lx, ly, lz;
-  brick_coords(s & 63, lx, ly, lz);
-  const int* nb27 = bnbr + (int64_t)(s >> 6) * 27;
-  const uint32_t pm = pmask[s];
-  const double* row = dis + (int64_t)dis_row[s] * (Q - 1);
-  for (int q = 0; q < 32; ++q) {
-    int32_t e = -1;
-    double d = -1.0;
-    if (q == 0) {
-      e = (int32_t)s;
-    } else if (q < Q) {
-      const int qi = qinv(q);
-      d = row[qi - 1];
-      const bool wall = d != -1.0;
-      const int sg = wall ? -1 : 1;
-      const bool have = wall ? ((pm >> qi) & 1) : ((pm >> q) & 1);
-      const int t = have ? slot_at(nb27, lx, ly, lz, sg * kEX[q], sg * kEY[q], sg * kEZ[q]) : -1;
-      e = t >= 0 ? (int32_t)(q * ns + t) : (int32_t)(qi * ns + s);
-    }
-    src[i * 32 + q] = e;
-    dist[i * 32 + q] = d;
-  }
-}
 // Coalescence inside the fine level's fused update: the 8 children of an F2C
 // parent form one aligned group of 8 consecutive fine slots, child j at offset j
 // (the child order j = x + 2y + 4z of LBM_Manager.cpp:1074-1082 is the low
@@ -558,21 +525,6 @@ int build_tables(Case& c) {
       k_flag_ib<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.kind, l.dis_row, l.n_slots, flag);
       r = compact_list(c, flag, l.n_slots, &l.ibl, &l.n_ibl);
       if (r) return r;
-      if (l.n_ibl) {
-        if ((int64_t)c.Q * l.n_slots >= ((int64_t)1 << 31)) {
-          set_error("finest level too large for 32-bit link table entries");
-          return MT_ERR_ARG;
-        }
-        l.ib_src = c.alloc<int32_t>(l.n_ibl * 32);
-        l.ib_dist = c.alloc<double>(l.n_ibl * 32);
-        if (!l.ib_src || !l.ib_dist) {
-          set_error("out of device memory");
-          return MT_ERR_HIP;
-        }
-        k_ib_table<<<nblk(l.n_ibl), 256, 0, c.stream>>>(l.ibl, l.n_ibl, l.bnbr, l.pmask, l.dis_row, l.dis, c.Q,
-                                                        l.n_slots, l.ib_src, l.ib_dist);
-        MT_HIP(hipGetLastError());
-      }
     }
     MT_HIP(hipStreamSynchronize(c.stream));
     MT_HIP(hipFree(flag));
