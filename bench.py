@@ -12,14 +12,19 @@ at tau0 = 0.5192 (NaN within ~100 coarse steps on the device and in the oracle).
 
 Prints one JSON line: MLUPS over the timed region (inputs resident in HBM),
 the HBM roofline of the dominant kernel (finest-level fused update) from HIP
-events, and the map-faithful CPU oracle timed on a bounded sample of the same
-workload as the baseline.
+events, the same case in FP32-shifted and FP16-storage populations with their
+errors against the FP64 run, and the map-faithful CPU oracle timed on a bounded
+sample of the same workload as the baseline.
+
+--counters runs a short eager pass plus a calibration copy for the rocprofv3
+FETCH_SIZE / WRITE_SIZE passes whose summary (profiles/) gives roofline.traffic.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -30,9 +35,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "MLUPS & % HBM roofline (FP64 vs FP32-shifted); served cases/s + p50/p99 latency"
+WORKLOAD = "sphere_amr_3level_d3q27_cumulant_fp64"
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_fused_sphere_amr_fp64_hbm_traffic.md")
 
 
 def bytes_per_update(Q, precision):
+    """Algorithmic bytes of one cell update: every population read once and written once."""
     b = {0: 8, 1: 4, 2: 2}[precision]
     return 2 * Q * b
 
@@ -42,25 +50,57 @@ def level_updates(case):
     return [c["fluid"] + c["c2f"] for c in C.level_counts(case)]
 
 
-def run_gpu(case, steps, warmup, precision, timing_steps):
-    from mortontrie_amr_amd import amr
+def measured_traffic():
+    """HBM bytes per launch of the finest-level fused kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None when absent)."""
+    try:
+        txt = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", txt, re.M)
+    return int(m.group(1)) if m else None
+
+
+def total_mass(g, Lmax):
+    # volume-weighted: a level-L cell is 8^-L of a coarse cell
+    return sum(g.reduce(L)[0] / 8.0 ** L for L in range(Lmax + 1))
+
+
+def run_gpu(case, steps, warmup, precision, timing_steps, keep_fields=False):
+    from mortontrie_amr_amd import amr, cases as C
+    Lmax = case["Lmax"]
     g = amr.AmrCase(case, precision=precision, graphs=True)
     g.step(warmup)
     g.sync()
+    m0 = total_mass(g, Lmax)
     g.ib_force(reset=True)
     t0 = time.perf_counter()
     g.step(steps)
     g.sync()
     t1 = time.perf_counter()
     cd = g.drag_coefficient(reset=True)
-    # kernel durations from HIP events around every fused launch, same stream
+    m1 = total_mass(g, Lmax)
+    fields = None
+    if keep_fields:
+        fields = []
+        for L in range(Lmax + 1):
+            lv = case["levels"][L]
+            keys = lv["keys"][(lv["kinds"] & (C.KIND_FLUID | C.KIND_F2C)) != 0]
+            fields.append((g.rho(L, keys), g.u(L, keys)))
+    # kernel durations: HIP start/stop events attached to every fused launch
     g.set_timing(True)
     g.step(timing_steps)
     ms, calls = g.timing()
     g.set_timing(False)
-    nan = sum(g.reduce(L)[2] for L in range(case["Lmax"] + 1))
+    nan = sum(g.reduce(L)[2] for L in range(Lmax + 1))
     g.close()
-    return (t1 - t0), ms, calls, nan, cd
+    return dict(secs=t1 - t0, ms=ms, calls=calls, nan=nan, cd=cd, mass0=m0, mass1=m1, fields=fields)
+
+
+def rel_l2(a, b):
+    a = np.concatenate([x.ravel() for x in a])
+    b = np.concatenate([x.ravel() for x in b])
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
 def counter_run(case, steps, warmup, copy_mib):
@@ -132,11 +172,11 @@ def main():
     per_level = level_updates(case)
     upd_step = sum(n << L for L, n in enumerate(per_level))
     Q = case["Q"]
-
-    secs, ms, calls, nan, cd = run_gpu(case, args.steps, args.warmup, amr.FP64, args.timing_steps)
-    mlups = upd_step * args.steps / secs / 1e6
     Lf = case["Lmax"]
-    avg_ms = ms[Lf] / max(calls[Lf], 1)
+
+    r = run_gpu(case, args.steps, args.warmup, amr.FP64, args.timing_steps, keep_fields=not args.no_variants)
+    mlups = upd_step * args.steps / r["secs"] / 1e6
+    avg_ms = r["ms"][Lf] / max(r["calls"][Lf], 1)
     bytes_launch = per_level[Lf] * bytes_per_update(Q, amr.FP64)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     out = {
@@ -146,39 +186,47 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(secs / args.steps * 1e3, 5),
+        "ms_per_step": round(r["secs"] / args.steps * 1e3, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "sphere_amr_3level_d3q27_cumulant_fp64",
+            "workload": WORKLOAD,
             "coarse": list(case["dims0"]), "sphere_radius": 8, "levels": Lf + 1,
             "updates_per_level": per_level, "updates_per_coarse_step": upd_step,
             "tau0": case["tau0"], "parallelism": "single_gpu",
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
             "kernel": f"k_fused<27,double> level {Lf}", "bytes_per_launch": bytes_launch,
-            "avg_launch_ms": round(avg_ms, 5), "launches_timed": int(calls[Lf]),
-            "per_level_avg_ms": [round(ms[L] / max(calls[L], 1), 5) for L in range(Lf + 1)],
+            "avg_launch_ms": round(avg_ms, 5), "launches_timed": int(r["calls"][Lf]),
+            "per_level_avg_ms": [round(r["ms"][L] / max(r["calls"][L], 1), 5) for L in range(Lf + 1)],
         },
-        "nan_count": nan,
-        "drag_coefficient": round(cd, 5),
+        "nan_count": r["nan"],
+        "drag_coefficient": round(r["cd"], 5),
+        "mass_drift": (r["mass1"] - r["mass0"]) / r["mass0"],
         "interface_option": "explode_both",
     }
     if not args.no_variants:
         var = {}
         for name, prec in (("fp32_shifted", amr.FP32_SHIFTED), ("fp16_storage", amr.FP16_STORAGE)):
-            s2, ms2, c2, nan2, cd2 = run_gpu(case, max(args.steps // 2, 10), args.warmup, prec, args.timing_steps)
-            a2 = ms2[Lf] / max(c2[Lf], 1)
+            v = run_gpu(case, args.steps, args.warmup, prec, args.timing_steps, keep_fields=True)
+            a2 = v["ms"][Lf] / max(v["calls"][Lf], 1)
             bl = per_level[Lf] * bytes_per_update(Q, prec)
-            var[name] = {"MLUPS": round(upd_step * max(args.steps // 2, 10) / s2 / 1e6, 2),
-                         "finest_kernel_GBs": round(bl / (a2 * 1e-3) / 1e9, 1),
-                         "finest_kernel_ms": round(a2, 5), "drag_coefficient": round(cd2, 5),
-                         "nan_count": nan2}
+            var[name] = {
+                "MLUPS": round(upd_step * args.steps / v["secs"] / 1e6, 2),
+                "finest_kernel_GBs": round(bl / (a2 * 1e-3) / 1e9, 1),
+                "finest_kernel_ms": round(a2, 5),
+                "drag_coefficient": round(v["cd"], 5),
+                "drag_rel_error_vs_fp64": abs(v["cd"] - r["cd"]) / abs(r["cd"]),
+                "rho_rel_l2_vs_fp64": rel_l2([f[0] for f in v["fields"]], [f[0] for f in r["fields"]]),
+                "u_rel_l2_vs_fp64": rel_l2([f[1] for f in v["fields"]], [f[1] for f in r["fields"]]),
+                "mass_drift": (v["mass1"] - v["mass0"]) / v["mass0"],
+                "nan_count": v["nan"],
+            }
         out["variants"] = var
     if not args.no_cpu:
         n = args.cpu_steps

@@ -17,6 +17,8 @@
 #include <type_traits>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "mt_case.h"
 #include "mt_device.h"
 
@@ -1187,6 +1189,19 @@ bool level_has_ib(const Case& c, int L) {
   return c.d.ib != MT_IB_NONE && L == c.Lmax && c.Lmax > 0 && c.lv[L].n_dis > 0 && c.lv[L].n_ibl > 0;
 }
 
+// Launch with start/stop events attached to the dispatch itself (timing mode):
+// the events bracket the kernel's execution, not the queue around it.
+template <typename Kern, typename Args>
+static int launch_events(Kern kern, int64_t nb, int bs, hipStream_t st, Args& a, hipEvent_t e0, hipEvent_t e1) {
+  if (e0) {
+    void* args[] = {&a};
+    MT_HIP(hipExtLaunchKernel((const void*)kern, dim3((uint32_t)nb), dim3(bs), args, 0, st, e0, e1, 0));
+  } else {
+    kern<<<nb, bs, 0, st>>>(a);
+  }
+  return MT_OK;
+}
+
 int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros, bool c2f_fcol,
                  const CoalReq* coal) {
   DevLevel& l = c.lv[L];
@@ -1252,17 +1267,15 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros
     c.ev_pool.pop_back();
     e1 = c.ev_pool.back();
     c.ev_pool.pop_back();
-    MT_HIP(hipEventRecord(e0, c.st[L]));
   }
+  int r = MT_OK;
   if (coal)
-    MT_DISPATCH(c, (k_fused<Q_, S_, true><<<nb, 64, 0, c.st[L]>>>(a)));
+    MT_DISPATCH(c, (r = launch_events(k_fused<Q_, S_, true>, nb, 64, c.st[L], a, e0, e1)));
   else
-    MT_DISPATCH(c, (k_fused<Q_, S_, false><<<nb, 64, 0, c.st[L]>>>(a)));
+    MT_DISPATCH(c, (r = launch_events(k_fused<Q_, S_, false>, nb, 64, c.st[L], a, e0, e1)));
+  if (r) return r;
   MT_HIP(hipGetLastError());
-  if (c.timing) {
-    MT_HIP(hipEventRecord(e1, c.st[L]));
-    c.ev_pending.push_back({L, {e0, e1}});
-  }
+  if (c.timing) c.ev_pending.push_back({L, {e0, e1}});
   if (nee_after) {
     // reservoir NEE reads the pre-sub-step maps (rho/u[pr]) and the f just written
     int r = launch_nee(c);

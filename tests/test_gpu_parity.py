@@ -228,6 +228,28 @@ def test_set_field_then_step():
     compare(g, o, case)
 
 
+def test_interface_cell_on_both_sides_rejected():
+    """A level whose cell is both coarse-side (F2C) and fine-side (C2F) of an
+    interface is rejected at upload (the per-level streams rely on the split)."""
+    import ctypes
+    lib = amr.load()
+    d = amr.CaseDesc()
+    d.Q, d.Lmax, d.collision, d.ib, d.precision = 19, 1, 0, 0, 0
+    d.dims0[:] = [8, 8, 8]
+    d.tau0, d.rho0 = 0.6, 1.0
+    h = ctypes.c_void_p()
+    assert lib.mt_case_create(ctypes.byref(d), ctypes.byref(h)) == 0
+    try:
+        keys = np.array([0, 8], dtype=np.uint64)
+        kinds = np.array([C.KIND_FLUID, C.KIND_F2C | C.KIND_C2F], dtype=np.uint16)
+        rc = lib.mt_case_upload_level(h, 1, keys.ctypes.data_as(amr._u64p),
+                                      kinds.ctypes.data_as(amr._u16p), 2)
+        assert rc == -1
+        assert "both sides" in lib.mt_last_error().decode()
+    finally:
+        lib.mt_case_destroy(h)
+
+
 def test_missing_key_raises_lookup_error():
     """An absent key is MT_ERR_LOOKUP (the reference's std::out_of_range from at())."""
     case = C.cavity(8)
