@@ -40,10 +40,18 @@ static bool has_reservoir(const Case& c) {
 
 // Return the case to reference-state buffers: after mt_step the current fcol
 // buffer already holds the next collision and the reference's fcol is the other.
-static void settle(Case& c) {
-  if (!c.ahead) return;
+static int settle(Case& c) {
+  if (!c.ahead) return MT_OK;
+  if (c.xin && c.xprimed) {
+    // the last pair's explosion targets sit in fcol[1 - pf of the coarser level]
+    // (the other buffer already holds the next pair's): move them to the
+    // buffer that becomes the level's reference fcol
+    for (int L = 1; L <= c.Lmax; ++L) TRY(launch_copy_targets(c, L, 1 - c.lv[L - 1].pf, 1 - c.lv[L].pf));
+  }
+  c.xprimed = false;
   for (int L = 0; L <= c.Lmax; ++L) c.lv[L].pf ^= 1;
   c.ahead = false;
+  return MT_OK;
 }
 
 // The prologue of a step: calculateMacros(L) + collide(L) on every level, the
@@ -106,7 +114,8 @@ static int run_level(Case& c, int L, bool last, const CoalReq* coal) {
   // of sub-step A's explosion: A's explosion fills both buffers instead, and the
   // fused updates skip the C2F cells' fcol (always overwritten before a read).
   const bool both = (c.d.options & MT_OPT_EXPLODE_BOTH) != 0;
-  TRY(substep(c, L, false, false, false, false, both ? 2 : 1, !both, nullptr));
+  // (with c.xin the pair's targets are already in place: no explosion kernel)
+  TRY(substep(c, L, false, false, false, false, c.xin ? 0 : (both ? 2 : 1), !both, nullptr));
   return substep(c, L, last, last, false, last, 0, !both, coal);
 }
 
@@ -366,6 +375,9 @@ static int field_io(mt_case* c, int level, int field, const uint64_t* keys, int6
         r = MT_ERR_LOOKUP;
         break;
       }
+  // with the explosions inside the fused updates, an interface cell's fcol is
+  // the reference's only after settling
+  if (r == MT_OK && out && field == MT_FIELD_FCOL && c->xin) r = settle(*c);
   if (r == MT_OK) {
     DevLevel& l = c->lv[level];
     int buf = field == MT_FIELD_FCOL ? (c->ahead ? 1 - l.pf : l.pf) : l.pr;
@@ -377,7 +389,7 @@ static int field_io(mt_case* c, int level, int field, const uint64_t* keys, int6
         if (e != hipSuccess) { set_error(hipGetErrorString(e)); r = MT_ERR_HIP; }
       }
     } else {
-      settle(*c);
+      r = settle(*c);
       buf = field == MT_FIELD_FCOL ? l.pf : l.pr;
       hipError_t e = hipMemcpyAsync(dv, in, 8 * n * rows, hipMemcpyHostToDevice, c->stream);
       if (e != hipSuccess) { set_error(hipGetErrorString(e)); r = MT_ERR_HIP; }
@@ -409,26 +421,26 @@ int mt_set_field(mt_case* c, int level, int field, const uint64_t* keys, int64_t
 int mt_macros(mt_case* c, int level) {
   CHECK_FINAL(c);
   CHECK_LEVEL(c, level);
-  settle(*c);
+  TRY(settle(*c));
   return launch_macros(*c, level);
 }
 int mt_collide(mt_case* c, int level) {
   CHECK_FINAL(c);
   CHECK_LEVEL(c, level);
-  settle(*c);
+  TRY(settle(*c));
   return launch_collide(*c, level);
 }
 int mt_stream(mt_case* c, int level) {
   CHECK_FINAL(c);
   CHECK_LEVEL(c, level);
-  settle(*c);
+  TRY(settle(*c));
   return launch_stream(*c, level);
 }
 int mt_explode(mt_case* c, int level) {
   CHECK_FINAL(c);
   CHECK_LEVEL(c, level);
   if (level == 0) { set_error("explosion needs a coarser level"); return MT_ERR_ARG; }
-  settle(*c);
+  TRY(settle(*c));
   // the seam writes the single logical fcol; the ghost layer is mirrored into
   // the second buffer as the schedule does
   return launch_explode(*c, level);
@@ -437,17 +449,17 @@ int mt_coalesce(mt_case* c, int level) {
   CHECK_FINAL(c);
   CHECK_LEVEL(c, level);
   if (level == c->Lmax) { set_error("coalescence needs a finer level"); return MT_ERR_ARG; }
-  settle(*c);
+  TRY(settle(*c));
   return launch_coalesce(*c, level, false);
 }
 int mt_apply_bc(mt_case* c) {
   CHECK_FINAL(c);
-  settle(*c);
+  TRY(settle(*c));
   return launch_bc(*c);
 }
 int mt_ib_treat(mt_case* c, double* force3) {
   CHECK_FINAL(c);
-  settle(*c);
+  TRY(settle(*c));
   if (c->d.ib == MT_IB_NONE || c->lv[c->Lmax].n_dis == 0) {
     if (force3) force3[0] = force3[1] = force3[2] = 0;
     return MT_OK;
@@ -469,6 +481,11 @@ int mt_step(mt_case* c, int n) {
   if (n < 0) { set_error("negative step count"); return MT_ERR_ARG; }
   if (n == 0) return MT_OK;
   if (!c->ahead) TRY(prologue(*c));
+  if (c->xin && !c->xprimed) {
+    // the first pair's explosion targets, from the coarser levels' next collision
+    for (int L = 1; L <= c->Lmax; ++L) TRY(launch_explode_to(*c, L, c->lv[L - 1].pf));
+    c->xprimed = true;
+  }
   use_level_streams(*c, true);
   int r = MT_OK;
   for (int i = 0; i < n && r == MT_OK; ++i) r = coarse_step(*c, i == n - 1);

@@ -57,6 +57,14 @@ struct DevLevel {
   // applies; fine side: parent slot per aligned group of 8 slots, -1 none)
   bool coal_in_fused = false;
   int32_t* cpar = nullptr;
+  // explosions inside the fused updates (Case::xin).  Fine side: per slot, bit q
+  // set when the pull source x + e_q is an explosion target (C2F / ghost_C2F, not
+  // in dis), read from the pair's target buffer; the target slots.  Coarse side:
+  // per slot, the row of a ghost_F2C parent in gf2c_child, else -1.
+  uint32_t* tmask = nullptr;
+  int32_t* xt_list = nullptr;
+  int64_t n_xt = 0;
+  int32_t* gxrow = nullptr;
 };
 
 // a coalescence into level L-1 requested from level L's last sub-step
@@ -75,6 +83,16 @@ struct Case {
   bool ahead = false;  // fcol[pf] already holds the next collision (mt_step state)
   bool graphs = true;
   bool timing = false;
+  // Explosions inside the fused updates (with MT_OPT_EXPLODE_BOTH).  Both
+  // explosions of a fine pair write the same values, so the pair's targets are
+  // written once, before the pair, into fcol[pf of the coarser level] of the
+  // fine level (never otherwise written there), and the fine pulls read them
+  // from that buffer: F2C parents are exploded by the fine level's coalescing
+  // update, ghost_F2C parents by the coarse level's update.  xprimed: the
+  // targets for the next pair are in place (an explosion kernel primes them
+  // at the start of mt_step).
+  bool xin = false;
+  bool xprimed = false;
   hipStream_t stream = nullptr;
   // per-level streams used by mt_step (level 0 uses `stream`); st[L] is the
   // stream a level's kernels go to: `stream` for the seam entry points
@@ -131,6 +149,9 @@ int launch_macros(Case& c, int L);
 int launch_collide(Case& c, int L);
 int launch_stream(Case& c, int L);
 int launch_explode(Case& c, int L, bool mirror_all = false);
+int launch_explode_to(Case& c, int L, int dst_buf);
+double explode_scale(const Case& c, int L);
+int launch_copy_targets(Case& c, int L, int from_buf, int to_buf);
 int launch_coalesce(Case& c, int L, bool collide_after, bool write_macros = true);
 int launch_bc(Case& c);
 int launch_ib(Case& c, bool reduce_into_acc);

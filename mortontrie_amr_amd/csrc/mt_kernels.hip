@@ -322,7 +322,38 @@ struct FusedArgs {
   void* cu_out;
   double ctau, cscale, rho0;
   int c_write_macros;
+  // explosions inside the update (Case::xin)
+  const uint32_t* __restrict__ tmask;  // pull sources that are explosion targets
+  const void* xbuf;                    // this pair's explosion targets (fcol[pf of level L-1])
+  const int32_t* __restrict__ gxrow;   // ghost_F2C parent -> row of gx_child, -1 none
+  const int32_t* __restrict__ gx_child;
+  void* gx_dst;                        // level L+1 targets of the next pair
+  int64_t ns_f;
+  double gx_scale;
+  void* cx_dst;                        // with COAL: this level's targets of the next pair
+  double cx_scale;
 };
+
+// AMR_transDDF_explosion (LBM_Manager.cpp:1059-1129) of one parent held in a
+// lane: its post-collision fcol rescaled around the He-Luo equilibrium of its
+// own moments, written to its (up to 8, -1 = none) targets.
+template <int Q, typename S, typename R>
+__device__ __forceinline__ void explode_lane(const R* fc, const int32_t* child, S* dst, int64_t ns, R sc, R rho0) {
+  R rho, ux, uy, uz;
+  cell_macros<Q, R>(fc, rho, ux, uy, uz);
+  const R uu = ux * ux + uy * uy + uz * uz;
+  int t[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) t[j] = child[j];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, rho0);
+    const R v = fe + sc * (fc[q] - fe);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (t[j] >= 0) store_pop<Q, S, R>(dst, ns, q, t[j], v);
+  }
+}
 
 // SinglePointInterpolation::treat (ComplexBoundary_Manager.cpp:63-225) on one
 // treated cell, in the lane that streams it.  On entry f holds the pulled
@@ -457,18 +488,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     for (int q = 0; q < Q; ++q) f[q] = 0;
   }
   double fx = 0, fy = 0, fz = 0;  // momentum exchange of this lane's treated cell
+  const int s = (b << 6) | lane;
+  const uint16_t k = a.v.kind[s];
   {
     int N[27];
 #pragma unroll
     for (int o = 0; o < 27; ++o) N[o] = __builtin_amdgcn_readfirstlane(a.v.bnbr[(int64_t)b * 27 + o]);
-    const int s = (b << 6) | lane;
-    const uint16_t k = a.v.kind[s];
     const int row = (a.do_ib && (k & MT_KIND_DIS) && (k & MT_KIND_FLUID)) ? a.dis_row[s] : -1;
     const bool treated = row >= 0;
     if (k & (MT_KIND_FLUID | MT_KIND_C2F)) {
       int lx, ly, lz;
       brick_coords(lane, lx, ly, lz);
       const uint32_t pmx = a.v.pmask[s];
+      const uint32_t tm = a.tmask ? a.tmask[s] : 0u;
+      const S* xb = (const S*)a.xbuf;
       f[0] = load_pop<Q, S, R>(fin, ns, 0, s);
       // pull f_q(x) <- fcol_q(x + e_q); a missing source keeps the cell's own f_q
       // (Stream_Manager.cpp:53-62), except on a treated solid-boundary cell,
@@ -479,7 +512,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         constexpr int q = decltype(qc)::value;
         const int t = nbr_slot_c<kEX[q], kEY[q], kEZ[q]>(N, lx, ly, lz);
         const bool p = (pmx >> q) & 1;
-        const S* src = p ? fin + (int64_t)q * ns + t
+        const S* pb = ((tm >> q) & 1) ? xb : fin;
+        const S* src = p ? pb + (int64_t)q * ns + t
                          : (treated ? fin + (int64_t)qinv(q) * ns + s : fst + (int64_t)q * ns + s);
         R v = (R)Store<S>::ld(src);
         if (Store<S>::shifted) v += (R)wq<Q>(q);
@@ -523,7 +557,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         collide_cell<Q, R>(f, rho, ux, uy, uz, (R)a.tau, (R)a.lesC, a.model);
 #pragma unroll
         for (int q = 0; q < Q; ++q) store_pop<Q, S, R>(fout, ns, q, s, f[q]);
-      } else if (a.c2f_fcol) {
+        if (a.gxrow) {
+          // a ghost_F2C parent explodes its next collision into the finer
+          // level's ghost_C2F targets of the next pair
+          const int gr = a.gxrow[s];
+          if (gr >= 0)
+            explode_lane<Q, S, R>(f, a.gx_child + (int64_t)gr * 8, (S*)a.gx_dst, a.ns_f, (R)a.gx_scale, (R)a.rho0);
+        }
+      } else if (a.c2f_fcol || (k & MT_KIND_DIS)) {
+        // (a C2F cell in dis is never exploded into: it keeps the stream-copy fcol)
 #pragma unroll
         for (int q = 0; q < Q; ++q) store_pop<Q, S, R>(fout, ns, q, s, f[q]);
       }
@@ -558,30 +600,50 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     for (int t = 1; t < 8; ++t) sum += __shfl(f[q], b0 + t, 64);
     if ((lane & 7) == 0) lavg[lane >> 3][q] = sum / (R)8.0;
   }
-  if ((lane & 7) || par < 0) return;
-  R avg[Q];
+  const int g = lane >> 3;
+  if (!(lane & 7) && par >= 0) {
+    R avg[Q];
 #pragma unroll
-  for (int q = 0; q < Q; ++q) avg[q] = lavg[lane >> 3][q];
-  R rho, ux, uy, uz;
-  cell_macros<Q, R>(avg, rho, ux, uy, uz);
-  const R uu = ux * ux + uy * uy + uz * uz;
-  const R sc = (R)a.cscale;
+    for (int q = 0; q < Q; ++q) avg[q] = lavg[g][q];
+    R rho, ux, uy, uz;
+    cell_macros<Q, R>(avg, rho, ux, uy, uz);
+    R uu = ux * ux + uy * uy + uz * uz;
+    const R sc = (R)a.cscale;
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)a.rho0);
-    avg[q] = fe + sc * (avg[q] - fe);
-    store_pop<Q, S, R>((S*)a.cf, a.ns_c, q, par, avg[q]);
+    for (int q = 0; q < Q; ++q) {
+      const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)a.rho0);
+      avg[q] = fe + sc * (avg[q] - fe);
+      store_pop<Q, S, R>((S*)a.cf, a.ns_c, q, par, avg[q]);
+    }
+    cell_macros<Q, R>(avg, rho, ux, uy, uz);
+    if (a.c_write_macros) {
+      st_real<R>(a.crho_out, par, rho);
+      st_real<R>(a.cu_out, par, ux);
+      st_real<R>(a.cu_out, a.ns_c + par, uy);
+      st_real<R>(a.cu_out, 2 * a.ns_c + par, uz);
+    }
+    collide_cell<Q, R>(avg, rho, ux, uy, uz, (R)a.ctau, (R)a.lesC, a.model);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)a.cfcol_out, a.ns_c, q, par, avg[q]);
+    if (a.cx_dst) {
+      // the parent's explosion for the next pair, handed to its 8 children
+      cell_macros<Q, R>(avg, rho, ux, uy, uz);
+      uu = ux * ux + uy * uy + uz * uz;
+      const R sx = (R)a.cx_scale;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)a.rho0);
+        lavg[g][q] = fe + sx * (avg[q] - fe);
+      }
+    }
   }
-  cell_macros<Q, R>(avg, rho, ux, uy, uz);
-  if (a.c_write_macros) {
-    st_real<R>(a.crho_out, par, rho);
-    st_real<R>(a.cu_out, par, ux);
-    st_real<R>(a.cu_out, a.ns_c + par, uy);
-    st_real<R>(a.cu_out, 2 * a.ns_c + par, uz);
-  }
-  collide_cell<Q, R>(avg, rho, ux, uy, uz, (R)a.ctau, (R)a.lesC, a.model);
+  if (!a.cx_dst) return;  // uniform
+  __syncthreads();
+  // children in dis are not exploded into (LBM_Manager.cpp:1095)
+  if (par >= 0 && !(k & MT_KIND_DIS)) {
 #pragma unroll
-  for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)a.cfcol_out, a.ns_c, q, par, avg[q]);
+    for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)a.cx_dst, ns, q, s, lavg[g][q]);
+  }
 }
 
 // fixed-order sum of the per-block IB force partials into the accumulator
@@ -1130,6 +1192,10 @@ int launch_ib(Case& c, bool acc) {
   return MT_OK;
 }
 
+double explode_scale(const Case& c, int L) {
+  return (c.d.options & MT_OPT_CONVECTIVE_XFER) ? c.tau[L] / (2.0 * c.tau[L - 1]) : (2.0 * c.tau[L]) / c.tau[L - 1];
+}
+
 int launch_explode(Case& c, int L, bool mirror_all) {
   if (L <= 0) return MT_OK;
   DevLevel& lc = c.lv[L - 1];
@@ -1141,8 +1207,7 @@ int launch_explode(Case& c, int L, bool mirror_all) {
   a.src = lc.fcol[lc.pf];
   a.dst = lf.fcol[lf.pf];
   a.dst2 = lf.fcol[1 - lf.pf];
-  a.scale = (c.d.options & MT_OPT_CONVECTIVE_XFER) ? c.tau[L] / (2.0 * c.tau[L - 1])
-                                                   : (2.0 * c.tau[L]) / c.tau[L - 1];
+  a.scale = explode_scale(c, L);
   a.rho0 = c.d.rho0;
   a.collide = 0;
   a.list = lc.xp_list;
@@ -1151,6 +1216,49 @@ int launch_explode(Case& c, int L, bool mirror_all) {
   // mirror_all: every target also goes to the second buffer, which then holds
   // what the next sub-step's explosion would write
   MT_DISPATCH(c, (k_explode<Q_, S_><<<(a.n + 7) / 8, 256, 0, c.st[L]>>>(a, mirror_all ? 0 : lc.n_f2c)));
+  MT_HIP(hipGetLastError());
+  return MT_OK;
+}
+
+// All targets of level L's explosion, once, into fcol[dst_buf] of level L
+// (explosions inside the fused updates: priming the first pair).
+int launch_explode_to(Case& c, int L, int dst_buf) {
+  if (L <= 0) return MT_OK;
+  DevLevel& lc = c.lv[L - 1];
+  DevLevel& lf = c.lv[L];
+  if (!lc.n_xp) return MT_OK;
+  XferArgs a;
+  a.ns_c = lc.n_slots;
+  a.ns_f = lf.n_slots;
+  a.src = lc.fcol[lc.pf];
+  a.dst = lf.fcol[dst_buf];
+  a.dst2 = nullptr;
+  a.scale = explode_scale(c, L);
+  a.rho0 = c.d.rho0;
+  a.collide = 0;
+  a.list = lc.xp_list;
+  a.n = lc.n_xp;
+  a.child = lc.xp_child;
+  MT_DISPATCH(c, (k_explode<Q_, S_><<<(a.n + 7) / 8, 256, 0, c.stream>>>(a, a.n)));
+  MT_HIP(hipGetLastError());
+  return MT_OK;
+}
+
+template <int Q, typename S>
+__global__ void k_copy_targets(const int32_t* list, int64_t n, int64_t ns, const S* from, S* to) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t s = list[i];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) to[q * ns + s] = from[q * ns + s];
+}
+
+// Copy the explosion targets of level L between its fcol buffers.
+int launch_copy_targets(Case& c, int L, int from_buf, int to_buf) {
+  DevLevel& l = c.lv[L];
+  if (!l.n_xt || from_buf == to_buf) return MT_OK;
+  MT_DISPATCH(c, (k_copy_targets<Q_, S_><<<nblk(l.n_xt), 256, 0, c.stream>>>(
+                     l.xt_list, l.n_xt, l.n_slots, (const S_*)l.fcol[from_buf], (S_*)l.fcol[to_buf])));
   MT_HIP(hipGetLastError());
   return MT_OK;
 }
@@ -1234,6 +1342,33 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros
   a.ctau = a.cscale = 0;
   a.rho0 = c.d.rho0;
   a.c_write_macros = 0;
+  a.tmask = nullptr;
+  a.xbuf = nullptr;
+  a.gxrow = nullptr;
+  a.gx_child = nullptr;
+  a.gx_dst = nullptr;
+  a.ns_f = 0;
+  a.gx_scale = 0;
+  a.cx_dst = nullptr;
+  a.cx_scale = 0;
+  if (c.xin) {
+    if (L >= 1) {
+      a.tmask = l.tmask;
+      a.xbuf = l.fcol[c.lv[L - 1].pf];
+    }
+    if (L < c.Lmax && l.gxrow) {
+      DevLevel& lf = c.lv[L + 1];
+      a.gxrow = l.gxrow;
+      a.gx_child = l.gf2c_child;
+      a.gx_dst = lf.fcol[1 - l.pf];
+      a.ns_f = lf.n_slots;
+      a.gx_scale = explode_scale(c, L + 1);
+    }
+    if (coal) {
+      a.cx_dst = l.fcol[1 - c.lv[L - 1].pf];
+      a.cx_scale = explode_scale(c, L);
+    }
+  }
   if (coal) {
     if (L == 0 || !c.lv[L - 1].coal_in_fused || !l.cpar) {
       set_error("coalescence in the fused update requested where it does not apply");

@@ -317,6 +317,54 @@ __global__ void k_cpar(const int32_t* f2c, const int32_t* child, int64_t n, cons
   if (ok) cpar[c0 >> 3] = f2c[i];
   else *bad = 1;
 }
+// Explosions inside the fused updates: bit q of tmask[s] is set when the pull
+// source x + e_q of a streamed cell is an explosion target (C2F / ghost_C2F not
+// in dis), whose value the pull then reads from the pair's target buffer.
+__global__ void k_tmask(const int32_t* bnbr, const uint16_t* kind, const uint32_t* pmask, int64_t ns, int Q,
+                        uint32_t* tmask) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  uint32_t m = 0;
+  if (kind[s] & (MT_KIND_FLUID | MT_KIND_C2F)) {
+    int lx, ly, lz;
+    brick_coords((int)(s & 63), lx, ly, lz);
+    const int* nb27 = bnbr + (s >> 6) * 27;
+    const uint32_t pm = pmask[s];
+    for (int q = 1; q < Q; ++q) {
+      if (!((pm >> q) & 1)) continue;
+      const int t = slot_at(nb27, lx, ly, lz, kEX[q], kEY[q], kEZ[q]);
+      if (t >= 0 && (kind[t] & (MT_KIND_C2F | MT_KIND_GHOST_C2F)) && !(kind[t] & MT_KIND_DIS)) m |= 1u << q;
+    }
+  }
+  tmask[s] = m;
+}
+__global__ void k_gxrow(const int32_t* gf2c, int64_t n, int32_t* gxrow) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) gxrow[gf2c[i]] = (int32_t)i;
+}
+__global__ void k_flag_target(const uint16_t* kind, int64_t ns, int* flag) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  flag[s] = (kind[s] & (MT_KIND_C2F | MT_KIND_GHOST_C2F)) && !(kind[s] & MT_KIND_DIS);
+}
+// flag[i] = 1 when treated cell list[i] has an explosion target among its 26
+// neighbours (its link treatment would read the pair's target buffer)
+__global__ void k_ib_touch(const int32_t* list, int64_t n, const int32_t* bnbr, const uint16_t* kind, int* flag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = list[i];
+  int lx, ly, lz;
+  brick_coords(s & 63, lx, ly, lz);
+  const int* nb27 = bnbr + (int64_t)(s >> 6) * 27;
+  int hit = 0;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int t = slot_at(nb27, lx, ly, lz, dx, dy, dz);
+        if (t >= 0 && (kind[t] & (MT_KIND_C2F | MT_KIND_GHOST_C2F))) hit = 1;
+      }
+  flag[i] = hit;
+}
 __global__ void k_flag_brick(const uint16_t* kind, int64_t nb, uint16_t mask, int* flag) {
   int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
@@ -583,6 +631,52 @@ int build_tables(Case& c) {
       MT_HIP(hipFree(d_bad));
       l.coal_in_fused = hbad == 0;
     }
+    // explosions inside the fused updates: needs the in-kernel coalescence on
+    // every interface and no treated cell next to an explosion target
+    bool xin = (c.d.options & MT_OPT_EXPLODE_BOTH) && c.Lmax >= 1;
+    for (int L = 0; L < c.Lmax && xin; ++L) xin = c.lv[L].coal_in_fused;
+    DevLevel& lm = c.lv[c.Lmax];
+    if (xin && lm.n_ibl) {
+      int* d_hit = nullptr;
+      MT_HIP(hipMalloc((void**)&d_hit, 4 * lm.n_ibl));
+      k_ib_touch<<<nblk(lm.n_ibl), 256, 0, c.stream>>>(lm.ibl, lm.n_ibl, lm.bnbr, lm.kind, d_hit);
+      MT_HIP(hipGetLastError());
+      std::vector<int> h(lm.n_ibl);
+      MT_HIP(hipMemcpyAsync(h.data(), d_hit, 4 * lm.n_ibl, hipMemcpyDeviceToHost, c.stream));
+      MT_HIP(hipStreamSynchronize(c.stream));
+      MT_HIP(hipFree(d_hit));
+      for (int v : h) xin = xin && v == 0;
+    }
+    for (int L = 0; L <= c.Lmax && xin; ++L) {
+      DevLevel& l = c.lv[L];
+      if (L >= 1) {
+        l.tmask = c.alloc<uint32_t>(l.n_slots);
+        int* flag = nullptr;
+        MT_HIP(hipMalloc((void**)&flag, 4 * (l.n_slots + 1)));
+        k_flag_target<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.kind, l.n_slots, flag);
+        int r = compact_list(c, flag, l.n_slots, &l.xt_list, &l.n_xt);
+        if (r) return r;
+        MT_HIP(hipStreamSynchronize(c.stream));
+        MT_HIP(hipFree(flag));
+        if (!l.tmask) {
+          set_error("out of device memory");
+          return MT_ERR_HIP;
+        }
+        k_tmask<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.bnbr, l.kind, l.pmask, l.n_slots, c.Q, l.tmask);
+      }
+      if (L < c.Lmax && l.n_gf2c) {
+        l.gxrow = c.alloc<int32_t>(l.n_slots);
+        if (!l.gxrow) {
+          set_error("out of device memory");
+          return MT_ERR_HIP;
+        }
+        k_fill_i32<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.gxrow, l.n_slots, -1);
+        k_gxrow<<<nblk(l.n_gf2c), 256, 0, c.stream>>>(l.gf2c, l.n_gf2c, l.gxrow);
+      }
+      MT_HIP(hipGetLastError());
+    }
+    MT_HIP(hipStreamSynchronize(c.stream));
+    c.xin = xin;
   }
   if (herr & 4) {
     set_error("an F2C cell has a child missing on the finer level");
