@@ -50,6 +50,7 @@ struct DevLevel {
   int32_t* ibl = nullptr;              // treated dis slots (dis and lat_f), finest level
   int64_t n_ibl = 0;
   double* dis = nullptr;               // [n_dis][Q-1]
+  uint32_t* wmask = nullptr;           // [n_slots] bit q: link q of a dis cell is a wall link
   int64_t n_dis = 0;
   int32_t* bcres = nullptr;
   int64_t n_bcres = 0;

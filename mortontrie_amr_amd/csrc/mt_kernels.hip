@@ -308,6 +308,7 @@ struct FusedArgs {
   int ib_method;
   const int32_t* __restrict__ dis_row;
   const double* __restrict__ dis;
+  const uint32_t* __restrict__ wmask;  // wall links of the dis cells
   double* force_slots;  // [n_active][3] per-brick running momentum-exchange sums
   double* force_acc;
   int write_macros;
@@ -374,14 +375,13 @@ __device__ __forceinline__ void ib_links_cell(R* f, const Args& a, const int* N,
   const int64_t ns = a.v.ns;
   const double* drow = a.dis + (int64_t)row * (Q - 1);
   const R rho_b = ld_real<R>(a.v.rho_in, s);
-  uint32_t wm = 0;
+  const uint32_t wm = a.wmask[s];
   R wxx = 0, wyy = 0, wzz = 0, wxy = 0, wxz = 0, wyz = 0, vx = 0, vy = 0, vz = 0;
   static_for<1, Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     constexpr int qi = qinv(q);
-    const double dist = drow[qi - 1];
-    if (dist != -1.0) {
-      wm |= 1u << q;
+    if ((wm >> q) & 1) {
+      const double dist = drow[qi - 1];
       const R finc = load_pop<Q, S, R>(fin, ns, qi, s);
       R fs = finc;
       if ((pm >> qi) & 1) fs = load_pop<Q, S, R>(fin, ns, q, nbr_slot_c<-kEX[q], -kEY[q], -kEZ[q]>(N, lx, ly, lz));
@@ -1327,6 +1327,7 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros
   a.ib_method = c.d.ib;
   a.dis_row = l.dis_row;
   a.dis = l.dis;
+  a.wmask = l.wmask;
   a.force_slots = c.force_slots;
   a.force_acc = c.force_acc;
   if (a.do_ib && l.n_active > c.force_part_cap) {

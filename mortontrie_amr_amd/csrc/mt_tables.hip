@@ -416,12 +416,71 @@ __global__ void k_dis_rows(const uint64_t* ks, const int32_t* vs, int64_t n, Lev
     kind[s] |= MT_KIND_DIS;
   }
 }
+// wall-link mask of the dis cells: bit q set when dis[inv(q) - 1] != -1
+// (SinglePointInterpolation::treat, ComplexBoundary_Manager.cpp:77-80)
+__global__ void k_wmask(const int32_t* dis_row, const double* dis, int64_t ns, int Q, uint32_t* wmask) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  uint32_t m = 0;
+  const int row = dis_row[s];
+  if (row >= 0)
+    for (int q = 1; q < Q; ++q)
+      if (dis[(int64_t)row * (Q - 1) + qinv(q) - 1] != -1.0) m |= 1u << q;
+  wmask[s] = m;
+}
 __global__ void k_fill_i32(int32_t* a, int64_t n, int32_t v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
 }
 
 static inline int64_t nblk(int64_t n, int b = 256) { return n <= 0 ? 1 : (n + b - 1) / b; }
+
+// The fused kernel gives XCD x the active-list range [x*per, (x+1)*per), per =
+// ceil(n_active/8), and dispatches each range in list order.  Bricks whose
+// update has a longer dependent tail go first within their range, so they do
+// not finish last: treated solid-boundary cells, then coalescence groups, then
+// ghost_F2C parents (explosion stores), then the rest; Morton order otherwise.
+static int order_active(Case& c, int L) {
+  DevLevel& l = c.lv[L];
+  const int64_t na = l.n_active;
+  if (na <= 1) return MT_OK;
+  std::vector<int32_t> act(na);
+  std::vector<uint16_t> kind(l.n_slots);
+  std::vector<int32_t> drow(l.n_slots), cpar, gx;
+  MT_HIP(hipMemcpyAsync(act.data(), l.active, 4 * na, hipMemcpyDeviceToHost, c.stream));
+  MT_HIP(hipMemcpyAsync(kind.data(), l.kind, 2 * l.n_slots, hipMemcpyDeviceToHost, c.stream));
+  MT_HIP(hipMemcpyAsync(drow.data(), l.dis_row, 4 * l.n_slots, hipMemcpyDeviceToHost, c.stream));
+  if (l.cpar) {
+    cpar.resize(l.n_slots / 8);
+    MT_HIP(hipMemcpyAsync(cpar.data(), l.cpar, 4 * (l.n_slots / 8), hipMemcpyDeviceToHost, c.stream));
+  }
+  if (l.gxrow) {
+    gx.resize(l.n_slots);
+    MT_HIP(hipMemcpyAsync(gx.data(), l.gxrow, 4 * l.n_slots, hipMemcpyDeviceToHost, c.stream));
+  }
+  MT_HIP(hipStreamSynchronize(c.stream));
+  auto cls = [&](int32_t b) {
+    int k = 3;
+    for (int i = 0; i < 64; ++i) {
+      const int64_t s = (int64_t)b * 64 + i;
+      if ((kind[s] & MT_KIND_DIS) && (kind[s] & MT_KIND_FLUID) && drow[s] >= 0) return 0;
+      if (!cpar.empty() && (i & 7) == 0 && cpar[s >> 3] >= 0) k = std::min(k, 1);
+      if (!gx.empty() && gx[s] >= 0) k = std::min(k, 2);
+    }
+    return k;
+  };
+  const int64_t per = ((na + 7) / 8 * 8) / 8;
+  for (int64_t x = 0; x * per < na; ++x) {
+    auto b = act.begin() + x * per, e = act.begin() + std::min<int64_t>(na, (x + 1) * per);
+    std::vector<std::pair<int, int32_t>> v;
+    for (auto it = b; it != e; ++it) v.push_back({cls(*it), *it});
+    std::stable_sort(v.begin(), v.end(), [](auto& p, auto& q) { return p.first < q.first; });
+    for (size_t i = 0; i < v.size(); ++i) b[i] = v[i].second;
+  }
+  MT_HIP(hipMemcpyAsync(l.active, act.data(), 4 * na, hipMemcpyHostToDevice, c.stream));
+  MT_HIP(hipStreamSynchronize(c.stream));
+  return MT_OK;
+}
 
 static int compact_list(Case& c, const int* d_flag, int64_t n, int32_t** out, int64_t* count) {
   int* pos = nullptr;
@@ -543,6 +602,12 @@ int build_tables(Case& c) {
     lm.dis = c.alloc<double>(nd * m);
     lm.n_dis = nd;
     k_dis_rows<<<nblk(nd), 256, 0, c.stream>>>(ks, vs, nd, view(c, Lm), d_v, m, lm.dis_row, lm.dis, lm.kind);
+    lm.wmask = c.alloc<uint32_t>(lm.n_slots);
+    if (!lm.wmask) {
+      set_error("out of device memory");
+      return MT_ERR_HIP;
+    }
+    k_wmask<<<nblk(lm.n_slots), 256, 0, c.stream>>>(lm.dis_row, lm.dis, lm.n_slots, c.Q, lm.wmask);
     d_dis_sorted = c.alloc<uint64_t>(nd);
     MT_HIP(hipMemcpyAsync(d_dis_sorted, ks, 8 * nd, hipMemcpyDeviceToDevice, c.stream));
     MT_HIP(hipStreamSynchronize(c.stream));
@@ -677,6 +742,10 @@ int build_tables(Case& c) {
     }
     MT_HIP(hipStreamSynchronize(c.stream));
     c.xin = xin;
+    for (int L = 0; L <= c.Lmax; ++L) {
+      int r = order_active(c, L);
+      if (r) return r;
+    }
   }
   if (herr & 4) {
     set_error("an F2C cell has a child missing on the finer level");
