@@ -95,6 +95,19 @@ typedef struct mt_case_desc {
 
 typedef struct mt_case mt_case;
 
+/* One member of a packed case (independent flows sharing every launch).  Its
+ * cells live in the box [origin0, origin0 + dims0) of level-0 cells (and the
+ * box scaled by 2^L on level L); cells outside the box are never its
+ * neighbours, and its periodic faces wrap inside the box.  origin0 must be a
+ * multiple of 4; periodic extents too. */
+typedef struct mt_member_desc {
+  int32_t origin0[3];
+  int32_t dims0[3];
+  int32_t face_kind[6];
+  double face_u[6][3];
+  double tau0, les_C, rho0, u0[3];
+} mt_member_desc;
+
 const char* mt_last_error(void);
 
 /* ---- lifecycle: replaces LBM_Manager::LBM_Manager + allocMacrosAndDDF
@@ -108,6 +121,12 @@ int mt_case_upload_dis(mt_case* c, const uint64_t* keys, const double* fractions
 /* sorts the keys on the device, builds bricks, neighbour / parent-child tables,
  * sets f = fcol = feq(rho0,u0) (solid 0) -- allocMacrosAndDDF + initialBC       */
 int mt_case_finalize(mt_case* c);
+/* Packed batching (no counterpart in the reference, which runs one flow per
+ * process): replace the case's single member, described by mt_case_desc, by n
+ * members sharing its Q, Lmax, collision, IB method, precision and options.
+ * Between mt_case_create and mt_case_finalize; keys of every member are
+ * uploaded together, in the case's coordinates.                              */
+int mt_case_set_members(mt_case* c, const mt_member_desc* members, int32_t n);
 int mt_case_level_info(mt_case* c, int level, int64_t* n_cells, int64_t* n_bricks);
 
 /* ---- key lookup: replaces D_map_define<T>::find (mortonTrieMap.cpp:116-126,
@@ -145,6 +164,11 @@ int mt_ib_treat(mt_case* c, double* force3);
  *      fused per-level kernels, captured as HIP graphs when graphs are on.    */
 int mt_step(mt_case* c, int n_coarse);
 int mt_set_graphs(mt_case* c, int on);
+/* Launch-configuration autotuner (no counterpart in the reference): tries the
+ * fused kernel's block -> brick maps and level concurrency, each for `steps`
+ * coarse steps of the real run (+1 untimed), keeps the fastest.  Results do not
+ * depend on the choice.  ms_per_step[4] may be NULL.                         */
+int mt_autotune(mt_case* c, int steps, double* ms_per_step);
 /* per-kernel event timing of mt_step (off by default; forces graphs off):
  * ms[level] accumulates the fused update kernel's time, calls[level] its count */
 int mt_set_timing(mt_case* c, int on);
@@ -156,6 +180,8 @@ int mt_sync(mt_case* c);
 int mt_reduce(mt_case* c, int level, double* mass, double* momentum3, int64_t* nan_count);
 /* accumulated IB force since the last reset (LBM_Manager.cpp:846-852, 964-979) */
 int mt_ib_force(mt_case* c, double* sum3, int64_t* count, int reset);
+/* the same per member: sum3[m][3] over member m's cells, fixed order          */
+int mt_ib_force_members(mt_case* c, double* sum3, int32_t n_members, int64_t* count, int reset);
 
 #ifdef __cplusplus
 }

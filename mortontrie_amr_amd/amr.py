@@ -37,11 +37,20 @@ class CaseDesc(C.Structure):
     ]
 
 
+class MemberDesc(C.Structure):
+    _fields_ = [
+        ("origin0", C.c_int32 * 3), ("dims0", C.c_int32 * 3), ("face_kind", C.c_int32 * 6),
+        ("face_u", (C.c_double * 3) * 6), ("tau0", C.c_double), ("les_C", C.c_double),
+        ("rho0", C.c_double), ("u0", C.c_double * 3),
+    ]
+
+
 EXPORTS = [
     "mt_last_error", "mt_case_create", "mt_case_destroy", "mt_case_upload_level", "mt_case_upload_dis",
     "mt_case_finalize", "mt_case_level_info", "mt_find", "mt_get_field", "mt_set_field", "mt_macros",
     "mt_collide", "mt_stream", "mt_explode", "mt_coalesce", "mt_apply_bc", "mt_ib_treat", "mt_step",
     "mt_set_graphs", "mt_set_timing", "mt_get_timing", "mt_sync", "mt_reduce", "mt_ib_force",
+    "mt_case_set_members", "mt_ib_force_members", "mt_autotune",
 ]
 
 _lib = None
@@ -80,6 +89,9 @@ def load():
     L.mt_sync.argtypes = [C.c_void_p]
     L.mt_reduce.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _i64p]
     L.mt_ib_force.argtypes = [C.c_void_p, _dp, _i64p, C.c_int]
+    L.mt_case_set_members.argtypes = [C.c_void_p, C.POINTER(MemberDesc), C.c_int32]
+    L.mt_ib_force_members.argtypes = [C.c_void_p, _dp, C.c_int32, _i64p, C.c_int]
+    L.mt_autotune.argtypes = [C.c_void_p, C.c_int, _dp]
     _lib = L
     return L
 
@@ -116,6 +128,20 @@ class AmrCase:
         h = C.c_void_p()
         _check(L.mt_case_create(C.byref(d), C.byref(h)))
         self.h = h
+        members = case.get("members")
+        if members:
+            md = (MemberDesc * len(members))()
+            for i, m in enumerate(members):
+                for a in range(3):
+                    md[i].origin0[a] = m["origin0"][a]
+                    md[i].dims0[a] = m["dims0"][a]
+                    md[i].u0[a] = m["u0"][a]
+                for f in range(6):
+                    md[i].face_kind[f] = m["faces"][f][0]
+                    for a in range(3):
+                        md[i].face_u[f][a] = m["faces"][f][1 + a]
+                md[i].tau0, md[i].les_C, md[i].rho0 = m["tau0"], m["les_C"], m["rho0"]
+            _check(L.mt_case_set_members(self.h, md, len(members)))
         for lvl, lv in enumerate(case["levels"]):
             keys, pk = _u64(lv["keys"])
             kinds = np.ascontiguousarray(lv["kinds"], dtype=np.uint16)
@@ -211,6 +237,13 @@ class AmrCase:
     def set_graphs(self, on):
         _check(load().mt_set_graphs(self.h, 1 if on else 0))
 
+    def autotune(self, steps=5):
+        """Pick the fastest launch configuration (advances the run by 4 (steps + 1) coarse steps);
+        returns each candidate's ms per coarse step."""
+        ms = np.zeros(4)
+        _check(load().mt_autotune(self.h, steps, ms.ctypes.data_as(_dp)))
+        return ms
+
     def set_timing(self, on):
         _check(load().mt_set_timing(self.h, 1 if on else 0))
 
@@ -232,6 +265,26 @@ class AmrCase:
         n = C.c_int64()
         _check(load().mt_ib_force(self.h, s.ctypes.data_as(_dp), C.byref(n), 1 if reset else 0))
         return s, n.value
+
+    def ib_force_members(self, reset=False):
+        """Per-member force sums of a packed case (mt_ib_force_members): (n, 3) and the call count."""
+        nm = len(self.case.get("members") or [0])
+        s = np.zeros((nm, 3))
+        n = C.c_int64()
+        _check(load().mt_ib_force_members(self.h, s.ctypes.data_as(_dp), nm, C.byref(n), 1 if reset else 0))
+        return s, n.value
+
+    def drag_coefficients(self, reset=True):
+        """Cd of every member of a packed case (nan for members without a solid boundary)."""
+        s, n = self.ib_force_members(reset=reset)
+        out = []
+        for i, m in enumerate(self.case.get("members") or [{"cd": self.case["cd"]}]):
+            cd = m.get("cd") or {}
+            if n == 0 or "U" not in cd:
+                out.append(float("nan"))
+            else:
+                out.append(2.0 * (s[i, 0] / n) / (cd["U"] ** 2 * np.pi * cd["d"] ** 2 / 4.0))
+        return np.array(out)
 
     def drag_coefficient(self, reset=True):
         """Cd = 2 Fx / (rho U^2 pi d^2 / 4), Fx the mean IB force per treat() call

@@ -456,3 +456,100 @@ def reference_sphere(coarse=(24, 16, 16), radius=3.0, center=(8.0, 8.0, 8.0), sh
                                   ib_dist=sphere_ib_dist(C, radius), reservoir=True)
     return _base_case(Q, Lmax, tau0, collision, coarse, faces, levels, dk, dv, u0=(U, 0.0, 0.0), ib=ib,
                       cd={"U": U, "d": 2.0 * radius * (1 << Lmax)}, name=name or "reference_sphere")
+
+
+# ---- packed batching ---------------------------------------------------------
+def pack(members, gap=4):
+    """Pack independent cases into one case of members (mt_case_set_members).
+
+    Member i keeps its own cells, faces, relaxation time and initial state in
+    the box [origin0_i, origin0_i + dims0_i) of level-0 cells; the boxes are laid
+    side by side along x, 4-aligned, `gap` cells apart.  Keys are re-encoded into
+    the packed case's coordinates and Lmax (the largest member Lmax); the solid
+    boundary (dis) applies on the packed finest level, so members with dis must
+    reach it.  All members share Q, collision and options."""
+    if not members:
+        raise ValueError("no members")
+    Q, coll, opts = members[0]["Q"], members[0]["collision"], members[0].get("options", 0)
+    ibs = {m["ib"] for m in members if len(m["dis_keys"])}
+    if len(ibs) > 1:
+        raise ValueError("members with a solid boundary must share its method")
+    ib = ibs.pop() if ibs else IB_NONE
+    Lmax = max(m["Lmax"] for m in members)
+    for m in members:
+        if m["Q"] != Q or m["collision"] != coll or m.get("options", 0) != opts:
+            raise ValueError("members must share Q, collision and options")
+        if len(m["dis_keys"]) and m["Lmax"] != Lmax:
+            raise ValueError("a member with a solid boundary must reach the packed finest level")
+    origins, ox = [], 0
+    for m in members:
+        origins.append((ox, 0, 0))
+        ox += (m["dims0"][0] + 3) // 4 * 4 + gap
+    dims0 = (ox - gap, max(m["dims0"][1] for m in members), max(m["dims0"][2] for m in members))
+    levels = []
+    for L in range(Lmax + 1):
+        ks, kd = [], []
+        for m, o in zip(members, origins):
+            if L > m["Lmax"]:
+                continue
+            lv = m["levels"][L]
+            ks.append(member_keys_at(lv["keys"], L, m["Lmax"], o, Lmax))
+            kd.append(lv["kinds"])
+        keys = np.concatenate(ks) if ks else np.zeros(0, np.uint64)
+        kinds = np.concatenate(kd) if kd else np.zeros(0, np.uint16)
+        order = np.argsort(keys, kind="stable")
+        levels.append({"keys": keys[order], "kinds": kinds[order]})
+    dks, dvs = [], []
+    for m, o in zip(members, origins):
+        if len(m["dis_keys"]):
+            dks.append(member_keys_at(m["dis_keys"], Lmax, Lmax, o, Lmax))
+            dvs.append(m["dis_vals"])
+    dis_keys = np.concatenate(dks) if dks else np.zeros(0, np.uint64)
+    dis_vals = np.concatenate(dvs) if dvs else np.zeros((0, Q - 1))
+    mdesc = [{"origin0": o, "dims0": tuple(m["dims0"]), "faces": m["faces"], "tau0": m["tau0"],
+              "les_C": m["les_C"], "rho0": m["rho0"], "u0": m["u0"], "Lmax": m["Lmax"],
+              "cd": m.get("cd", {})} for m, o in zip(members, origins)]
+    case = _base_case(Q, Lmax, members[0]["tau0"], coll, dims0, [(BC_NONE, 0.0, 0.0, 0.0)] * 6, levels,
+                      dis_keys, dis_vals, ib=ib, name=f"packed[{len(members)}]", options=opts)
+    case["members"] = mdesc
+    return case
+
+
+def member_keys_at(keys, level, member_Lmax, origin0, packed_Lmax):
+    """A member's level-`level` keys in the packed case's coordinates."""
+    x, y, z = decode_key(np.asarray(keys, dtype=np.uint64), level, member_Lmax)
+    return encode_key(x + (origin0[0] << level), y + (origin0[1] << level), z + (origin0[2] << level),
+                      level, packed_Lmax)
+
+
+def member_keys(packed, i, level, keys):
+    """Translate member i's own keys on `level` into the packed case's keys."""
+    m = packed["members"][i]
+    return member_keys_at(keys, level, m["Lmax"], m["origin0"], packed["Lmax"])
+
+
+def cavity_batch(n_cases=256, seed=0, Q=19, edges=(16, 24, 32, 48, 64), options=2):
+    """The packed-batching workload: n_cases lid-driven cavities, edge L drawn
+    from `edges`, lid speed U ~ U[0.02, 0.08], tau ~ U[0.52, 0.9] (drawn per case
+    in the order L, U, tau from numpy.random.default_rng(seed)); every odd-index
+    case is two-level, refined in an axis-aligned box whose volume fraction is
+    drawn from U[0.1, 0.3] and whose lower corner is drawn uniformly among the
+    positions 2 or more cells from every face (box draws follow the case's
+    L, U, tau).  Returns the list of member cases (pack() joins them)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_cases):
+        L = int(rng.choice(np.asarray(edges)))
+        U = float(rng.uniform(0.02, 0.08))
+        tau = float(rng.uniform(0.52, 0.9))
+        box = None
+        if i % 2 == 1:
+            frac = float(rng.uniform(0.10, 0.30))
+            e = max(4, int(round(L * frac ** (1.0 / 3.0))))
+            e = min(e, L - 4)
+            lo = rng.integers(2, L - e - 2 + 1, size=3)
+            box = (tuple(int(v) for v in lo), tuple(int(v) + e for v in lo))
+        c = cavity(L, Q=Q, tau=tau, lid=U, refine_box=box, name=f"cavity{L}_{i}")
+        c["options"] = int(options)
+        out.append(c)
+    return out

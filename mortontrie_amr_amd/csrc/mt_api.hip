@@ -32,11 +32,7 @@ void set_error(const std::string& s) { g_err = s; }
 
 static int run_level(Case& c, int L, bool last, const CoalReq* coal);
 
-static bool has_reservoir(const Case& c) {
-  for (int f = 0; f < 6; ++f)
-    if (c.d.face_kind[f] == MT_BC_RESERVOIR) return true;
-  return false;
-}
+static bool has_reservoir(const Case& c) { return c.res_any; }
 
 // Return the case to reference-state buffers: after mt_step the current fcol
 // buffer already holds the next collision and the reference's fcol is the other.
@@ -120,7 +116,13 @@ static int run_level(Case& c, int L, bool last, const CoalReq* coal) {
 }
 
 static void use_level_streams(Case& c, bool on) {
+  on = on && c.level_streams;
   for (int L = 0; L <= c.Lmax; ++L) c.st[L] = (on && L > 0) ? c.lstream[L] : c.stream;
+}
+
+static void drop_graphs(Case& c) {
+  for (auto& g : c.graph_exec) hipGraphExecDestroy(g.second);
+  c.graph_exec.clear();
 }
 
 static int parity_key(const Case& c, bool last) {
@@ -207,8 +209,21 @@ int mt_case_create(const mt_case_desc* d, mt_case** out) {
   c->Q = d->Q;
   c->Lmax = d->Lmax;
   c->prec = d->precision;
-  // tau_L = 2^L (tau0 - 1/2) + 1/2: nu_lb doubles per level (LBM_Manager.cpp:814-818)
-  for (int L = 0; L <= c->Lmax; ++L) c->tau[L] = double(1 << L) * (d->tau0 - 0.5) + 0.5;
+  // the case is one member until mt_case_set_members says otherwise
+  mt_member_desc m = {};
+  for (int a = 0; a < 3; ++a) {
+    m.origin0[a] = 0;
+    m.dims0[a] = d->dims0[a];
+    m.u0[a] = d->u0[a];
+  }
+  for (int f = 0; f < 6; ++f) {
+    m.face_kind[f] = d->face_kind[f];
+    for (int a = 0; a < 3; ++a) m.face_u[f][a] = d->face_u[f][a];
+  }
+  m.tau0 = d->tau0;
+  m.les_C = d->les_C;
+  m.rho0 = d->rho0;
+  c->h_mdesc.assign(1, m);
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   for (int L = 1; L <= c->Lmax && e == hipSuccess; ++L) e = hipStreamCreateWithFlags(&c->lstream[L], hipStreamNonBlocking);
   for (int L = 0; L <= c->Lmax && e == hipSuccess; ++L) {
@@ -276,9 +291,54 @@ int mt_case_upload_dis(mt_case* c, const uint64_t* keys, const double* vals, int
   return MT_OK;
 }
 
+static int check_member(const mt_case_desc& d, const mt_member_desc& m, std::string& why) {
+  if (m.tau0 <= 0.5) { why = "tau0 must exceed 1/2"; return MT_ERR_ARG; }
+  for (int f = 0; f < 6; ++f)
+    if (m.face_kind[f] < MT_BC_NONE || m.face_kind[f] > MT_BC_PERIODIC) { why = "bad face kind"; return MT_ERR_ARG; }
+  for (int a = 0; a < 3; ++a) {
+    bool p0 = m.face_kind[2 * a] == MT_BC_PERIODIC, p1 = m.face_kind[2 * a + 1] == MT_BC_PERIODIC;
+    if (p0 != p1) { why = "periodic faces must come in pairs"; return MT_ERR_ARG; }
+    if (p0 && (m.dims0[a] % 4)) { why = "periodic axis extent must be a multiple of 4"; return MT_ERR_ARG; }
+    if (m.origin0[a] % 4 || m.origin0[a] < 0) { why = "member origin must be a non-negative multiple of 4"; return MT_ERR_ARG; }
+    if (m.dims0[a] <= 0 || m.origin0[a] + m.dims0[a] > d.dims0[a]) { why = "member box outside the case"; return MT_ERR_ARG; }
+  }
+  return MT_OK;
+}
+
+int mt_case_set_members(mt_case* c, const mt_member_desc* m, int32_t n) {
+  CHECK_CASE(c);
+  if (c->finalized) { set_error("case already finalized"); return MT_ERR_STATE; }
+  if (n <= 0 || !m) { set_error("no members"); return MT_ERR_ARG; }
+  for (int32_t i = 0; i < n; ++i) {
+    std::string why;
+    if (check_member(c->d, m[i], why) != MT_OK) {
+      set_error("member " + std::to_string(i) + ": " + why);
+      return MT_ERR_ARG;
+    }
+  }
+  // boxes are disjoint in whole bricks: each brick belongs to one member
+  for (int32_t i = 0; i < n; ++i)
+    for (int32_t j = i + 1; j < n; ++j) {
+      bool overlap = true;
+      for (int a = 0; a < 3; ++a) {
+        const int64_t ai = m[i].origin0[a], bi = ai + m[i].dims0[a];
+        const int64_t aj = m[j].origin0[a], bj = aj + m[j].dims0[a];
+        // brick-rounded extents
+        if ((bi + 3) / 4 * 4 <= aj || (bj + 3) / 4 * 4 <= ai) overlap = false;
+      }
+      if (overlap) {
+        set_error("member boxes " + std::to_string(i) + " and " + std::to_string(j) + " share a brick");
+        return MT_ERR_ARG;
+      }
+    }
+  c->h_mdesc.assign(m, m + n);
+  return MT_OK;
+}
+
 int mt_case_finalize(mt_case* c) {
   CHECK_CASE(c);
   if (c->finalized) { set_error("case already finalized"); return MT_ERR_STATE; }
+  TRY(build_members(*c));
   TRY(build_tables(*c));
   int64_t max_fp = 1;
   for (int L = 0; L <= c->Lmax; ++L) {
@@ -495,6 +555,53 @@ int mt_step(mt_case* c, int n) {
   return MT_OK;
 }
 
+// Autotuner of the launch configuration.  Each candidate (fused-kernel block ->
+// brick map x level concurrency) runs `steps` coarse steps of the real
+// simulation after one untimed step that captures its graphs; every candidate
+// computes the same values, so the tuning steps are ordinary steps.  The
+// fastest candidate is kept; ms_out[i] (may be NULL) receives candidate i's
+// time per coarse step, in the order (map 0, streams), (map 1, streams),
+// (map 0, one stream), (map 1, one stream).
+int mt_autotune(mt_case* c, int steps, double* ms_out) {
+  CHECK_FINAL(c);
+  if (steps <= 0) { set_error("steps must be positive"); return MT_ERR_ARG; }
+  const bool timing = c->timing;
+  c->timing = false;
+  hipEvent_t e0, e1;
+  MT_HIP(hipEventCreate(&e0));
+  MT_HIP(hipEventCreate(&e1));
+  double best = 1e300;
+  int best_i = 0;
+  int r = MT_OK;
+  for (int i = 0; i < 4 && r == MT_OK; ++i) {
+    c->xmap = i & 1;
+    c->level_streams = i < 2;
+    drop_graphs(*c);
+    r = mt_step(c, 1);
+    if (r != MT_OK) break;
+    if (hipEventRecord(e0, c->stream) != hipSuccess) { r = MT_ERR_HIP; break; }
+    r = mt_step(c, steps);
+    if (r != MT_OK) break;
+    if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { r = MT_ERR_HIP; break; }
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { r = MT_ERR_HIP; break; }
+    const double per = ms / steps;
+    if (ms_out) ms_out[i] = per;
+    if (per < best) {
+      best = per;
+      best_i = i;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  c->xmap = best_i & 1;
+  c->level_streams = best_i < 2;
+  drop_graphs(*c);
+  c->timing = timing;
+  if (r == MT_ERR_HIP) set_error("autotune: HIP event failure");
+  return r;
+}
+
 int mt_set_graphs(mt_case* c, int on) {
   CHECK_CASE(c);
   c->graphs = on != 0;
@@ -555,6 +662,42 @@ int mt_ib_force(mt_case* c, double* sum3, int64_t* count, int reset) {
     sum3[1] = s[1];
     sum3[2] = s[2];
   }
+  if (count) *count = (int64_t)(acc[3] + acc[7]);
+  if (reset) {
+    MT_HIP(hipMemsetAsync(c->force_acc, 0, 4 * sizeof(double), c->stream));
+    MT_HIP(hipMemsetAsync(c->force_acc + 7, 0, sizeof(double), c->stream));
+    MT_HIP(hipMemsetAsync(c->force_slots, 0, sizeof(double) * slots.size(), c->stream));
+    MT_HIP(hipStreamSynchronize(c->stream));
+  }
+  return MT_OK;
+}
+
+int mt_ib_force_members(mt_case* c, double* sum3, int32_t n_members, int64_t* count, int reset) {
+  CHECK_FINAL(c);
+  const int32_t nm = (int32_t)c->h_mem.size();
+  if (n_members != nm || (!sum3 && nm)) { set_error("member count mismatch"); return MT_ERR_ARG; }
+  double acc[8];
+  MT_HIP(hipMemcpyAsync(acc, c->force_acc, sizeof acc, hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> slots(3 * c->force_part_cap);
+  MT_HIP(hipMemcpyAsync(slots.data(), c->force_slots, sizeof(double) * slots.size(), hipMemcpyDeviceToHost,
+                        c->stream));
+  DevLevel& l = c->lv[c->Lmax];
+  std::vector<int32_t> act(l.n_active), bm(l.n_bricks);
+  if (l.n_active) {
+    MT_HIP(hipMemcpyAsync(act.data(), l.active, 4 * l.n_active, hipMemcpyDeviceToHost, c->stream));
+    MT_HIP(hipMemcpyAsync(bm.data(), l.bmember, 4 * l.n_bricks, hipMemcpyDeviceToHost, c->stream));
+  }
+  MT_HIP(hipStreamSynchronize(c->stream));
+  if (acc[3] != 0 && nm > 1) {
+    set_error("per-member forces cover mt_step only (mt_ib_treat was called on a packed case)");
+    return MT_ERR_STATE;
+  }
+  for (int32_t m = 0; m < nm; ++m)
+    for (int a = 0; a < 3; ++a) sum3[3 * m + a] = 0;
+  for (int a = 0; a < 3; ++a) sum3[a] += acc[a];
+  // the fused update's slot of block bi belongs to brick active[bi]
+  for (int64_t b = 0; b < l.n_active && b < c->force_part_cap; ++b)
+    for (int a = 0; a < 3; ++a) sum3[3 * bm[act[b]] + a] += slots[3 * b + a];
   if (count) *count = (int64_t)(acc[3] + acc[7]);
   if (reset) {
     MT_HIP(hipMemsetAsync(c->force_acc, 0, 4 * sizeof(double), c->stream));

@@ -13,6 +13,28 @@ namespace mt {
 
 constexpr int kMaxLevels = 7;
 
+// level-0 wet-node boundary configuration of one member
+struct BcCfg {
+  int kind[6];
+  double u[6][3];
+  int wet_mask;  // faces with MT_BC_ZOUHE or MT_BC_OUTFLOW
+};
+
+// Per-member parameters.  A case holds one or more independent members (packed
+// batching); every brick belongs to one member, and every kernel takes its
+// relaxation times, transfer factors, boundary configuration and initial state
+// from its brick's member.  A single case is member 0.
+struct MemberParams {
+  double tau[kMaxLevels];     // tau_L = 2^L (tau0 - 1/2) + 1/2
+  double xscale[kMaxLevels];  // explosion factor into level L (L >= 1)
+  double cscale[kMaxLevels];  // coalescence factor into level L (from L + 1)
+  double les_C, rho0, u0[3];
+  BcCfg bc;
+  int32_t lo[kMaxLevels][3];  // member box per level, cells [lo, hi)
+  int32_t hi[kMaxLevels][3];
+  int32_t periodic[3];
+};
+
 struct DevLevel {
   int64_t n_cells = 0, n_bricks = 0, n_slots = 0;
   int32_t dims[3] = {0, 0, 0};
@@ -66,6 +88,8 @@ struct DevLevel {
   int32_t* xt_list = nullptr;
   int64_t n_xt = 0;
   int32_t* gxrow = nullptr;
+  int32_t* bmember = nullptr;          // [n_bricks] member of the brick
+  std::vector<int32_t> h_members;      // upload staging: member of each cell (empty: all 0)
 };
 
 // a coalescence into level L-1 requested from level L's last sub-step
@@ -78,12 +102,20 @@ struct Case {
   int Q = 19, Lmax = 0, prec = MT_FP64;
   double tau[kMaxLevels];
   DevLevel lv[kMaxLevels];
+  std::vector<mt_member_desc> h_mdesc;  // members as given (one by default)
+  std::vector<MemberParams> h_mem;
+  MemberParams* d_mem = nullptr;
   std::vector<uint64_t> h_dis_keys;
   std::vector<double> h_dis_vals;
   bool finalized = false;
   bool ahead = false;  // fcol[pf] already holds the next collision (mt_step state)
   bool graphs = true;
   bool timing = false;
+  // launch configuration (results do not depend on it; mt_autotune picks it)
+  int xmap = 0;               // fused kernel block -> brick map (FusedArgs::xmap)
+  bool level_streams = true;  // levels on concurrent streams in mt_step
+  bool wet_any = false;  // some member has a Zou-He / outflow face
+  bool res_any = false;  // some member has a reservoir face
   // Explosions inside the fused updates (with MT_OPT_EXPLODE_BOTH).  Both
   // explosions of a fine pair write the same values, so the pair's targets are
   // written once, before the pair, into fcol[pf of the coarser level] of the
@@ -142,6 +174,7 @@ void set_error(const std::string& s);
   } while (0)
 
 // tables (mt_tables.hip)
+int build_members(Case& c);
 int build_tables(Case& c);
 int find_slots(Case& c, int level, const uint64_t* d_keys, int64_t n, int32_t* d_out);
 // kernels (mt_kernels.hip)
@@ -151,7 +184,6 @@ int launch_collide(Case& c, int L);
 int launch_stream(Case& c, int L);
 int launch_explode(Case& c, int L, bool mirror_all = false);
 int launch_explode_to(Case& c, int L, int dst_buf);
-double explode_scale(const Case& c, int L);
 int launch_copy_targets(Case& c, int L, int from_buf, int to_buf);
 int launch_coalesce(Case& c, int L, bool collide_after, bool write_macros = true);
 int launch_bc(Case& c);

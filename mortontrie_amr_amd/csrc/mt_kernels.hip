@@ -26,14 +26,11 @@ namespace mt {
 
 constexpr uint16_t kHasPop = MT_KIND_FLUID | MT_KIND_F2C | MT_KIND_C2F | MT_KIND_GHOST_C2F | MT_KIND_SOLID | MT_KIND_BCRES;
 
-struct BcCfg {
-  int kind[6];
-  double u[6][3];
-  int wet_mask;   // faces with MT_BC_ZOUHE or MT_BC_OUTFLOW
-};
-
 struct LvArgs {
   int64_t ns;
+  int L;
+  const MemberParams* __restrict__ mp;   // per-member parameters
+  const int32_t* __restrict__ bmember;   // [n_bricks] member of each brick
   const int32_t* __restrict__ bnbr;
   const uint16_t* __restrict__ kind;
   const uint32_t* __restrict__ pmask;
@@ -45,6 +42,9 @@ struct LvArgs {
   void* rho_out;
   void* u_out;
 };
+
+// parameters of the member owning slot s
+__device__ __forceinline__ const MemberParams& member_of(const LvArgs& v, int64_t s) { return v.mp[v.bmember[s >> 6]]; }
 
 // ------------------------------------------------------------------ addressing
 // slot of the neighbour at (DX,DY,DZ) in {-1,0,1}^3; N holds the 27
@@ -299,11 +299,10 @@ struct FusedArgs {
   LvArgs v;
   const int32_t* __restrict__ active;
   int64_t n_active;
-  double tau, lesC;
+  int xmap;  // block -> brick map: 0 one contiguous range per XCD, 1 in order
   int model;
   int write_f;
-  int do_bc;
-  BcCfg bc;
+  int do_bc;  // level 0 with a wet-node face in some member
   int do_ib;  // the finest level's solid-boundary links (SinglePointInterpolation::treat)
   int ib_method;
   const int32_t* __restrict__ dis_row;
@@ -321,7 +320,6 @@ struct FusedArgs {
   void* cfcol_out;  // coarse fcol after the F2C cell's collision
   void* crho_out;
   void* cu_out;
-  double ctau, cscale, rho0;
   int c_write_macros;
   // explosions inside the update (Case::xin)
   const uint32_t* __restrict__ tmask;  // pull sources that are explosion targets
@@ -330,9 +328,7 @@ struct FusedArgs {
   const int32_t* __restrict__ gx_child;
   void* gx_dst;                        // level L+1 targets of the next pair
   int64_t ns_f;
-  double gx_scale;
   void* cx_dst;                        // with COAL: this level's targets of the next pair
-  double cx_scale;
 };
 
 // AMR_transDDF_explosion (LBM_Manager.cpp:1059-1129) of one parent held in a
@@ -475,13 +471,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   const int lane = threadIdx.x;
   const int64_t nb = gridDim.x;
   const int64_t per_xcd = (nb + 7) >> 3;
-  const int64_t bi = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  const int64_t bi = a.xmap ? (int64_t)blockIdx.x : (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
   const S* fin = (const S*)a.v.fcol_in;
   S* fout = (S*)a.v.fcol_out;
   S* fst = (S*)a.v.f;
   const int64_t ns = a.v.ns;
   if (bi >= a.n_active) return;  // uniform over the block
   const int b = __builtin_amdgcn_readfirstlane(a.active[bi]);
+  const int L = a.v.L;
+  const MemberParams& P = a.v.mp[__builtin_amdgcn_readfirstlane(a.v.bmember[b])];
+  const double p_tau = P.tau[L], p_les = P.les_C;
   R f[Q];
   if (COAL) {
 #pragma unroll
@@ -533,7 +532,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
           }
           return load_pop<Q, S, R>(fst, ns, q, t);
         };
-        wet_bc<Q, R>(f, k, a.bc, nf);
+        wet_bc<Q, R>(f, k, P.bc, nf);
       }
       // A C2F cell's stored f is read by a separate coalescence kernel and, for
       // the directions with no source, by its own next pull; those entries never
@@ -554,7 +553,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
           st_real<R>(a.v.u_out, ns + s, uy);
           st_real<R>(a.v.u_out, 2 * ns + s, uz);
         }
-        collide_cell<Q, R>(f, rho, ux, uy, uz, (R)a.tau, (R)a.lesC, a.model);
+        collide_cell<Q, R>(f, rho, ux, uy, uz, (R)p_tau, (R)p_les, a.model);
 #pragma unroll
         for (int q = 0; q < Q; ++q) store_pop<Q, S, R>(fout, ns, q, s, f[q]);
         if (a.gxrow) {
@@ -562,7 +561,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
           // level's ghost_C2F targets of the next pair
           const int gr = a.gxrow[s];
           if (gr >= 0)
-            explode_lane<Q, S, R>(f, a.gx_child + (int64_t)gr * 8, (S*)a.gx_dst, a.ns_f, (R)a.gx_scale, (R)a.rho0);
+            explode_lane<Q, S, R>(f, a.gx_child + (int64_t)gr * 8, (S*)a.gx_dst, a.ns_f, (R)P.xscale[L + 1],
+                                  (R)P.rho0);
         }
       } else if (a.c2f_fcol || (k & MT_KIND_DIS)) {
         // (a C2F cell in dis is never exploded into: it keeps the stream-copy fcol)
@@ -608,10 +608,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     R rho, ux, uy, uz;
     cell_macros<Q, R>(avg, rho, ux, uy, uz);
     R uu = ux * ux + uy * uy + uz * uz;
-    const R sc = (R)a.cscale;
+    const R sc = (R)P.cscale[L - 1];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)a.rho0);
+      const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)P.rho0);
       avg[q] = fe + sc * (avg[q] - fe);
       store_pop<Q, S, R>((S*)a.cf, a.ns_c, q, par, avg[q]);
     }
@@ -622,17 +622,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
       st_real<R>(a.cu_out, a.ns_c + par, uy);
       st_real<R>(a.cu_out, 2 * a.ns_c + par, uz);
     }
-    collide_cell<Q, R>(avg, rho, ux, uy, uz, (R)a.ctau, (R)a.lesC, a.model);
+    collide_cell<Q, R>(avg, rho, ux, uy, uz, (R)P.tau[L - 1], (R)P.les_C, a.model);
 #pragma unroll
     for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)a.cfcol_out, a.ns_c, q, par, avg[q]);
     if (a.cx_dst) {
       // the parent's explosion for the next pair, handed to its 8 children
       cell_macros<Q, R>(avg, rho, ux, uy, uz);
       uu = ux * ux + uy * uy + uz * uz;
-      const R sx = (R)a.cx_scale;
+      const R sx = (R)P.xscale[L];
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
-        const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)a.rho0);
+        const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)P.rho0);
         lavg[g][q] = fe + sx * (avg[q] - fe);
       }
     }
@@ -680,7 +680,6 @@ __global__ void k_force_reduce(const double* part, int64_t nb, double* acc) {
 // ------------------------------------------------------------------ seams
 struct SeamArgs {
   LvArgs v;
-  double tau, lesC;
   int model;
 };
 
@@ -718,7 +717,8 @@ __global__ __launch_bounds__(256) void k_collide(SeamArgs a) {
     const R rho = ld_real<R>(a.v.rho_in, s);
     const R ux = ld_real<R>(a.v.u_in, s), uy = ld_real<R>(a.v.u_in, a.v.ns + s),
             uz = ld_real<R>(a.v.u_in, 2 * a.v.ns + s);
-    collide_cell<Q, R>(f, rho, ux, uy, uz, (R)a.tau, (R)a.lesC, a.model);
+    const MemberParams& P = member_of(a.v, s);
+    collide_cell<Q, R>(f, rho, ux, uy, uz, (R)P.tau[a.v.L], (R)P.les_C, a.model);
   }
 #pragma unroll
   for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)a.v.fcol_out, a.v.ns, q, s, f[q]);
@@ -745,11 +745,12 @@ __global__ __launch_bounds__(256) void k_stream(SeamArgs a) {
 
 // level-0 wet-node boundaries applied to f after stream
 template <int Q, typename S>
-__global__ __launch_bounds__(256) void k_bc_wet(SeamArgs a, BcCfg bc) {
+__global__ __launch_bounds__(256) void k_bc_wet(SeamArgs a) {
   typedef typename RealOf<S>::type R;
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.v.ns) return;
   const uint16_t k = a.v.kind[s];
+  const BcCfg& bc = member_of(a.v, s).bc;
   if (!(k & MT_KIND_FLUID) || !(((k >> 8) & 63) & bc.wet_mask)) return;
   S* fst = (S*)a.v.f;
   R f[Q];
@@ -771,12 +772,13 @@ __global__ __launch_bounds__(256) void k_bc_wet(SeamArgs a, BcCfg bc) {
 // (BC/NonEquilibrumExtrapolation.cpp:50-66), Jacobi form: every read of f is
 // of the pre-pass state; results staged in tmp[Q][n] then scattered.
 template <int Q, typename S>
-__global__ void k_nee(SeamArgs a, BcCfg bc, const int32_t* list, int64_t n, double* tmp) {
+__global__ void k_nee(SeamArgs a, const int32_t* list, int64_t n, double* tmp) {
   typedef typename RealOf<S>::type R;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int s = list[i];
   const uint16_t k = a.v.kind[s];
+  const BcCfg& bc = member_of(a.v, s).bc;
   const int fm = (k >> 8) & 63;
   const S* fst = (const S*)a.v.f;
   const R rb = ld_real<R>(a.v.rho_in, s);
@@ -862,13 +864,15 @@ struct XferArgs {
   const int32_t* list; // parent slots
   int64_t n;
   const int32_t* child;  // [n][8]
-  double scale, rho0;
+  // factors, relaxation and reference density from the parent's member
+  const MemberParams* mp;
+  const int32_t* bmember_c;  // coarse level's brick -> member
+  int Lc;                    // coarse level
   // coalescence + collision of the F2C cell
   int collide;
   void* fcol_out;
   void* rho_out;
   void* u_out;
-  double tau, lesC;
   int model;
   int write_macros;
 };
@@ -899,8 +903,9 @@ __global__ __launch_bounds__(256) void k_explode(XferArgs a, int64_t n_plain) {
     R rho, ux, uy, uz;
     cell_macros<Q, R>(fc, rho, ux, uy, uz);
     const R uu = ux * ux + uy * uy + uz * uz;
-    const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)a.rho0);
-    lv[pl][q] = fe + (R)a.scale * (lf[pl][q] - fe);
+    const MemberParams& P = a.mp[a.bmember_c[a.list[i] >> 6]];
+    const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)P.rho0);
+    lv[pl][q] = fe + (R)P.xscale[a.Lc + 1] * (lf[pl][q] - fe);
   }
   __syncthreads();
   const int pair = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -950,16 +955,17 @@ __global__ __launch_bounds__(64) void k_coalesce(XferArgs a) {
   const int64_t i = base + lane;
   if (lane >= P || i >= a.n) return;
   const int p = a.list[i];
+  const MemberParams& M = a.mp[a.bmember_c[p >> 6]];
   R f[Q];
 #pragma unroll
   for (int r = 0; r < Q; ++r) f[r] = ls[lane][r];
   R rho, ux, uy, uz;
   cell_macros<Q, R>(f, rho, ux, uy, uz);
   const R uu = ux * ux + uy * uy + uz * uz;
-  const R sc = (R)a.scale;
+  const R sc = (R)M.cscale[a.Lc];
 #pragma unroll
   for (int r = 0; r < Q; ++r) {
-    const R fe = feq_hl<Q, R>(r, rho, ux, uy, uz, uu, (R)a.rho0);
+    const R fe = feq_hl<Q, R>(r, rho, ux, uy, uz, uu, (R)M.rho0);
     f[r] = fe + sc * (f[r] - fe);
     store_pop<Q, S, R>((S*)a.dst, a.ns_c, r, p, f[r]);
   }
@@ -971,7 +977,7 @@ __global__ __launch_bounds__(64) void k_coalesce(XferArgs a) {
     st_real<R>(a.u_out, a.ns_c + p, uy);
     st_real<R>(a.u_out, 2 * a.ns_c + p, uz);
   }
-  collide_cell<Q, R>(f, rho, ux, uy, uz, (R)a.tau, (R)a.lesC, a.model);
+  collide_cell<Q, R>(f, rho, ux, uy, uz, (R)M.tau[a.Lc], (R)M.les_C, a.model);
 #pragma unroll
   for (int r = 0; r < Q; ++r) store_pop<Q, S, R>((S*)a.fcol_out, a.ns_c, r, p, f[r]);
 }
@@ -980,13 +986,14 @@ __global__ __launch_bounds__(64) void k_coalesce(XferArgs a) {
 // allocMacrosAndDDF (LBM_Manager.cpp:482-592): f = fcol = feq(rho0,u0); solid
 // cells zero with rho = 1, u = 0
 template <int Q, typename S>
-__global__ void k_init(LvArgs v, void* fcol2, void* rho2, void* u2, double rho0, double u0x, double u0y,
-                       double u0z) {
+__global__ void k_init(LvArgs v, void* fcol2, void* rho2, void* u2) {
   typedef typename RealOf<S>::type R;
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= v.ns) return;
   const uint16_t k = v.kind[s];
   const bool sol = k & MT_KIND_SOLID;
+  const MemberParams& P = member_of(v, s);
+  const double rho0 = P.rho0, u0x = P.u0[0], u0y = P.u0[1], u0z = P.u0[2];
   const R uu = (R)(u0x * u0x + u0y * u0y + u0z * u0z);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -1074,6 +1081,9 @@ static LvArgs lvargs(Case& c, int L, int fbuf_in, int fbuf_out, int rbuf_in, int
   DevLevel& l = c.lv[L];
   LvArgs v;
   v.ns = l.n_slots;
+  v.L = L;
+  v.mp = c.d_mem;
+  v.bmember = l.bmember;
   v.bnbr = l.bnbr;
   v.kind = l.kind;
   v.pmask = l.pmask;
@@ -1085,16 +1095,6 @@ static LvArgs lvargs(Case& c, int L, int fbuf_in, int fbuf_out, int rbuf_in, int
   v.rho_out = l.rho[rbuf_out];
   v.u_out = l.u[rbuf_out];
   return v;
-}
-static BcCfg bccfg(const Case& c) {
-  BcCfg b;
-  b.wet_mask = 0;
-  for (int f = 0; f < 6; ++f) {
-    b.kind[f] = c.d.face_kind[f];
-    for (int a = 0; a < 3; ++a) b.u[f][a] = c.d.face_u[f][a];
-    if (b.kind[f] == MT_BC_ZOUHE || b.kind[f] == MT_BC_OUTFLOW) b.wet_mask |= 1 << f;
-  }
-  return b;
 }
 
 // dispatch on (Q, storage)
@@ -1115,8 +1115,7 @@ int init_state(Case& c) {
   for (int L = 0; L <= c.Lmax; ++L) {
     DevLevel& l = c.lv[L];
     LvArgs v = lvargs(c, L, 0, 0, 0, 0);
-    MT_DISPATCH(c, (k_init<Q_, S_><<<nblk(l.n_slots), 256, 0, c.stream>>>(v, l.fcol[1], l.rho[1], l.u[1], c.d.rho0,
-                                                                          c.d.u0[0], c.d.u0[1], c.d.u0[2])));
+    MT_DISPATCH(c, (k_init<Q_, S_><<<nblk(l.n_slots), 256, 0, c.stream>>>(v, l.fcol[1], l.rho[1], l.u[1])));
     l.pf = 0;
     l.pr = 0;
   }
@@ -1128,8 +1127,6 @@ static SeamArgs seam(Case& c, int L) {
   DevLevel& l = c.lv[L];
   SeamArgs a;
   a.v = lvargs(c, L, l.pf, l.pf, l.pr, l.pr);
-  a.tau = c.tau[L];
-  a.lesC = c.d.les_C;
   a.model = c.d.collision;
   return a;
 }
@@ -1156,23 +1153,19 @@ static int launch_nee(Case& c) {
   DevLevel& l = c.lv[0];
   if (!l.n_bcres) return MT_OK;
   SeamArgs a = seam(c, 0);
-  BcCfg bc = bccfg(c);
-  MT_DISPATCH(c, (k_nee<Q_, S_><<<nblk(l.n_bcres), 256, 0, c.stream>>>(a, bc, l.bcres, l.n_bcres, c.nee_tmp)));
+  MT_DISPATCH(c, (k_nee<Q_, S_><<<nblk(l.n_bcres), 256, 0, c.stream>>>(a, l.bcres, l.n_bcres, c.nee_tmp)));
   MT_DISPATCH(c, (k_nee_scatter<Q_, S_><<<nblk(l.n_bcres), 256, 0, c.stream>>>(a, l.bcres, l.n_bcres, c.nee_tmp)));
   MT_HIP(hipGetLastError());
   return MT_OK;
 }
 int launch_bc(Case& c) {
-  BcCfg bc = bccfg(c);
-  bool res = false;
-  for (int f = 0; f < 6; ++f) res |= c.d.face_kind[f] == MT_BC_RESERVOIR;
-  if (res) {
+  if (c.res_any) {
     int r = launch_nee(c);
     if (r) return r;
   }
-  if (bc.wet_mask) {
+  if (c.wet_any) {
     SeamArgs a = seam(c, 0);
-    MT_DISPATCH(c, (k_bc_wet<Q_, S_><<<nblk(a.v.ns), 256, 0, c.stream>>>(a, bc)));
+    MT_DISPATCH(c, (k_bc_wet<Q_, S_><<<nblk(a.v.ns), 256, 0, c.stream>>>(a)));
   }
   MT_HIP(hipGetLastError());
   return MT_OK;
@@ -1192,10 +1185,6 @@ int launch_ib(Case& c, bool acc) {
   return MT_OK;
 }
 
-double explode_scale(const Case& c, int L) {
-  return (c.d.options & MT_OPT_CONVECTIVE_XFER) ? c.tau[L] / (2.0 * c.tau[L - 1]) : (2.0 * c.tau[L]) / c.tau[L - 1];
-}
-
 int launch_explode(Case& c, int L, bool mirror_all) {
   if (L <= 0) return MT_OK;
   DevLevel& lc = c.lv[L - 1];
@@ -1207,8 +1196,9 @@ int launch_explode(Case& c, int L, bool mirror_all) {
   a.src = lc.fcol[lc.pf];
   a.dst = lf.fcol[lf.pf];
   a.dst2 = lf.fcol[1 - lf.pf];
-  a.scale = explode_scale(c, L);
-  a.rho0 = c.d.rho0;
+  a.mp = c.d_mem;
+  a.bmember_c = lc.bmember;
+  a.Lc = L - 1;
   a.collide = 0;
   a.list = lc.xp_list;
   a.n = lc.n_xp;
@@ -1233,8 +1223,9 @@ int launch_explode_to(Case& c, int L, int dst_buf) {
   a.src = lc.fcol[lc.pf];
   a.dst = lf.fcol[dst_buf];
   a.dst2 = nullptr;
-  a.scale = explode_scale(c, L);
-  a.rho0 = c.d.rho0;
+  a.mp = c.d_mem;
+  a.bmember_c = lc.bmember;
+  a.Lc = L - 1;
   a.collide = 0;
   a.list = lc.xp_list;
   a.n = lc.n_xp;
@@ -1277,16 +1268,14 @@ int launch_coalesce(Case& c, int L, bool collide_after, bool write_macros) {
   a.list = lc.f2c;
   a.n = lc.n_f2c;
   a.child = lc.f2c_child;
-  a.scale = (c.d.options & MT_OPT_CONVECTIVE_XFER) ? (2.0 * c.tau[L]) / c.tau[L + 1]
-                                                   : c.tau[L] / (2.0 * c.tau[L + 1]);
-  a.rho0 = c.d.rho0;
+  a.mp = c.d_mem;
+  a.bmember_c = lc.bmember;
+  a.Lc = L;
   a.collide = collide_after ? 1 : 0;
   a.write_macros = write_macros ? 1 : 0;
   a.fcol_out = lc.fcol[1 - lc.pf];
   a.rho_out = lc.rho[1 - lc.pr];
   a.u_out = lc.u[1 - lc.pr];
-  a.tau = c.tau[L];
-  a.lesC = c.d.les_C;
   a.model = c.d.collision;
   MT_DISPATCH(c, (k_coalesce<Q_, S_><<<(a.n + 15) / 16, 64, 0, c.st[L]>>>(a)));
   MT_HIP(hipGetLastError());
@@ -1317,12 +1306,10 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros
   a.v = lvargs(c, L, l.pf, 1 - l.pf, l.pr, 1 - l.pr);
   a.active = l.active;
   a.n_active = l.n_active;
-  a.tau = c.tau[L];
-  a.lesC = c.d.les_C;
+  a.xmap = c.xmap;
   a.model = c.d.collision;
   a.write_f = write_f ? 1 : 0;
-  a.bc = bccfg(c);
-  a.do_bc = (L == 0 && a.bc.wet_mask) ? 1 : 0;
+  a.do_bc = (L == 0 && c.wet_any) ? 1 : 0;
   a.do_ib = level_has_ib(c, L) ? 1 : 0;
   a.ib_method = c.d.ib;
   a.dis_row = l.dis_row;
@@ -1340,8 +1327,6 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros
   a.cpar = nullptr;
   a.ns_c = 0;
   a.cf = a.cfcol_out = a.crho_out = a.cu_out = nullptr;
-  a.ctau = a.cscale = 0;
-  a.rho0 = c.d.rho0;
   a.c_write_macros = 0;
   a.tmask = nullptr;
   a.xbuf = nullptr;
@@ -1349,9 +1334,7 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros
   a.gx_child = nullptr;
   a.gx_dst = nullptr;
   a.ns_f = 0;
-  a.gx_scale = 0;
   a.cx_dst = nullptr;
-  a.cx_scale = 0;
   if (c.xin) {
     if (L >= 1) {
       a.tmask = l.tmask;
@@ -1363,11 +1346,9 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros
       a.gx_child = l.gf2c_child;
       a.gx_dst = lf.fcol[1 - l.pf];
       a.ns_f = lf.n_slots;
-      a.gx_scale = explode_scale(c, L + 1);
     }
     if (coal) {
       a.cx_dst = l.fcol[1 - c.lv[L - 1].pf];
-      a.cx_scale = explode_scale(c, L);
     }
   }
   if (coal) {
@@ -1382,9 +1363,6 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros
     a.cfcol_out = lc.fcol[1 - lc.pf];
     a.crho_out = lc.rho[1 - lc.pr];
     a.cu_out = lc.u[1 - lc.pr];
-    a.ctau = c.tau[L - 1];
-    a.cscale = (c.d.options & MT_OPT_CONVECTIVE_XFER) ? (2.0 * c.tau[L - 1]) / c.tau[L]
-                                                      : c.tau[L - 1] / (2.0 * c.tau[L]);
     a.c_write_macros = coal->write_macros;
   }
   // grid: a multiple of 8 blocks so the XCD remap covers every brick index

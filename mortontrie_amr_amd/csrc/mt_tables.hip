@@ -247,24 +247,56 @@ __global__ void k_place(const uint64_t* ks, const int32_t* vs, const uint16_t* k
   kind[s] = kinds_in[vs[i]];
   if (head[i]) bcode[b] = code >> 6;
 }
-__global__ void k_bnbr(const uint64_t* bcode, int64_t nb, int bx_n, int by_n, int bz_n, int px, int py, int pz,
+// brick -> the member whose level-L box holds it (boxes are brick-aligned)
+__global__ void k_bmember(const uint64_t* bcode, int64_t nb, const MemberParams* mp, int nm, int L, int32_t* bm,
+                          int* err) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  uint64_t c = bcode[b];
+  const int64_t x = 4 * (int64_t)compact3(c), y = 4 * (int64_t)compact3(c >> 1), z = 4 * (int64_t)compact3(c >> 2);
+  int found = -1;
+  for (int m = 0; m < nm && found < 0; ++m) {
+    const MemberParams& P = mp[m];
+    if (x >= P.lo[L][0] && x < P.hi[L][0] && y >= P.lo[L][1] && y < P.hi[L][1] && z >= P.lo[L][2] &&
+        z < P.hi[L][2])
+      found = m;
+  }
+  if (found < 0) {
+    atomicOr(err, 8);
+    found = 0;
+  }
+  bm[b] = found;
+}
+// 27 neighbour bricks of each brick within its member's box; a periodic axis of
+// the member wraps inside the box, any other step out of the box is no brick
+__global__ void k_bnbr(const uint64_t* bcode, int64_t nb, const int32_t* bm, const MemberParams* mp, int L,
                        int32_t* bnbr) {
   int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   uint64_t c = bcode[b];
-  int64_t x = (int64_t)compact3(c), y = (int64_t)compact3(c >> 1), z = (int64_t)compact3(c >> 2);
+  const int64_t p[3] = {(int64_t)compact3(c), (int64_t)compact3(c >> 1), (int64_t)compact3(c >> 2)};
+  const MemberParams& P = mp[bm[b]];
+  int64_t lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = P.lo[L][a] >> 2;
+    hi[a] = (P.hi[L][a] + 3) >> 2;
+  }
   for (int o = 0; o < 27; ++o) {
-    int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
-    int64_t X = x + dx, Y = y + dy, Z = z + dz;
-    if (px) X = (X + bx_n) % bx_n;
-    if (py) Y = (Y + by_n) % by_n;
-    if (pz) Z = (Z + bz_n) % bz_n;
-    int32_t r = -1;
-    if (X >= 0 && Y >= 0 && Z >= 0) {
-      int64_t f = bsearch_u64(bcode, nb, morton3(X, Y, Z));
-      r = (int32_t)f;
+    const int d[3] = {o % 3 - 1, (o / 3) % 3 - 1, o / 9 - 1};
+    int64_t V[3];
+    bool ok = true;
+    for (int a = 0; a < 3; ++a) {
+      V[a] = p[a] + d[a];
+      if (V[a] < lo[a] || V[a] >= hi[a]) {
+        if (P.periodic[a]) {
+          const int64_t n = hi[a] - lo[a];
+          V[a] = lo[a] + ((V[a] - lo[a]) % n + n) % n;
+        } else {
+          ok = false;
+        }
+      }
     }
-    bnbr[b * 27 + o] = r;
+    bnbr[b * 27 + o] = ok ? (int32_t)bsearch_u64(bcode, nb, morton3(V[0], V[1], V[2])) : -1;
   }
 }
 __global__ void k_pmask(const int32_t* bnbr, const uint16_t* kind, int64_t ns, int Q, uint32_t* pmask) {
@@ -435,6 +467,48 @@ __global__ void k_fill_i32(int32_t* a, int64_t n, int32_t v) {
 
 static inline int64_t nblk(int64_t n, int b = 256) { return n <= 0 ? 1 : (n + b - 1) / b; }
 
+// Per-member parameters on the device: tau_L = 2^L (tau0 - 1/2) + 1/2
+// (LBM_Manager.cpp:814-818), the transfer factors 2 tau_f / tau_c and
+// tau_c / (2 tau_f) (LBM_Manager.cpp:1065, 1137; swapped for the convective
+// option), the wet-node face configuration and the member's box per level.
+int build_members(Case& c) {
+  const bool conv = (c.d.options & MT_OPT_CONVECTIVE_XFER) != 0;
+  c.h_mem.clear();
+  c.wet_any = c.res_any = false;
+  for (const mt_member_desc& m : c.h_mdesc) {
+    MemberParams P = {};
+    for (int L = 0; L <= c.Lmax; ++L) P.tau[L] = double(1 << L) * (m.tau0 - 0.5) + 0.5;
+    for (int L = 1; L <= c.Lmax; ++L) P.xscale[L] = conv ? P.tau[L] / (2.0 * P.tau[L - 1]) : (2.0 * P.tau[L]) / P.tau[L - 1];
+    for (int L = 0; L < c.Lmax; ++L) P.cscale[L] = conv ? (2.0 * P.tau[L]) / P.tau[L + 1] : P.tau[L] / (2.0 * P.tau[L + 1]);
+    P.les_C = m.les_C;
+    P.rho0 = m.rho0;
+    for (int a = 0; a < 3; ++a) P.u0[a] = m.u0[a];
+    P.bc.wet_mask = 0;
+    for (int f = 0; f < 6; ++f) {
+      P.bc.kind[f] = m.face_kind[f];
+      for (int a = 0; a < 3; ++a) P.bc.u[f][a] = m.face_u[f][a];
+      if (m.face_kind[f] == MT_BC_ZOUHE || m.face_kind[f] == MT_BC_OUTFLOW) P.bc.wet_mask |= 1 << f;
+      if (m.face_kind[f] == MT_BC_RESERVOIR) c.res_any = true;
+    }
+    if (P.bc.wet_mask) c.wet_any = true;
+    for (int L = 0; L <= c.Lmax; ++L)
+      for (int a = 0; a < 3; ++a) {
+        P.lo[L][a] = m.origin0[a] << L;
+        P.hi[L][a] = (m.origin0[a] + m.dims0[a]) << L;
+      }
+    for (int a = 0; a < 3; ++a) P.periodic[a] = m.face_kind[2 * a] == MT_BC_PERIODIC;
+    c.h_mem.push_back(P);
+  }
+  c.d_mem = c.alloc<MemberParams>((int64_t)c.h_mem.size());
+  if (!c.d_mem) {
+    set_error("out of device memory");
+    return MT_ERR_HIP;
+  }
+  MT_HIP(hipMemcpyAsync(c.d_mem, c.h_mem.data(), sizeof(MemberParams) * c.h_mem.size(), hipMemcpyHostToDevice,
+                        c.stream));
+  return MT_OK;
+}
+
 // The fused kernel gives XCD x the active-list range [x*per, (x+1)*per), per =
 // ceil(n_active/8), and dispatches each range in list order.  Bricks whose
 // update has a longer dependent tail go first within their range, so they do
@@ -542,7 +616,7 @@ int build_tables(Case& c) {
     int herr = 0;
     MT_HIP(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, c.stream));
     MT_HIP(hipStreamSynchronize(c.stream));
-    if (herr) {
+    if (herr & 3) {
       set_error("level " + std::to_string(L) + ": " +
                 ((herr & 2) ? "duplicate key in the uploaded set" : "key is not a cell of this level"));
       return MT_ERR_ARG;
@@ -563,15 +637,13 @@ int build_tables(Case& c) {
     k_fill_empty<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.key, l.kind, l.n_slots);
     k_fill_i32<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.dis_row, l.n_slots, -1);
     k_place<<<nblk(n), 256, 0, c.stream>>>(ks, vs, d_kinds, head, bidx, n, shift, l.bcode, l.key, l.kind);
-    int per[3] = {c.d.face_kind[0] == MT_BC_PERIODIC, c.d.face_kind[2] == MT_BC_PERIODIC,
-                  c.d.face_kind[4] == MT_BC_PERIODIC};
-    for (int a = 0; a < 3; ++a)
-      if (per[a] && (l.dims[a] % 4)) {
-        set_error("periodic axis extent must be a multiple of 4 on every level");
-        return MT_ERR_ARG;
-      }
-    k_bnbr<<<nblk(nb), 256, 0, c.stream>>>(l.bcode, nb, (l.dims[0] + 3) / 4, (l.dims[1] + 3) / 4,
-                                           (l.dims[2] + 3) / 4, per[0], per[1], per[2], l.bnbr);
+    l.bmember = c.alloc<int32_t>(nb);
+    if (!l.bmember) {
+      set_error("hipMalloc failed for level tables");
+      return MT_ERR_HIP;
+    }
+    k_bmember<<<nblk(nb), 256, 0, c.stream>>>(l.bcode, nb, c.d_mem, (int)c.h_mem.size(), L, l.bmember, d_err);
+    k_bnbr<<<nblk(nb), 256, 0, c.stream>>>(l.bcode, nb, l.bmember, c.d_mem, L, l.bnbr);
     k_pmask<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.bnbr, l.kind, l.n_slots, c.Q, l.pmask);
     MT_HIP(hipGetLastError());
     MT_HIP(hipStreamSynchronize(c.stream));
@@ -750,6 +822,10 @@ int build_tables(Case& c) {
   if (herr & 4) {
     set_error("an F2C cell has a child missing on the finer level");
     return MT_ERR_LOOKUP;
+  }
+  if (herr & 8) {
+    set_error("a cell lies outside every member's box");
+    return MT_ERR_ARG;
   }
   return MT_OK;
 }

@@ -152,3 +152,32 @@ def test_cpp_driver_header_compiles():
         pytest.skip("no g++")
     src = os.path.join(ROOT, "tests", "cpp", "driver_check.cpp")
     subprocess.check_call([gxx, "-std=c++17", "-fsyntax-only", "-Wall", "-I", os.path.join(ROOT, "include"), src])
+
+
+# ---------------------------------------------------------------- packed cases
+def test_pack_layout_and_key_translation():
+    """pack(): members keep their cells, boxes are 4-aligned and disjoint, and
+    member keys translate into the packed coordinates level by level."""
+    ms = [C.cavity(12), C.cavity(16, refine_box=((5, 5, 5), (11, 11, 11))), C.cavity(8, tau=0.8, lid=0.08)]
+    p = C.pack(ms)
+    assert p["Lmax"] == 1 and len(p["members"]) == 3
+    for L in range(p["Lmax"] + 1):
+        n = sum(m["levels"][L]["keys"].size for m in ms if L <= m["Lmax"])
+        assert p["levels"][L]["keys"].size == n
+        assert np.unique(p["levels"][L]["keys"]).size == n
+    xs = []
+    for i, (m, md) in enumerate(zip(ms, p["members"])):
+        assert md["origin0"][0] % 4 == 0
+        xs.append((md["origin0"][0], md["origin0"][0] + md["dims0"][0]))
+        for L in range(m["Lmax"] + 1):
+            k = C.member_keys(p, i, L, m["levels"][L]["keys"])
+            assert np.isin(k, p["levels"][L]["keys"]).all()
+            x, y, z = C.decode_key(k, L, p["Lmax"])
+            assert ((x >> L) >= md["origin0"][0]).all() and ((x >> L) < md["origin0"][0] + md["dims0"][0]).all()
+    xs.sort()
+    assert all((a1 + 3) // 4 * 4 <= b0 for (a0, a1), (b0, b1) in zip(xs, xs[1:]))
+
+
+def test_pack_rejects_mixed_models():
+    with pytest.raises(ValueError):
+        C.pack([C.cavity(8), C.cavity(8, Q=27)])
