@@ -194,6 +194,43 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def run_serve(args):
+    """Continuous batching: Poisson arrivals (20 req/s, seed 1) of two-level
+    D3Q27 cumulant sphere cases, served FP32-shifted in one packed case repacked
+    as requests come and go, against the plain FP64 path (each request alone)."""
+    from mortontrie_amr_amd import amr, serve
+    reqs = serve.poisson_requests(args.requests, rate=20.0, seed=1)
+    srv = serve.Server(precision=amr.FP32_SHIFTED, chunk=100)
+    t0 = time.perf_counter()
+    done = srv.run(reqs)
+    served_s = time.perf_counter() - t0
+    lat = np.array([r.done - r.arrival for r in done])
+    t0 = time.perf_counter()
+    plain = {r.id: serve.run_alone(r, precision=amr.FP64) for r in reqs}
+    plain_s = time.perf_counter() - t0
+    errs = []
+    for r in done:
+        p = plain[r.id]
+        k = min(len(r.cd), len(p.cd)) - 1
+        errs.append(abs(r.cd[k] - p.cd[k]) / abs(p.cd[k]) if np.isfinite(p.cd[k]) and p.cd[k] else float("nan"))
+    errs = np.array(errs)
+    out = {
+        "metric": "served cases/s + p50/p99 latency", "value": round(len(done) / served_s, 3), "unit": "cases/s",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "serve_sphere2level_d3q27_cumulant", "requests": len(reqs), "rate_per_s": 20.0,
+                   "chunk_steps": 100, "max_steps": 2000, "drag_tol": 1e-4, "parallelism": "single_gpu"},
+        "latency_p50_s": round(float(np.percentile(lat, 50)), 3),
+        "latency_p99_s": round(float(np.percentile(lat, 99)), 3),
+        "plain_fp64_cases_per_s": round(len(reqs) / plain_s, 3),
+        "speedup_vs_plain_fp64": round((len(done) / served_s) / (len(reqs) / plain_s), 3),
+        "drag_rel_error_max": float(np.nanmax(errs)) if np.isfinite(errs).any() else None,
+        "drag_rel_error_median": float(np.nanmedian(errs)) if np.isfinite(errs).any() else None,
+        "non_finite_drag": int((~np.isfinite(errs)).sum()),
+        "repacks": srv.repacks, "steps_per_request": [r.steps for r in sorted(done, key=lambda r: r.id)],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(case, steps):
     from oracle import pyoracle as O
     o = O.OracleCase(case)
@@ -214,9 +251,10 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--counters", action="store_true",
                     help="short eager run plus a calibration copy, for rocprofv3 --pmc passes")
-    ap.add_argument("--workload", choices=("sphere", "batch"), default="sphere",
-                    help="sphere: the 3-level sphere (default); batch: packed cavities")
+    ap.add_argument("--workload", choices=("sphere", "batch", "serve"), default="sphere",
+                    help="sphere: the 3-level sphere (default); batch: packed cavities; serve: request stream")
     ap.add_argument("--batch", type=int, default=256, help="members of the packed batch")
+    ap.add_argument("--requests", type=int, default=32, help="requests of the serving run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the update runs on a "
@@ -227,6 +265,9 @@ def main():
         if args.steps == 4000:
             args.steps = 2000
         run_batch(args)
+        return
+    if args.workload == "serve":
+        run_serve(args)
         return
     from mortontrie_amr_amd import cases as C, amr
     case = C.sphere_amr(options=2)

@@ -182,6 +182,12 @@ int mt_reduce(mt_case* c, int level, double* mass, double* momentum3, int64_t* n
 int mt_ib_force(mt_case* c, double* sum3, int64_t* count, int reset);
 /* the same per member: sum3[m][3] over member m's cells, fixed order          */
 int mt_ib_force_members(mt_case* c, double* sum3, int32_t n_members, int64_t* count, int reset);
+/* Continuous batching (no counterpart in the reference): copy a member's
+ * populations and macroscopic maps from one case to a member of another case
+ * with the same cells (its box may sit elsewhere, by whole bricks); both cases
+ * are settled first.  A server repacks its running members this way when
+ * requests arrive or finish.                                                  */
+int mt_case_copy_member(mt_case* dst, int32_t dst_member, mt_case* src, int32_t src_member);
 
 #ifdef __cplusplus
 }

@@ -50,7 +50,7 @@ EXPORTS = [
     "mt_case_finalize", "mt_case_level_info", "mt_find", "mt_get_field", "mt_set_field", "mt_macros",
     "mt_collide", "mt_stream", "mt_explode", "mt_coalesce", "mt_apply_bc", "mt_ib_treat", "mt_step",
     "mt_set_graphs", "mt_set_timing", "mt_get_timing", "mt_sync", "mt_reduce", "mt_ib_force",
-    "mt_case_set_members", "mt_ib_force_members", "mt_autotune",
+    "mt_case_set_members", "mt_ib_force_members", "mt_autotune", "mt_case_copy_member",
 ]
 
 _lib = None
@@ -92,6 +92,7 @@ def load():
     L.mt_case_set_members.argtypes = [C.c_void_p, C.POINTER(MemberDesc), C.c_int32]
     L.mt_ib_force_members.argtypes = [C.c_void_p, _dp, C.c_int32, _i64p, C.c_int]
     L.mt_autotune.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.mt_case_copy_member.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]
     _lib = L
     return L
 
@@ -236,6 +237,10 @@ class AmrCase:
 
     def set_graphs(self, on):
         _check(load().mt_set_graphs(self.h, 1 if on else 0))
+
+    def copy_member_from(self, member, src, src_member):
+        """This case's member `member` takes the state of `src`'s member `src_member`."""
+        _check(load().mt_case_copy_member(self.h, member, src.h, src_member))
 
     def autotune(self, steps=5):
         """Pick the fastest launch configuration (advances the run by 4 (steps + 1) coarse steps);

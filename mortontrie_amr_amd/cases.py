@@ -553,3 +553,23 @@ def cavity_batch(n_cases=256, seed=0, Q=19, edges=(16, 24, 32, 48, 64), options=
         c["options"] = int(options)
         out.append(c)
     return out
+
+
+SERVE_DOMAINS = ((48, 24, 24), (64, 32, 32), (96, 48, 48))
+
+
+def sphere_request(rng, U=0.04, shell=3.0, options=2):
+    """One served request: a two-level D3Q27 cumulant sphere in a channel
+    (Zou-He inflow, outflow, periodic y/z), coarse domain drawn from
+    SERVE_DOMAINS, sphere radius ~ U[4, 12] coarse cells and Re ~ U[20, 200]
+    (draws in that order).  The sphere sits at (nx/3, ny/2, nz/2); its radius is
+    capped so that the refined shell keeps 2 coarse cells from every face."""
+    dims = SERVE_DOMAINS[int(rng.integers(0, len(SERVE_DOMAINS)))]
+    r = float(rng.uniform(4.0, 12.0))
+    Re = float(rng.uniform(20.0, 200.0))
+    cx, cy, cz = dims[0] / 3.0, dims[1] / 2.0, dims[2] / 2.0
+    r = min(r, cy - shell - 2.0, cx - shell - 2.0)
+    case = sphere_amr(dims, radius=r, center=(cx, cy, cz), shells=(shell,), U=U, Re=Re, options=options,
+                      name=f"sphere{dims[0]}_r{r:.2f}_Re{Re:.0f}")
+    case["request"] = {"dims": dims, "radius": r, "Re": Re}
+    return case

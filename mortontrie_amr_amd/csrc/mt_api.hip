@@ -672,6 +672,20 @@ int mt_ib_force(mt_case* c, double* sum3, int64_t* count, int reset) {
   return MT_OK;
 }
 
+int mt_case_copy_member(mt_case* dst, int32_t dst_member, mt_case* src, int32_t src_member) {
+  CHECK_FINAL(dst);
+  CHECK_FINAL(src);
+  if (dst_member < 0 || dst_member >= (int32_t)dst->h_mem.size() || src_member < 0 ||
+      src_member >= (int32_t)src->h_mem.size()) {
+    set_error("member index out of range");
+    return MT_ERR_ARG;
+  }
+  TRY(settle(*src));
+  TRY(settle(*dst));
+  MT_HIP(hipStreamSynchronize(src->stream));
+  return copy_member(*dst, dst_member, *src, src_member);
+}
+
 int mt_ib_force_members(mt_case* c, double* sum3, int32_t n_members, int64_t* count, int reset) {
   CHECK_FINAL(c);
   const int32_t nm = (int32_t)c->h_mem.size();

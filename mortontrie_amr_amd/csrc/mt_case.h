@@ -175,6 +175,7 @@ void set_error(const std::string& s);
 
 // tables (mt_tables.hip)
 int build_members(Case& c);
+int copy_member(Case& dst, int32_t dm, Case& src, int32_t sm);
 int build_tables(Case& c);
 int find_slots(Case& c, int level, const uint64_t* d_keys, int64_t n, int32_t* d_out);
 // kernels (mt_kernels.hip)

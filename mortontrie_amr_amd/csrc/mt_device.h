@@ -92,6 +92,19 @@ __device__ __forceinline__ R load_pop(const S* base, int64_t ns, int q, int slot
   if (Store<S>::shifted) v += (R)wq<Q>(q);
   return v;
 }
+// The value a population takes once stored and reloaded (identity for FP64).
+// Values computed in registers and used again before any store (boundary
+// reconstruction, coalescence, a collision feeding an explosion) are snapped so
+// that the fused schedule computes exactly what the per-method kernels compute
+// from the stored state.
+template <int Q, typename S, typename R>
+__device__ __forceinline__ R snap_pop(int q, R v) {
+  if (!Store<S>::shifted) return v;
+  const R w = (R)wq<Q>(q);
+  S t;
+  Store<S>::st(&t, v - w);
+  return (R)Store<S>::ld(&t) + w;
+}
 template <int Q, typename S, typename R>
 __device__ __forceinline__ void store_pop(S* base, int64_t ns, int q, int slot, R v) {
   if (Store<S>::shifted) v -= (R)wq<Q>(q);

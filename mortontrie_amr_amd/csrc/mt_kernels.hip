@@ -518,7 +518,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         if (Store<S>::shifted) v += (R)wq<Q>(q);
         f[q] = v;
       });
-      if (treated) ib_links_cell<Q, S, R>(f, a, N, lx, ly, lz, s, row, pmx, fx, fy, fz);
+      if (treated) {
+        ib_links_cell<Q, S, R>(f, a, N, lx, ly, lz, s, row, pmx, fx, fy, fz);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) f[q] = snap_pop<Q, S, R>(q, f[q]);
+      }
       if (a.do_bc && (k & 0x3F00)) {
         const int32_t* bn = a.v.bnbr;
         const uint32_t* pmk = a.v.pmask;
@@ -533,6 +537,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
           return load_pop<Q, S, R>(fst, ns, q, t);
         };
         wet_bc<Q, R>(f, k, P.bc, nf);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) f[q] = snap_pop<Q, S, R>(q, f[q]);
       }
       // A C2F cell's stored f is read by a separate coalescence kernel and, for
       // the directions with no source, by its own next pull; those entries never
@@ -560,9 +566,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
           // a ghost_F2C parent explodes its next collision into the finer
           // level's ghost_C2F targets of the next pair
           const int gr = a.gxrow[s];
-          if (gr >= 0)
+          if (gr >= 0) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) f[q] = snap_pop<Q, S, R>(q, f[q]);
             explode_lane<Q, S, R>(f, a.gx_child + (int64_t)gr * 8, (S*)a.gx_dst, a.ns_f, (R)P.xscale[L + 1],
                                   (R)P.rho0);
+          }
         }
       } else if (a.c2f_fcol || (k & MT_KIND_DIS)) {
         // (a C2F cell in dis is never exploded into: it keeps the stream-copy fcol)
@@ -612,7 +621,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)P.rho0);
-      avg[q] = fe + sc * (avg[q] - fe);
+      avg[q] = snap_pop<Q, S, R>(q, fe + sc * (avg[q] - fe));
       store_pop<Q, S, R>((S*)a.cf, a.ns_c, q, par, avg[q]);
     }
     cell_macros<Q, R>(avg, rho, ux, uy, uz);
@@ -627,6 +636,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)a.cfcol_out, a.ns_c, q, par, avg[q]);
     if (a.cx_dst) {
       // the parent's explosion for the next pair, handed to its 8 children
+#pragma unroll
+      for (int q = 0; q < Q; ++q) avg[q] = snap_pop<Q, S, R>(q, avg[q]);
       cell_macros<Q, R>(avg, rho, ux, uy, uz);
       uu = ux * ux + uy * uy + uz * uz;
       const R sx = (R)P.xscale[L];
@@ -966,7 +977,7 @@ __global__ __launch_bounds__(64) void k_coalesce(XferArgs a) {
 #pragma unroll
   for (int r = 0; r < Q; ++r) {
     const R fe = feq_hl<Q, R>(r, rho, ux, uy, uz, uu, (R)M.rho0);
-    f[r] = fe + sc * (f[r] - fe);
+    f[r] = snap_pop<Q, S, R>(r, fe + sc * (f[r] - fe));
     store_pop<Q, S, R>((S*)a.dst, a.ns_c, r, p, f[r]);
   }
   if (!a.collide) return;

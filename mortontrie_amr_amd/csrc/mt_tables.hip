@@ -467,6 +467,113 @@ __global__ void k_fill_i32(int32_t* a, int64_t n, int32_t v) {
 
 static inline int64_t nblk(int64_t n, int b = 256) { return n <= 0 ? 1 : (n + b - 1) / b; }
 
+// Copy one member's state between two cases, brick by brick: every brick of
+// member sm in the source, moved by (dx,dy,dz) bricks, is a brick of the
+// destination; rows[r] = (source row, destination row) of n_slots elements.
+template <typename T>
+__global__ void k_copy_member(const uint64_t* sbcode, const int32_t* sbm, int32_t sm, const uint64_t* dbcode,
+                              int64_t dnb, int64_t dx, int64_t dy, int64_t dz, const T* const* srows,
+                              T* const* drows, int nrows, int* err) {
+  const int64_t b = blockIdx.x;
+  if (sbm[b] != sm) return;  // uniform over the block
+  const uint64_t c = sbcode[b];
+  const uint64_t code = morton3(compact3(c) + dx, compact3(c >> 1) + dy, compact3(c >> 2) + dz);
+  const int64_t db = bsearch_u64(dbcode, dnb, code);
+  if (db < 0) {
+    if (threadIdx.x == 0) atomicOr(err, 1);
+    return;
+  }
+  for (int r = 0; r < nrows; ++r) drows[r][db * 64 + threadIdx.x] = srows[r][b * 64 + threadIdx.x];
+}
+
+template <typename T>
+static int copy_rows(Case& dst, DevLevel& ld, Case& src, DevLevel& ls, int32_t sm, int64_t d[3],
+                     const std::vector<std::pair<void*, void*>>& rows, int* d_err) {
+  std::vector<const T*> hs;
+  std::vector<T*> hd;
+  for (auto& r : rows) {
+    hs.push_back((const T*)r.first);
+    hd.push_back((T*)r.second);
+  }
+  T** dev = nullptr;
+  MT_HIP(hipMalloc((void**)&dev, 2 * sizeof(T*) * rows.size()));
+  MT_HIP(hipMemcpyAsync(dev, hs.data(), sizeof(T*) * rows.size(), hipMemcpyHostToDevice, dst.stream));
+  MT_HIP(hipMemcpyAsync(dev + rows.size(), hd.data(), sizeof(T*) * rows.size(), hipMemcpyHostToDevice, dst.stream));
+  k_copy_member<T><<<ls.n_bricks, 64, 0, dst.stream>>>(ls.bcode, ls.bmember, sm, ld.bcode, ld.n_bricks, d[0], d[1], d[2],
+                                                       (const T* const*)dev, dev + rows.size(), (int)rows.size(),
+                                                       d_err);
+  MT_HIP(hipGetLastError());
+  MT_HIP(hipStreamSynchronize(dst.stream));
+  MT_HIP(hipFree(dev));
+  return MT_OK;
+}
+
+// Both cases settled (reference-state buffers); member geometries equal up to
+// a whole-brick shift.  Copies f, both fcol buffers and both rho / u buffers.
+int copy_member(Case& dst, int32_t dm, Case& src, int32_t sm) {
+  if (dst.Q != src.Q || dst.prec != src.prec) {
+    set_error("members differ in Q or precision");
+    return MT_ERR_ARG;
+  }
+  const MemberParams& PS = src.h_mem[sm];
+  const MemberParams& PD = dst.h_mem[dm];
+  int* d_err = nullptr;
+  MT_HIP(hipMalloc((void**)&d_err, 4));
+  MT_HIP(hipMemsetAsync(d_err, 0, 4, dst.stream));
+  MT_HIP(hipStreamSynchronize(src.stream));
+  int r = MT_OK;
+  for (int L = 0; L <= src.Lmax && r == MT_OK; ++L) {
+    if (L > dst.Lmax) {
+      set_error("destination has fewer levels");
+      r = MT_ERR_ARG;
+      break;
+    }
+    DevLevel& ls = src.lv[L];
+    DevLevel& ld = dst.lv[L];
+    int64_t d[3];
+    for (int a = 0; a < 3; ++a) {
+      const int64_t sh = (int64_t)PD.lo[L][a] - PS.lo[L][a];
+      if (sh % 4) {
+        set_error("member origins differ by a partial brick");
+        r = MT_ERR_ARG;
+      }
+      d[a] = sh / 4;
+    }
+    if (r) break;
+    const size_t pb = src.pop_bytes(), rb = src.real_bytes();
+    std::vector<std::pair<void*, void*>> pops, reals;
+    auto prow = [&](void* base, int64_t ns, int q) { return (void*)((char*)base + pb * (size_t)(q * ns)); };
+    auto rrow = [&](void* base, int64_t ns, int j) { return (void*)((char*)base + rb * (size_t)(j * ns)); };
+    for (int q = 0; q < src.Q; ++q) {
+      pops.push_back({prow(ls.f, ls.n_slots, q), prow(ld.f, ld.n_slots, q)});
+      for (int k = 0; k < 2; ++k)
+        pops.push_back({prow(ls.fcol[k ^ ls.pf], ls.n_slots, q), prow(ld.fcol[k ^ ld.pf], ld.n_slots, q)});
+    }
+    for (int k = 0; k < 2; ++k) {
+      reals.push_back({rrow(ls.rho[k ^ ls.pr], ls.n_slots, 0), rrow(ld.rho[k ^ ld.pr], ld.n_slots, 0)});
+      for (int j = 0; j < 3; ++j)
+        reals.push_back({rrow(ls.u[k ^ ls.pr], ls.n_slots, j), rrow(ld.u[k ^ ld.pr], ld.n_slots, j)});
+    }
+    if (pb == 8) r = copy_rows<uint64_t>(dst, ld, src, ls, sm, d, pops, d_err);
+    else if (pb == 4) r = copy_rows<uint32_t>(dst, ld, src, ls, sm, d, pops, d_err);
+    else r = copy_rows<uint16_t>(dst, ld, src, ls, sm, d, pops, d_err);
+    if (r == MT_OK) {
+      if (rb == 8) r = copy_rows<uint64_t>(dst, ld, src, ls, sm, d, reals, d_err);
+      else r = copy_rows<uint32_t>(dst, ld, src, ls, sm, d, reals, d_err);
+    }
+  }
+  int herr = 0;
+  if (r == MT_OK) {
+    MT_HIP(hipMemcpy(&herr, d_err, 4, hipMemcpyDeviceToHost));
+    if (herr) {
+      set_error("a source brick of the member has no destination brick (geometries differ)");
+      r = MT_ERR_ARG;
+    }
+  }
+  hipFree(d_err);
+  return r;
+}
+
 // Per-member parameters on the device: tau_L = 2^L (tau0 - 1/2) + 1/2
 // (LBM_Manager.cpp:814-818), the transfer factors 2 tau_f / tau_c and
 // tau_c / (2 tau_f) (LBM_Manager.cpp:1065, 1137; swapped for the convective

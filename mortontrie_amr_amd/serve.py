@@ -1,0 +1,155 @@
+"""Continuous-batching server for flow cases.
+
+Requests (case dictionaries with a solid boundary) arrive over time.  The
+server keeps every running request as a member of one packed case, so all of
+them share each level sub-step's launch.  It advances the packed case in
+chunks of `chunk` coarse steps; between chunks it
+  * retires requests that finished (max_steps reached, or the drag averaged
+    over the last chunk moved by less than `tol` from the previous chunk's),
+  * admits arrived requests while their device footprint fits `hbm_budget`,
+  * repacks when membership changed, carrying each running member's state over
+    on the device (mt_case_copy_member), so a member computes exactly what it
+    would compute alone.
+Latency is completion time minus arrival time, on the server's wall clock.
+"""
+from __future__ import annotations
+
+import time
+from collections import deque
+
+import numpy as np
+
+from . import amr, cases as C
+
+
+def footprint_bytes(case, precision):
+    """Device bytes of one case: per slot 3Q populations (f and two fcol), two
+    rho/u buffers and the tables; slots counted as cells rounded up to bricks
+    (x1.3 for partly filled bricks)."""
+    pb = {amr.FP64: 8, amr.FP32_SHIFTED: 4, amr.FP16_STORAGE: 2}[precision]
+    rb = 8 if precision == amr.FP64 else 4
+    cells = sum(lv["keys"].size for lv in case["levels"])
+    return int(cells * 1.3 * (3 * case["Q"] * pb + 8 * rb + 24))
+
+
+class Request:
+    def __init__(self, rid, case, arrival, max_steps=2000, tol=1e-4):
+        self.id, self.case, self.arrival = rid, case, arrival
+        self.max_steps, self.tol = max_steps, tol
+        self.steps = 0
+        self.cd = []
+        self.admitted = None
+        self.done = None
+
+    def cd_of(self, fx_per_call):
+        cd = self.case["cd"]
+        return 2.0 * fx_per_call / (cd["U"] ** 2 * np.pi * cd["d"] ** 2 / 4.0)
+
+    def finished(self):
+        if self.steps >= self.max_steps:
+            return True
+        return len(self.cd) >= 2 and abs(self.cd[-1] - self.cd[-2]) < self.tol
+
+
+class Server:
+    """chunk_seconds: None serves in wall-clock time; a number makes time virtual
+    (each chunk takes that long, idle waits are skipped) for reproducible tests."""
+
+    def __init__(self, precision=amr.FP32_SHIFTED, chunk=100, hbm_budget=200 << 30, chunk_seconds=None):
+        self.precision, self.chunk, self.budget = precision, chunk, hbm_budget
+        self.chunk_seconds = chunk_seconds
+        self.vtime = 0.0
+        self.batch = None      # packed AmrCase of the running requests
+        self.members = []      # running requests, in member order
+        self.repacks = 0
+        self.chunks = 0
+
+    def _now(self, t0):
+        return self.vtime if self.chunk_seconds is not None else time.perf_counter() - t0
+
+    def _wait(self, dt):
+        if self.chunk_seconds is not None:
+            self.vtime += dt
+        else:
+            time.sleep(dt)
+
+    def _repack(self, running):
+        packed = C.pack([r.case for r in running])
+        g = amr.AmrCase(packed, precision=self.precision, graphs=True)
+        if self.batch is not None:
+            old = {id(r): i for i, r in enumerate(self.members)}
+            for j, r in enumerate(running):
+                if id(r) in old:
+                    g.copy_member_from(j, self.batch, old[id(r)])
+            self.batch.close()
+        self.batch, self.members = g, list(running)
+        self.repacks += 1
+
+    def run(self, requests):
+        """Serve `requests` (sorted by arrival, seconds from now); returns them finished."""
+        t0 = time.perf_counter()
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        running, done = [], []
+        changed = False
+        while pending or running:
+            now = self._now(t0)
+            used = sum(footprint_bytes(r.case, self.precision) for r in running)
+            while pending and pending[0].arrival <= now:
+                need = footprint_bytes(pending[0].case, self.precision)
+                if running and used + need > self.budget:
+                    break
+                r = pending.popleft()
+                r.admitted = now
+                running.append(r)
+                used += need
+                changed = True
+            if not running:
+                self._wait(max(0.0, pending[0].arrival - now))
+                continue
+            if changed:
+                self._repack(running)
+                changed = False
+            self.batch.step(self.chunk)
+            sums, n = self.batch.ib_force_members(reset=True)
+            self.chunks += 1
+            if self.chunk_seconds is not None:
+                self.vtime += self.chunk_seconds
+            now = self._now(t0)
+            still = []
+            for i, r in enumerate(self.members):
+                r.steps += self.chunk
+                r.cd.append(r.cd_of(sums[i, 0] / max(n, 1)))
+                if r.finished():
+                    r.done = now
+                    done.append(r)
+                    changed = True
+                else:
+                    still.append(r)
+            running = still
+        if self.batch is not None:
+            self.batch.close()
+            self.batch, self.members = None, []
+        return done
+
+
+def run_alone(req, precision=amr.FP64, chunk=100):
+    """The plain path: one request on its own case, the same chunked rule."""
+    g = amr.AmrCase(req.case, precision=precision, graphs=True)
+    r = Request(req.id, req.case, 0.0, req.max_steps, req.tol)
+    while not r.finished():
+        g.step(chunk)
+        s, n = g.ib_force(reset=True)
+        r.steps += chunk
+        r.cd.append(r.cd_of(s[0] / max(n, 1)))
+    g.close()
+    return r
+
+
+def poisson_requests(n, rate=20.0, seed=1, max_steps=2000, tol=1e-4):
+    """n requests with exponential inter-arrival times (mean 1/rate s)."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    for i in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        out.append(Request(i, C.sphere_request(rng), t, max_steps, tol))
+    return out
