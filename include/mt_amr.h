@@ -49,6 +49,9 @@ extern "C" {
 #define MT_BC_ZOUHE 2     /* wet-node velocity (Zou-He / Hecht-Harting closure)         */
 #define MT_BC_OUTFLOW 3   /* zero-gradient copy of the unknown populations              */
 #define MT_BC_PERIODIC 4  /* wrapped neighbour tables                                   */
+#define MT_BC_NEE 5       /* wet-node velocity wall by non-equilibrium extrapolation from
+                             the inward neighbour (the NonEquilibrumExtrapolation rule,
+                             BC/NonEquilibrumExtrapolation.cpp:50-66, on the wall cells) */
 
 #define MT_COLLIDE_LBGK 0
 #define MT_COLLIDE_CUMULANT 1

@@ -199,7 +199,7 @@ int mt_case_create(const mt_case_desc* d, mt_case** out) {
   if (d->precision < MT_FP64 || d->precision > MT_FP16_STORAGE) { set_error("bad precision"); return MT_ERR_ARG; }
   if (d->tau0 <= 0.5) { set_error("tau0 must exceed 1/2"); return MT_ERR_ARG; }
   for (int f = 0; f < 6; ++f)
-    if (d->face_kind[f] < MT_BC_NONE || d->face_kind[f] > MT_BC_PERIODIC) { set_error("bad face kind"); return MT_ERR_ARG; }
+    if (d->face_kind[f] < MT_BC_NONE || d->face_kind[f] > MT_BC_NEE) { set_error("bad face kind"); return MT_ERR_ARG; }
   for (int a = 0; a < 3; ++a) {
     bool p0 = d->face_kind[2 * a] == MT_BC_PERIODIC, p1 = d->face_kind[2 * a + 1] == MT_BC_PERIODIC;
     if (p0 != p1) { set_error("periodic faces must come in pairs"); return MT_ERR_ARG; }
@@ -294,7 +294,7 @@ int mt_case_upload_dis(mt_case* c, const uint64_t* keys, const double* vals, int
 static int check_member(const mt_case_desc& d, const mt_member_desc& m, std::string& why) {
   if (m.tau0 <= 0.5) { why = "tau0 must exceed 1/2"; return MT_ERR_ARG; }
   for (int f = 0; f < 6; ++f)
-    if (m.face_kind[f] < MT_BC_NONE || m.face_kind[f] > MT_BC_PERIODIC) { why = "bad face kind"; return MT_ERR_ARG; }
+    if (m.face_kind[f] < MT_BC_NONE || m.face_kind[f] > MT_BC_NEE) { why = "bad face kind"; return MT_ERR_ARG; }
   for (int a = 0; a < 3; ++a) {
     bool p0 = m.face_kind[2 * a] == MT_BC_PERIODIC, p1 = m.face_kind[2 * a + 1] == MT_BC_PERIODIC;
     if (p0 != p1) { why = "periodic faces must come in pairs"; return MT_ERR_ARG; }

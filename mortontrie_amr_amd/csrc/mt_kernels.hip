@@ -110,6 +110,21 @@ __device__ __forceinline__ void wet_bc(R* f, uint16_t k, const BcCfg& bc, NF nf)
     const int face = __ffs(act) - 1;
     const int ax = face >> 1;
     const int sg = (face & 1) ? 1 : -1;
+    if (bc.kind[face] == MT_BC_NEE) {
+      // f_q = feq_q(rho_n, u_w) + f_q(x - n) - feq_q(rho_n, u_n), all q
+      const int nx = ax == 0 ? sg : 0, ny = ax == 1 ? sg : 0, nz = ax == 2 ? sg : 0;
+      R fn[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) fn[q] = nf(-nx, -ny, -nz, q);
+      R rn, ux, uy, uz;
+      cell_macros<Q, R>(fn, rn, ux, uy, uz);
+      const R wx = (R)bc.u[face][0], wy = (R)bc.u[face][1], wz = (R)bc.u[face][2];
+      const R uu = ux * ux + uy * uy + uz * uz, ww = wx * wx + wy * wy + wz * wz;
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+        f[q] = feq_c<Q, R>(q, rn, wx, wy, wz, ww) + fn[q] - feq_c<Q, R>(q, rn, ux, uy, uz, uu);
+      return;
+    }
     if (bc.kind[face] == MT_BC_OUTFLOW) {
       const int nx = ax == 0 ? sg : 0, ny = ax == 1 ? sg : 0, nz = ax == 2 ? sg : 0;
 #pragma unroll
@@ -168,7 +183,7 @@ __device__ __forceinline__ void wet_bc(R* f, uint16_t k, const BcCfg& bc, NF nf)
     if (act & (1 << ff)) {
       const int sg = (ff & 1) ? 1 : -1, ax = ff >> 1;
       if (ax == 0) dx -= sg; else if (ax == 1) dy -= sg; else dz -= sg;
-      if (bc.kind[ff] == MT_BC_ZOUHE) uface = ff;
+      if (bc.kind[ff] == MT_BC_ZOUHE || bc.kind[ff] == MT_BC_NEE) uface = ff;
     }
   R fn[Q];
 #pragma unroll

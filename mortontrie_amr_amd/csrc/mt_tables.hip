@@ -594,7 +594,8 @@ int build_members(Case& c) {
     for (int f = 0; f < 6; ++f) {
       P.bc.kind[f] = m.face_kind[f];
       for (int a = 0; a < 3; ++a) P.bc.u[f][a] = m.face_u[f][a];
-      if (m.face_kind[f] == MT_BC_ZOUHE || m.face_kind[f] == MT_BC_OUTFLOW) P.bc.wet_mask |= 1 << f;
+      if (m.face_kind[f] == MT_BC_ZOUHE || m.face_kind[f] == MT_BC_OUTFLOW || m.face_kind[f] == MT_BC_NEE)
+        P.bc.wet_mask |= 1 << f;
       if (m.face_kind[f] == MT_BC_RESERVOIR) c.res_any = true;
     }
     if (P.bc.wet_mask) c.wet_any = true;

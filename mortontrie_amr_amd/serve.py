@@ -55,10 +55,15 @@ class Server:
     """chunk_seconds: None serves in wall-clock time; a number makes time virtual
     (each chunk takes that long, idle waits are skipped) for reproducible tests."""
 
-    def __init__(self, precision=amr.FP32_SHIFTED, chunk=100, hbm_budget=200 << 30, chunk_seconds=None):
+    def __init__(self, precision=amr.FP32_SHIFTED, chunk=100, hbm_budget=200 << 30, chunk_seconds=None,
+                 slot_dims=(100, 48, 48)):
         self.precision, self.chunk, self.budget = precision, chunk, hbm_budget
         self.chunk_seconds = chunk_seconds
         self.vtime = 0.0
+        # every running request keeps one slot (a fixed box along x) for its life,
+        # so its packed keys are encoded once and repacks move no state
+        self.slot_dims = slot_dims
+        self.slot_of = {}
         self.batch = None      # packed AmrCase of the running requests
         self.members = []      # running requests, in member order
         self.repacks = 0
@@ -73,8 +78,27 @@ class Server:
         else:
             time.sleep(dt)
 
+    def _slot(self, r):
+        if id(r) not in self.slot_of:
+            used = set(self.slot_of.values())
+            k = 0
+            while k in used:
+                k += 1
+            self.slot_of[id(r)] = k
+        return self.slot_of[id(r)]
+
     def _repack(self, running):
-        packed = C.pack([r.case for r in running])
+        live = {id(r) for r in running}
+        self.slot_of = {k: v for k, v in self.slot_of.items() if k in live}
+        for r in running:
+            d = r.case["dims0"]
+            if any(d[a] > self.slot_dims[a] - (4 if a == 0 else 0) for a in range(3)):
+                raise ValueError(f"request {r.id} does not fit a serving slot {self.slot_dims}")
+        slots = [self._slot(r) for r in running]
+        sx = self.slot_dims[0]
+        origins = [(k * sx, 0, 0) for k in slots]
+        dims0 = ((max(slots) + 1) * sx, self.slot_dims[1], self.slot_dims[2])
+        packed = C.pack([r.case for r in running], origins=origins, dims0=dims0)
         g = amr.AmrCase(packed, precision=self.precision, graphs=True)
         if self.batch is not None:
             old = {id(r): i for i, r in enumerate(self.members)}

@@ -45,7 +45,7 @@ enum Kind : uint16_t {
   K_FLUID = 0x1, K_F2C = 0x2, K_C2F = 0x4, K_GHOST_C2F = 0x8, K_GHOST_F2C = 0x10,
   K_SOLID = 0x20, K_BCRES = 0x40, K_DIS = 0x80
 };
-enum FaceKind { BC_NONE = 0, BC_RESERVOIR = 1, BC_ZOUHE = 2, BC_OUTFLOW = 3, BC_PERIODIC = 4 };
+enum FaceKind { BC_NONE = 0, BC_RESERVOIR = 1, BC_ZOUHE = 2, BC_OUTFLOW = 3, BC_PERIODIC = 4, BC_NEE = 5 };
 
 // ---------------------------------------------------------------- Morton keys
 // Dilated-integer neighbour arithmetic (header/Morton_Assist.h:229-289,
@@ -595,6 +595,8 @@ struct Case {
   //    N_t = 1/2 sum_{e.n=0} f_j (e_j.t) - rho (u_w.t)/3 over the two tangents t.
   //    For D3Q19 this is Hecht-Harting; it reproduces rho and u_w exactly.
   //  * Outflow (definition): f_q(x_b) = f_q(x_b - n) for the unknown q.
+  //  * Wet-node NEE (definition): f_q = feq_q(rho_n, u_w) + f_q(x_n) - feq_q(rho_n, u_n)
+  //    for all q, x_n = x_b - n.
   //  * Edges/corners (>= 2 wet faces, definition): f_q = feq_q(rho_n, u_w) +
   //    f_q(x_n) - feq_q(rho_n, u_n) for all q, x_n = x_b - sum n, u_w the
   //    velocity of the last Zou-He face in W,E,S,N,B,T order (u_n if none).
@@ -606,7 +608,7 @@ struct Case {
       int fm = faces_of(l.kind.at(k));
       int act = 0, nact = 0;
       for (int ff = 0; ff < 6; ++ff)
-        if ((fm & (1 << ff)) && (face_kind[ff] == BC_ZOUHE || face_kind[ff] == BC_OUTFLOW)) {
+        if ((fm & (1 << ff)) && (face_kind[ff] == BC_ZOUHE || face_kind[ff] == BC_OUTFLOW || face_kind[ff] == BC_NEE)) {
           act |= 1 << ff;
           ++nact;
         }
@@ -618,7 +620,18 @@ struct Case {
         int n[3];
         normal(face, n);
         auto en = [&](int q) { return EX[q] * n[0] + EY[q] * n[1] + EZ[q] * n[2]; };
-        if (face_kind[face] == BC_OUTFLOW) {
+        if (face_kind[face] == BC_NEE) {
+          // wet-node NEE (definition): the NonEquilibrumExtrapolation rule of
+          // BC/NonEquilibrumExtrapolation.cpp:50-66 applied on the wall cell, from
+          // its inward neighbour x - n for every q
+          uint64_t kin = nbr(k, 0, -n[0], -n[1], -n[2]);
+          double fn[27], un[3], rn, fw[27], fq[27];
+          load(0, kin, fn, false);
+          macros(Q, fn, rn, un);
+          feq_c(Q, rn, face_u[face], fw);
+          feq_c(Q, rn, un, fq);
+          for (int q = 0; q < Q; ++q) f[q] = fw[q] + fn[q] - fq[q];
+        } else if (face_kind[face] == BC_OUTFLOW) {
           uint64_t kin = nbr(k, 0, -n[0], -n[1], -n[2]);
           for (int q = 0; q < Q; ++q)
             if (en(q) == 1) f[q] = l.f[q].at(kin);
@@ -661,7 +674,7 @@ struct Case {
             int n[3];
             normal(ff, n);
             for (int a = 0; a < 3; ++a) d[a] -= n[a];
-            if (face_kind[ff] == BC_ZOUHE) uw = face_u[ff];
+            if (face_kind[ff] == BC_ZOUHE || face_kind[ff] == BC_NEE) uw = face_u[ff];
           }
         uint64_t kn = nbr(k, 0, d[0], d[1], d[2]);
         double fn[27], un[3], rn;

@@ -82,6 +82,15 @@ def test_cavity_1000_steps():
     assert np.isclose(m_g, o.f(0).sum(), rtol=1e-13)
 
 
+def test_cavity_nee_walls():
+    """Wet-node NEE walls and lid at tau 0.53, 300 steps (the batch's wall kind)."""
+    case = C.cavity(16, tau=0.53, lid=0.06, wall=C.BC_NEE)
+    g, o = amr.AmrCase(case), O.OracleCase(case)
+    g.step(300)
+    o.step(300)
+    compare(g, o, case)
+
+
 def test_cavity_64_full_size():
     """The 64^3 cavity workload at full size, 30 steps against the oracle."""
     case = C.cavity(64)

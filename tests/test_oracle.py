@@ -269,6 +269,46 @@ def test_zouhe_reconstructs_wall_velocity():
     assert single.sum() == 6 * 36
 
 
+def test_nee_wall_velocity_and_density():
+    """Wet-node NEE wall (definition): after a step every single-face wall cell
+    carries exactly the wall velocity and the density of its inward neighbour."""
+    case = C.cavity(8, lid=0.05, wall=C.BC_NEE)
+    o = O.OracleCase(case)
+    o.step(3)
+    lv = case["levels"][0]
+    E, _ = lattice(19)
+    f = o.f(0)
+    rho = f.sum(0)
+    u = (E.T @ f) / rho
+    x, y, z = C.decode_key(lv["keys"], 0, 0)
+    pos = {(int(a), int(b), int(c)): i for i, (a, b, c) in enumerate(zip(x, y, z))}
+    faces = (lv["kinds"] >> 8) & 63
+    normals = [(-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)]
+    n_checked = 0
+    for bit, face in zip([1, 2, 4, 8, 16, 32], range(6)):
+        want = np.array(case["faces"][face][1:4])
+        for i in np.nonzero(faces == bit)[0]:
+            assert np.abs(u[:, i] - want).max() < 1e-14
+            nb = pos[(int(x[i]) - normals[face][0], int(y[i]) - normals[face][1], int(z[i]) - normals[face][2])]
+            assert abs(rho[i] - rho[nb]) < 1e-14
+            n_checked += 1
+    assert n_checked == 6 * 36
+
+
+def test_nee_walls_stable_where_zouhe_diverges():
+    """At tau = 0.55 and lid 0.04 the Zou-He cavity (extrapolated edges)
+    diverges within a few hundred steps; the NEE-walled one stays bounded by the
+    lid speed."""
+    zh = O.OracleCase(C.cavity(12, tau=0.55, lid=0.04))
+    ne = O.OracleCase(C.cavity(12, tau=0.55, lid=0.04, wall=C.BC_NEE))
+    zh.step(300)
+    ne.step(300)
+    u_ne = ne.u(0)
+    assert np.isfinite(u_ne).all() and np.abs(u_ne).max() <= 0.04 + 1e-12
+    u_zh = zh.u(0)
+    assert not np.isfinite(u_zh).all() or np.abs(u_zh).max() > 0.2
+
+
 def test_outflow_copies_unknowns():
     """Outflow (definition; BC/Outflow.cpp:39-52 text): the e.n = +1 populations of an
     x-max cell equal those of its x-1 neighbour after the step."""
