@@ -130,6 +130,16 @@ int mt_case_finalize(mt_case* c);
  * Between mt_case_create and mt_case_finalize; keys of every member are
  * uploaded together, in the case's coordinates.                              */
 int mt_case_set_members(mt_case* c, const mt_member_desc* members, int32_t n);
+/* Lagrangian markers on the finest level (no counterpart in the reference, whose
+ * marker interface is a sketch): n markers, each with 27 finest-level cell keys
+ * and kernel weights (weight 0 = unused entry), a surface element dV and the body
+ * velocity ub[3].  Every finest sub-step of mt_step applies explicit direct
+ * forcing to the post-collision populations: u_k and rho_k are the weighted sums
+ * of the cells' moments, F_k = dV_k rho_k (ub - u_k), each cell's populations
+ * shift by feq(rho, u + sum_k w F_k / rho) - feq(rho, u); -sum_k F_k adds to the
+ * force mt_ib_force reports.  Before mt_case_finalize; single-member cases.    */
+int mt_case_upload_markers(mt_case* c, int64_t n, const uint64_t* keys27, const double* weights27,
+                           const double* dV, const double* ub3);
 int mt_case_level_info(mt_case* c, int level, int64_t* n_cells, int64_t* n_bricks);
 
 /* ---- key lookup: replaces D_map_define<T>::find (mortonTrieMap.cpp:116-126,

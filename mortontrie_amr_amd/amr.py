@@ -51,6 +51,7 @@ EXPORTS = [
     "mt_collide", "mt_stream", "mt_explode", "mt_coalesce", "mt_apply_bc", "mt_ib_treat", "mt_step",
     "mt_set_graphs", "mt_set_timing", "mt_get_timing", "mt_sync", "mt_reduce", "mt_ib_force",
     "mt_case_set_members", "mt_ib_force_members", "mt_autotune", "mt_case_copy_member",
+    "mt_case_upload_markers",
 ]
 
 _lib = None
@@ -93,6 +94,7 @@ def load():
     L.mt_ib_force_members.argtypes = [C.c_void_p, _dp, C.c_int32, _i64p, C.c_int]
     L.mt_autotune.argtypes = [C.c_void_p, C.c_int, _dp]
     L.mt_case_copy_member.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]
+    L.mt_case_upload_markers.argtypes = [C.c_void_p, C.c_int64, _u64p, _dp, _dp, _dp]
     _lib = L
     return L
 
@@ -151,6 +153,16 @@ class AmrCase:
             dk, pdk = _u64(case["dis_keys"])
             dv = np.ascontiguousarray(case["dis_vals"], dtype=np.float64)
             _check(L.mt_case_upload_dis(self.h, pdk, dv.ctypes.data_as(_dp), dk.size))
+        mk = case.get("markers")
+        if mk is not None:
+            mkeys, pmk = _u64(mk["keys"])
+            w = np.ascontiguousarray(mk["w"], dtype=np.float64)
+            dvm = np.ascontiguousarray(mk["dv"], dtype=np.float64)
+            ub = np.ascontiguousarray(mk["ub"], dtype=np.float64)
+            if mkeys.shape != (dvm.size, 27) or w.shape != mkeys.shape or ub.size != 3:
+                raise MtError("markers: keys and w are [n][27], dv [n], ub [3]")
+            _check(L.mt_case_upload_markers(self.h, dvm.size, pmk, w.ctypes.data_as(_dp),
+                                            dvm.ctypes.data_as(_dp), ub.ctypes.data_as(_dp)))
         _check(L.mt_case_finalize(self.h))
         _check(L.mt_set_graphs(self.h, 1 if graphs else 0))
 

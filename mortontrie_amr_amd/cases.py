@@ -589,3 +589,73 @@ def sphere_request(rng, U=0.04, shell=3.0, options=2):
                       name=f"sphere{dims[0]}_r{r:.2f}_Re{Re:.0f}")
     case["request"] = {"dims": dims, "radius": r, "Re": Re}
     return case
+
+
+# ---- Lagrangian-marker sphere ----------------------------------------------------
+def roma3(r):
+    """Three-point kernel of Roma, Peskin & Berger (J. Comput. Phys. 153, 1999):
+    sums to 1 over unit-spaced nodes and has zero first moment."""
+    r = np.abs(np.asarray(r, dtype=np.float64))
+    out = np.zeros_like(r)
+    a = r <= 0.5
+    out[a] = (1.0 + np.sqrt(1.0 - 3.0 * r[a] ** 2)) / 3.0
+    b = (r > 0.5) & (r < 1.5)
+    out[b] = (5.0 - 3.0 * r[b] - np.sqrt(np.maximum(0.0, 1.0 - 3.0 * (1.0 - r[b]) ** 2))) / 6.0
+    return out
+
+
+def fibonacci_sphere(n):
+    """n points on the unit sphere: z_i = 1 - 2(i + 1/2)/n, azimuth i times the
+    golden angle pi (3 - sqrt 5)."""
+    i = np.arange(n, dtype=np.float64)
+    z = 1.0 - 2.0 * (i + 0.5) / n
+    rr = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    phi = i * np.pi * (3.0 - np.sqrt(5.0))
+    return np.stack([np.cos(phi) * rr, np.sin(phi) * rr, z], axis=1)
+
+
+def marker_stencils(xyz, Lmax):
+    """Per marker (finest-level cell units, cell i centred at i + 1/2) the 27
+    cells of its 3-point kernel support and their weights."""
+    c = np.asarray(xyz, dtype=np.float64) - 0.5          # node-index coordinates
+    base = np.floor(c).astype(np.int64) - 1                # candidates base .. base + 3
+    keys, ws = [], []
+    offs = np.array([(i, j, k) for k in range(4) for j in range(4) for i in range(4)])
+    for m in range(c.shape[0]):
+        idx = base[m][None, :] + offs
+        w = roma3(idx[:, 0] - c[m, 0]) * roma3(idx[:, 1] - c[m, 1]) * roma3(idx[:, 2] - c[m, 2])
+        keep = np.nonzero(w > 0)[0]
+        if keep.size > 27:
+            raise ValueError("marker support exceeds 27 cells")
+        kk = np.zeros(27, np.uint64)
+        ww = np.zeros(27)
+        kk[:keep.size] = encode_key(idx[keep, 0], idx[keep, 1], idx[keep, 2], Lmax, Lmax)
+        kk[keep.size:] = kk[0]
+        ww[:keep.size] = w[keep]
+        keys.append(kk)
+        ws.append(ww)
+    return np.array(keys, dtype=np.uint64), np.array(ws)
+
+
+def sphere_markers(coarse=(64, 32, 32), radius=8.0, center=(16.0, 16.0, 16.0), shells=(6.0, 3.0),
+                   n_markers=6000, Q=27, collision=COLLIDE_CUMULANT, U=0.04, Re=100.0, les_C=0.1, options=2):
+    """The sphere of sphere_amr represented by n_markers Lagrangian markers on a
+    Fibonacci lattice (explicit direct forcing on the finest level, see the
+    device's marker kernels) instead of cut links; the grid is the same refined
+    channel with no solid cells.  Smagorinsky constant les_C."""
+    case = sphere_amr(coarse, radius=radius, center=center, shells=shells, Q=Q, collision=collision, U=U,
+                      Re=Re, les_C=les_C, with_sphere=False, options=options, name="sphere_markers")
+    Lmax = case["Lmax"]
+    s = float(1 << Lmax)
+    R = radius * s
+    xyz = np.asarray(center, dtype=np.float64) * s + R * fibonacci_sphere(n_markers)
+    keys, w = marker_stencils(xyz, Lmax)
+    top = case["levels"][Lmax]
+    pure = top["keys"][((top["kinds"] & KIND_FLUID) != 0)
+                       & ((top["kinds"] & (KIND_C2F | KIND_GHOST_C2F | KIND_SOLID)) == 0)]
+    if not np.isin(keys[w != 0], pure).all():
+        raise ValueError("marker support reaches beyond the finest level's fluid cells: widen shells[-1]")
+    case["markers"] = {"xyz": xyz, "keys": keys, "w": w, "dv": np.full(n_markers, 4.0 * np.pi * R * R / n_markers),
+                       "ub": np.zeros(3)}
+    case["cd"] = {"U": reported(U), "d": 2.0 * R}
+    return case

@@ -57,6 +57,8 @@ static int prologue(Case& c) {
     TRY(launch_macros(c, L));
     TRY(launch_collide(c, L));
   }
+  // the markers act on every finest-level collision
+  TRY(launch_markers(c, c.lv[c.Lmax].pf, c.st[c.Lmax]));
   c.ahead = true;
   return MT_OK;
 }
@@ -89,6 +91,7 @@ static int substep(Case& c, int L, bool child_last, bool write_f, bool nee, bool
   }
   if (explode) TRY(launch_explode(c, L, explode == 2));
   TRY(launch_fused(c, L, write_f, nee, write_macros, c2f_fcol, coal));
+  if (L == c.Lmax) TRY(launch_markers(c, 1 - l.pf, s));
   if (L < c.Lmax) {
     MT_HIP(hipEventRecord(c.ev_done[L + 1], c.st[L + 1]));
     MT_HIP(hipStreamWaitEvent(s, c.ev_done[L + 1], 0));
@@ -335,11 +338,26 @@ int mt_case_set_members(mt_case* c, const mt_member_desc* m, int32_t n) {
   return MT_OK;
 }
 
+int mt_case_upload_markers(mt_case* c, int64_t n, const uint64_t* keys27, const double* w27, const double* dv,
+                           const double* ub3) {
+  CHECK_CASE(c);
+  if (c->finalized) { set_error("case already finalized"); return MT_ERR_STATE; }
+  if (n < 0 || (n && (!keys27 || !w27 || !dv || !ub3))) { set_error("bad marker arrays"); return MT_ERR_ARG; }
+  c->n_mk = n;
+  c->h_mk_keys.assign(keys27, keys27 + 27 * n);
+  c->h_mk_w.assign(w27, w27 + 27 * n);
+  c->h_mk_dv.assign(dv, dv + n);
+  for (int a = 0; a < 3 && n; ++a) c->mk_ub[a] = ub3[a];
+  return MT_OK;
+}
+
 int mt_case_finalize(mt_case* c) {
   CHECK_CASE(c);
   if (c->finalized) { set_error("case already finalized"); return MT_ERR_STATE; }
+  if (c->n_mk && c->h_mdesc.size() > 1) { set_error("markers need a single-member case"); return MT_ERR_ARG; }
   TRY(build_members(*c));
   TRY(build_tables(*c));
+  TRY(build_markers(*c));
   int64_t max_fp = 1;
   for (int L = 0; L <= c->Lmax; ++L) {
     DevLevel& l = c->lv[L];
@@ -488,7 +506,9 @@ int mt_collide(mt_case* c, int level) {
   CHECK_FINAL(c);
   CHECK_LEVEL(c, level);
   TRY(settle(*c));
-  return launch_collide(*c, level);
+  TRY(launch_collide(*c, level));
+  if (level == c->Lmax) TRY(launch_markers(*c, c->lv[level].pf, c->st[level]));
+  return MT_OK;
 }
 int mt_stream(mt_case* c, int level) {
   CHECK_FINAL(c);

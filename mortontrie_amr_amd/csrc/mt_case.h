@@ -105,6 +105,21 @@ struct Case {
   std::vector<mt_member_desc> h_mdesc;  // members as given (one by default)
   std::vector<MemberParams> h_mem;
   MemberParams* d_mem = nullptr;
+  // Lagrangian markers on the finest level (explicit direct forcing)
+  int64_t n_mk = 0;
+  std::vector<uint64_t> h_mk_keys;  // [n][27] upload staging
+  std::vector<double> h_mk_w;
+  std::vector<double> h_mk_dv;
+  double mk_ub[3] = {0, 0, 0};
+  int32_t* mk_slot = nullptr;       // [n][27]
+  double* mk_w = nullptr;           // [n][27]
+  double* mk_dv = nullptr;          // [n]
+  double* mk_F = nullptr;           // [n][3] this call's marker forces
+  int32_t* band_slot = nullptr;     // cells in some marker's support
+  int64_t n_band = 0;
+  int32_t* band_ptr = nullptr;      // [n_band + 1]
+  int32_t* band_mk = nullptr;       // markers of each cell, ascending
+  double* band_w = nullptr;
   std::vector<uint64_t> h_dis_keys;
   std::vector<double> h_dis_vals;
   bool finalized = false;
@@ -192,6 +207,8 @@ int launch_ib(Case& c, bool reduce_into_acc);
 int launch_fused(Case& c, int L, bool write_f, bool bc_nee_after, bool write_macros = true, bool c2f_fcol = true,
                  const CoalReq* coal = nullptr);
 bool level_has_ib(const Case& c, int L);
+int build_markers(Case& c);
+int launch_markers(Case& c, int buf, hipStream_t st);
 int launch_gather(Case& c, int L, int field, const int32_t* d_slots, int64_t n, double* d_out, int buf);
 int launch_scatter(Case& c, int L, int field, const int32_t* d_slots, int64_t n, const double* d_in, int buf);
 int launch_reduce(Case& c, int L, double* h_out);

@@ -337,7 +337,8 @@ struct FusedArgs {
   void* cu_out;
   int c_write_macros;
   // explosions inside the update (Case::xin)
-  const uint32_t* __restrict__ tmask;  // pull sources that are explosion targets
+  const uint32_t* __restrict__ tmask;  // bit q: the pull source of q is an explosion target
+                                       // (bit 0: the cell itself)
   const void* xbuf;                    // this pair's explosion targets (fcol[pf of level L-1])
   const int32_t* __restrict__ gxrow;   // ghost_F2C parent -> row of gx_child, -1 none
   const int32_t* __restrict__ gx_child;
@@ -516,7 +517,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
       const uint32_t pmx = a.v.pmask[s];
       const uint32_t tm = a.tmask ? a.tmask[s] : 0u;
       const S* xb = (const S*)a.xbuf;
-      f[0] = load_pop<Q, S, R>(fin, ns, 0, s);
+      f[0] = load_pop<Q, S, R>((tm & 1u) ? xb : fin, ns, 0, s);
       // pull f_q(x) <- fcol_q(x + e_q); a missing source keeps the cell's own f_q
       // (Stream_Manager.cpp:53-62), except on a treated solid-boundary cell,
       // which bounces back its own fcol_inv(q) (ComplexBoundary_Manager.cpp:
@@ -1421,6 +1422,173 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros
     int r = launch_nee(c);
     if (r) return r;
   }
+  return MT_OK;
+}
+
+// ------------------------------------------------------------------ markers
+// Explicit direct forcing of the finest level by Lagrangian markers (the
+// definition in mt_amr.h).  k_mk_interp: one thread per marker, its velocity
+// and density from the 27 weighted cells' moments of fcol.  k_mk_spread: one
+// thread per cell of the support band, the momentum of its markers (ascending
+// marker order) turned into the exact-difference population shift.
+template <int Q, typename S>
+__global__ __launch_bounds__(256) void k_mk_interp(const S* fcol, int64_t ns, const int32_t* slot, const double* w,
+                                                   const double* dv, int64_t n, double ubx, double uby, double ubz,
+                                                   double* F) {
+  typedef typename RealOf<S>::type R;
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  double rk = 0, ux = 0, uy = 0, uz = 0;
+  for (int j = 0; j < 27; ++j) {
+    const double wj = w[m * 27 + j];
+    if (wj == 0.0) continue;
+    const int sl = slot[m * 27 + j];
+    R f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = load_pop<Q, S, R>(fcol, ns, q, sl);
+    R rho, a, b, c;
+    cell_macros<Q, R>(f, rho, a, b, c);
+    rk += wj * (double)rho;
+    ux += wj * (double)a;
+    uy += wj * (double)b;
+    uz += wj * (double)c;
+  }
+  F[3 * m] = dv[m] * rk * (ubx - ux);
+  F[3 * m + 1] = dv[m] * rk * (uby - uy);
+  F[3 * m + 2] = dv[m] * rk * (ubz - uz);
+}
+
+template <int Q, typename S>
+__global__ __launch_bounds__(256) void k_mk_spread(S* fcol, int64_t ns, const int32_t* band, const int32_t* ptr,
+                                                   const int32_t* mk, const double* bw, int64_t nb, const double* F) {
+  typedef typename RealOf<S>::type R;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  double mx = 0, my = 0, mz = 0;
+  for (int32_t e = ptr[i]; e < ptr[i + 1]; ++e) {
+    const double we = bw[e];
+    const int32_t k = mk[e];
+    mx += we * F[3 * k];
+    my += we * F[3 * k + 1];
+    mz += we * F[3 * k + 2];
+  }
+  const int sl = band[i];
+  R f[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) f[q] = load_pop<Q, S, R>(fcol, ns, q, sl);
+  R rho, ux, uy, uz;
+  cell_macros<Q, R>(f, rho, ux, uy, uz);
+  const R vx = ux + (R)(mx / (double)rho), vy = uy + (R)(my / (double)rho), vz = uz + (R)(mz / (double)rho);
+  const R uu = ux * ux + uy * uy + uz * uz, vv = vx * vx + vy * vy + vz * vz;
+#pragma unroll
+  for (int q = 0; q < Q; ++q)
+    store_pop<Q, S, R>(fcol, ns, q, sl, f[q] + (feq_c<Q, R>(q, rho, vx, vy, vz, vv) - feq_c<Q, R>(q, rho, ux, uy, uz, uu)));
+}
+
+// the body force -sum_k F_k of this call into acc[0..2] (count in acc[3]), in
+// a fixed order: 256 strided partial sums, then a fixed tree
+__global__ __launch_bounds__(256) void k_mk_force(const double* F, int64_t n, double* acc) {
+  __shared__ double red[3][256];
+  double s[3] = {0, 0, 0};
+  for (int64_t m = threadIdx.x; m < n; m += 256)
+    for (int a = 0; a < 3; ++a) s[a] -= F[3 * m + a];
+  for (int a = 0; a < 3; ++a) red[a][threadIdx.x] = s[a];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int a = 0; a < 3; ++a) red[a][threadIdx.x] += red[a][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int a = 0; a < 3; ++a) {
+      acc[a] += red[a][0];
+      acc[4 + a] = red[a][0];
+    }
+    acc[3] += 1.0;
+  }
+}
+
+int launch_markers(Case& c, int buf, hipStream_t st) {
+  if (!c.n_mk) return MT_OK;
+  DevLevel& l = c.lv[c.Lmax];
+  MT_DISPATCH(c, (k_mk_interp<Q_, S_><<<nblk(c.n_mk), 256, 0, st>>>((const S_*)l.fcol[buf], l.n_slots, c.mk_slot,
+                                                                     c.mk_w, c.mk_dv, c.n_mk, c.mk_ub[0], c.mk_ub[1],
+                                                                     c.mk_ub[2], c.mk_F)));
+  MT_DISPATCH(c, (k_mk_spread<Q_, S_><<<nblk(c.n_band), 256, 0, st>>>((S_*)l.fcol[buf], l.n_slots, c.band_slot,
+                                                                      c.band_ptr, c.band_mk, c.band_w, c.n_band,
+                                                                      c.mk_F)));
+  k_mk_force<<<1, 256, 0, st>>>(c.mk_F, c.n_mk, c.force_acc);
+  MT_HIP(hipGetLastError());
+  return MT_OK;
+}
+
+// Device tables of the markers: the slots of their 27 cells (each must be a
+// streamed lat_f cell of the finest level) and the band of cells in their
+// support with each cell's markers in ascending order (CSR).
+int build_markers(Case& c) {
+  if (!c.n_mk) return MT_OK;
+  const int64_t n = c.n_mk, ne = 27 * n;
+  DevLevel& l = c.lv[c.Lmax];
+  uint64_t* dk = nullptr;
+  MT_HIP(hipMalloc((void**)&dk, 8 * ne));
+  c.mk_slot = c.alloc<int32_t>(ne);
+  c.mk_w = c.alloc<double>(ne);
+  c.mk_dv = c.alloc<double>(n);
+  c.mk_F = c.alloc<double>(3 * n);
+  if (!c.mk_slot || !c.mk_w || !c.mk_dv || !c.mk_F) {
+    set_error("out of device memory");
+    return MT_ERR_HIP;
+  }
+  MT_HIP(hipMemcpyAsync(dk, c.h_mk_keys.data(), 8 * ne, hipMemcpyHostToDevice, c.stream));
+  MT_HIP(hipMemcpyAsync(c.mk_w, c.h_mk_w.data(), 8 * ne, hipMemcpyHostToDevice, c.stream));
+  MT_HIP(hipMemcpyAsync(c.mk_dv, c.h_mk_dv.data(), 8 * n, hipMemcpyHostToDevice, c.stream));
+  int r = find_slots(c, c.Lmax, dk, ne, c.mk_slot);
+  if (r) return r;
+  std::vector<int32_t> hs(ne);
+  std::vector<uint16_t> kind(l.n_slots);
+  MT_HIP(hipMemcpyAsync(hs.data(), c.mk_slot, 4 * ne, hipMemcpyDeviceToHost, c.stream));
+  MT_HIP(hipMemcpyAsync(kind.data(), l.kind, 2 * l.n_slots, hipMemcpyDeviceToHost, c.stream));
+  MT_HIP(hipStreamSynchronize(c.stream));
+  MT_HIP(hipFree(dk));
+  struct E { int32_t slot, mk; double w; };
+  std::vector<E> es;
+  for (int64_t m = 0; m < n; ++m)
+    for (int j = 0; j < 27; ++j) {
+      const double w = c.h_mk_w[m * 27 + j];
+      if (w == 0.0) continue;
+      const int32_t sl = hs[m * 27 + j];
+      if (sl < 0 || !(kind[sl] & MT_KIND_FLUID) || (kind[sl] & (MT_KIND_C2F | MT_KIND_DIS | MT_KIND_SOLID))) {
+        set_error("marker " + std::to_string(m) + ": a kernel cell is not a streamed fluid cell of the finest level");
+        return MT_ERR_ARG;
+      }
+      es.push_back({sl, (int32_t)m, w});
+    }
+  std::stable_sort(es.begin(), es.end(), [](const E& a, const E& b) { return a.slot != b.slot ? a.slot < b.slot : a.mk < b.mk; });
+  std::vector<int32_t> band, ptr, mk;
+  std::vector<double> bw;
+  for (size_t i = 0; i < es.size(); ++i) {
+    if (i == 0 || es[i].slot != es[i - 1].slot) {
+      band.push_back(es[i].slot);
+      ptr.push_back((int32_t)i);
+    }
+    mk.push_back(es[i].mk);
+    bw.push_back(es[i].w);
+  }
+  ptr.push_back((int32_t)es.size());
+  c.n_band = (int64_t)band.size();
+  c.band_slot = c.alloc<int32_t>(c.n_band);
+  c.band_ptr = c.alloc<int32_t>(c.n_band + 1);
+  c.band_mk = c.alloc<int32_t>((int64_t)mk.size());
+  c.band_w = c.alloc<double>((int64_t)bw.size());
+  if (!c.band_slot || !c.band_ptr || !c.band_mk || !c.band_w) {
+    set_error("out of device memory");
+    return MT_ERR_HIP;
+  }
+  MT_HIP(hipMemcpyAsync(c.band_slot, band.data(), 4 * band.size(), hipMemcpyHostToDevice, c.stream));
+  MT_HIP(hipMemcpyAsync(c.band_ptr, ptr.data(), 4 * ptr.size(), hipMemcpyHostToDevice, c.stream));
+  MT_HIP(hipMemcpyAsync(c.band_mk, mk.data(), 4 * mk.size(), hipMemcpyHostToDevice, c.stream));
+  MT_HIP(hipMemcpyAsync(c.band_w, bw.data(), 8 * bw.size(), hipMemcpyHostToDevice, c.stream));
+  MT_HIP(hipStreamSynchronize(c.stream));
   return MT_OK;
 }
 

@@ -367,6 +367,9 @@ __global__ void k_tmask(const int32_t* bnbr, const uint16_t* kind, const uint32_
       const int t = slot_at(nb27, lx, ly, lz, kEX[q], kEY[q], kEZ[q]);
       if (t >= 0 && (kind[t] & (MT_KIND_C2F | MT_KIND_GHOST_C2F)) && !(kind[t] & MT_KIND_DIS)) m |= 1u << q;
     }
+    // bit 0: the cell is a target itself, so its rest population (the pull
+    // from its own slot) is in the target buffer too
+    if ((kind[s] & MT_KIND_C2F) && !(kind[s] & MT_KIND_DIS)) m |= 1u;
   }
   tmask[s] = m;
 }

@@ -19,6 +19,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <unordered_map>
 #include <unordered_set>
 #include <vector>
@@ -333,6 +334,73 @@ struct Case {
   std::vector<Level> lv;
   std::map<uint64_t, std::array<double, 26>> dis;  // ascending-key iteration
   std::vector<std::array<double, 3>> force_log;
+  // Lagrangian markers on the finest level (definition, no reference
+  // counterpart): per marker up to 27 (cell key, weight) pairs of a 3-point
+  // kernel, its surface element dV and the body velocity ub.
+  int64_t n_mk = 0;
+  std::vector<uint64_t> mk_keys;  // [n][27]
+  std::vector<double> mk_w;       // [n][27]
+  std::vector<double> mk_dv;      // [n]
+  double mk_ub[3] = {0, 0, 0};
+
+  // Explicit direct forcing on the post-collision populations of the finest
+  // level: the marker velocity and density are kernel-weighted sums of the
+  // cells' moments (taken from fcol), the marker force density is
+  // F_k = dV_k rho_k (ub - u_k), each cell receives the momentum
+  // dm(x) = sum_k w_k(x) F_k and its populations shift by the exact-difference
+  // term feq(rho, u + dm/rho) - feq(rho, u) (Kupershtokh's method).  The body
+  // takes -sum_k F_k (logged like a treat() force).
+  void marker_correct() {
+    if (!n_mk) return;
+    Level& l = lv[Lmax];
+    std::vector<std::array<double, 3>> F(n_mk);
+    auto moments = [&](uint64_t k, double& rho, double* u) {
+      double f[27];
+      load(Lmax, k, f, true);
+      macros(Q, f, rho, u);
+    };
+    for (int64_t m = 0; m < n_mk; ++m) {
+      double rk = 0, uk[3] = {0, 0, 0};
+      for (int j = 0; j < 27; ++j) {
+        const double w = mk_w[m * 27 + j];
+        if (w == 0.0) continue;
+        double rho, u[3];
+        moments(mk_keys[m * 27 + j], rho, u);
+        rk += w * rho;
+        for (int a = 0; a < 3; ++a) uk[a] += w * u[a];
+      }
+      for (int a = 0; a < 3; ++a) F[m][a] = mk_dv[m] * rk * (mk_ub[a] - uk[a]);
+    }
+    // spread in ascending (cell key, marker) order
+    std::map<uint64_t, std::array<double, 3>> dm;
+    std::vector<std::tuple<uint64_t, int64_t, double>> pairs;
+    for (int64_t m = 0; m < n_mk; ++m)
+      for (int j = 0; j < 27; ++j)
+        if (mk_w[m * 27 + j] != 0.0) pairs.emplace_back(mk_keys[m * 27 + j], m, mk_w[m * 27 + j]);
+    std::sort(pairs.begin(), pairs.end(), [](auto& a, auto& b) {
+      return std::get<0>(a) != std::get<0>(b) ? std::get<0>(a) < std::get<0>(b) : std::get<1>(a) < std::get<1>(b);
+    });
+    for (auto& p : pairs) {
+      auto& d = dm[std::get<0>(p)];
+      for (int a = 0; a < 3; ++a) d[a] += std::get<2>(p) * F[std::get<1>(p)][a];
+    }
+    for (auto& kv : dm) {
+      double f[27], rho, u[3], u2[3], fe1[27], fe2[27];
+      load(Lmax, kv.first, f, true);
+      macros(Q, f, rho, u);
+      for (int a = 0; a < 3; ++a) u2[a] = u[a] + kv.second[a] / rho;
+      feq_c(Q, rho, u2, fe2);
+      feq_c(Q, rho, u, fe1);
+      for (int q = 0; q < Q; ++q) l.fcol[q].at(kv.first) = f[q] + (fe2[q] - fe1[q]);
+    }
+    double fx = 0, fy = 0, fz = 0;
+    for (int64_t m = 0; m < n_mk; ++m) {
+      fx -= F[m][0];
+      fy -= F[m][1];
+      fz -= F[m][2];
+    }
+    force_log.push_back({fx, fy, fz});
+  }
 
   // neighbour with periodic wrap on periodic axes (definition: periodic = wrapped
   // neighbour lookup, the limit of the ghost-copy text in BC/Periodic.cpp:8-445)
@@ -799,6 +867,7 @@ struct Case {
   void kernel(int L) {
     calc_macros(L);
     collide(L);
+    if (L == Lmax) marker_correct();
     if (L < Lmax) kernel(L + 1);
     if (L > 0) explode(L);
     stream(L);
@@ -808,6 +877,7 @@ struct Case {
     if (L == 0) return;
     calc_macros(L);
     collide(L);
+    if (L == Lmax) marker_correct();
     if (L < Lmax) kernel(L + 1);
     if (options & 2) explode(L);
     stream(L);
@@ -861,6 +931,16 @@ int mto_add_level(void* h, int L, const uint64_t* keys, const uint16_t* kinds, i
   }
   return 0;
 }
+int mto_set_markers(void* h, int64_t n, const uint64_t* keys, const double* w, const double* dv, const double* ub) {
+  Case* c = (Case*)h;
+  c->n_mk = n;
+  c->mk_keys.assign(keys, keys + 27 * n);
+  c->mk_w.assign(w, w + 27 * n);
+  c->mk_dv.assign(dv, dv + n);
+  for (int a = 0; a < 3; ++a) c->mk_ub[a] = ub[a];
+  return 0;
+}
+
 int mto_set_dis(void* h, const uint64_t* keys, const double* vals, int64_t n) {
   Case* c = (Case*)h;
   int m = c->Q - 1;
@@ -922,6 +1002,7 @@ int mto_explode(void* h, int L) { ((Case*)h)->explode(L); return 0; }
 int mto_coalesce(void* h, int L) { ((Case*)h)->coalesce(L); return 0; }
 int mto_bc_after_stream(void* h) { ((Case*)h)->bc_after_stream(); return 0; }
 int mto_ib_treat(void* h, double* F) { ((Case*)h)->ib_treat(F); return 0; }
+int mto_marker_correct(void* h) { ((Case*)h)->marker_correct(); return 0; }
 int64_t mto_force_log(void* h, double* out, int64_t max) {
   Case* c = (Case*)h;
   int64_t n = (int64_t)c->force_log.size();

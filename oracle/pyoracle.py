@@ -39,6 +39,7 @@ def lib():
         L.mto_set_options.argtypes = [C.c_void_p, C.c_int]
         L.mto_add_level.argtypes = [C.c_void_p, C.c_int, _u64p, _u16p, C.c_int64]
         L.mto_set_dis.argtypes = [C.c_void_p, _u64p, _dp, C.c_int64]
+        L.mto_set_markers.argtypes = [C.c_void_p, C.c_int64, _u64p, _dp, _dp, _dp]
         L.mto_init.argtypes = [C.c_void_p]
         L.mto_step.argtypes = [C.c_void_p, C.c_int]
         L.mto_get.argtypes = [C.c_void_p, C.c_int, C.c_int, _u64p, C.c_int64, _dp]
@@ -47,6 +48,7 @@ def lib():
             getattr(L, name).argtypes = [C.c_void_p, C.c_int]
         L.mto_bc_after_stream.argtypes = [C.c_void_p]
         L.mto_ib_treat.argtypes = [C.c_void_p, _dp]
+        L.mto_marker_correct.argtypes = [C.c_void_p]
         L.mto_force_log.restype = C.c_int64
         L.mto_force_log.argtypes = [C.c_void_p, _dp, C.c_int64]
         L.mto_feq.argtypes = [C.c_int, C.c_double, _dp, _dp]
@@ -100,6 +102,14 @@ class OracleCase:
             dk, pdk = _k(case["dis_keys"])
             dv, pdv = _d(case["dis_vals"])
             L.mto_set_dis(self.h, pdk, pdv, dk.size)
+        mk = case.get("markers")
+        if mk is not None:
+            keys = np.ascontiguousarray(mk["keys"], dtype=np.uint64)
+            w = np.ascontiguousarray(mk["w"], dtype=np.float64)
+            dv = np.ascontiguousarray(mk["dv"], dtype=np.float64)
+            ub = np.ascontiguousarray(mk["ub"], dtype=np.float64)
+            L.mto_set_markers(self.h, dv.size, keys.ctypes.data_as(_u64p), w.ctypes.data_as(_dp),
+                              dv.ctypes.data_as(_dp), ub.ctypes.data_as(_dp))
         L.mto_init(self.h)
 
     def __del__(self):
@@ -162,6 +172,10 @@ class OracleCase:
         F = np.zeros(3)
         lib().mto_ib_treat(self.h, F.ctypes.data_as(_dp))
         return F
+
+    def marker_correct(self):
+        """The markers' direct forcing of the finest level's fcol (one call)."""
+        lib().mto_marker_correct(self.h)
 
     def force_log(self):
         n = lib().mto_force_log(self.h, None, 0)

@@ -145,6 +145,25 @@ def test_sphere_three_levels(kw):
     assert np.allclose(s, fl.sum(0), rtol=1e-11, atol=1e-11)
 
 
+@pytest.mark.parametrize("opts", [2, 3])
+def test_sphere_explosions_inside_updates(opts):
+    """A wider finest shell keeps the treated cells away from the C2F layer, so
+    with both-sub-step explosions the fused updates carry the explosions
+    themselves (targets double-buffered per coarse parity).  Steps 2+ read the
+    target buffer that differs from the level's own fcol, the case that shows
+    whether every read of a target (including a C2F cell's rest population)
+    goes to the target buffer."""
+    case = C.sphere_amr((32, 24, 24), radius=3.0, center=(12, 12, 12), shells=(5.5, 2.0), options=opts)
+    g, o = amr.AmrCase(case), O.OracleCase(case)
+    g.step(5)
+    o.step(5)
+    compare(g, o, case)
+    fl = o.force_log()
+    s, n = g.ib_force()
+    assert n == len(fl)
+    assert np.allclose(s, fl.sum(0), rtol=1e-11, atol=1e-11)
+
+
 def test_reference_sphere_layout():
     """The configuration the compiled reference runs: LBGK, reservoir faces with
     NEE, FH + MDF on the finest of 3 levels."""
