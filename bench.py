@@ -16,6 +16,13 @@ events, the same case in FP32-shifted and FP16-storage populations with their
 errors against the FP64 run, and the map-faithful CPU oracle timed on a bounded
 sample of the same workload as the baseline.
 
+--workload markers runs the same grid with the sphere given by 6000 Lagrangian
+markers (Fibonacci lattice, explicit direct forcing on the finest level) and
+Smagorinsky constant 0.1 for 5000 steps, in FP64, FP32-shifted and FP16-storage,
+with each variant's drag, velocity / density L2 and mass drift against FP64.
+--workload batch and --workload serve run the packed-cavity batch and the
+request stream.
+
 --counters runs a short eager pass plus a calibration copy for the rocprofv3
 FETCH_SIZE / WRITE_SIZE passes whose summary (profiles/) gives roofline.traffic.
 """
@@ -36,6 +43,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "MLUPS & % HBM roofline (FP64 vs FP32-shifted); served cases/s + p50/p99 latency"
 WORKLOAD = "sphere_amr_3level_d3q27_cumulant_fp64"
+MARKER_WORKLOAD = "sphere_markers6000_3level_d3q27_cumulant_les0.1_fp64"
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_fused_sphere_amr_fp64_hbm_traffic.md")
 
 
@@ -261,8 +269,9 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--counters", action="store_true",
                     help="short eager run plus a calibration copy, for rocprofv3 --pmc passes")
-    ap.add_argument("--workload", choices=("sphere", "batch", "serve"), default="sphere",
-                    help="sphere: the 3-level sphere (default); batch: packed cavities; serve: request stream")
+    ap.add_argument("--workload", choices=("sphere", "markers", "batch", "serve"), default="sphere",
+                    help="sphere: the 3-level sphere (default); markers: the same sphere as 6000 Lagrangian "
+                         "markers with Smagorinsky LES, 5000 steps; batch: packed cavities; serve: request stream")
     ap.add_argument("--batch", type=int, default=256, help="members of the packed batch")
     ap.add_argument("--requests", type=int, default=32, help="requests of the serving run")
     args = ap.parse_args()
@@ -280,7 +289,16 @@ def main():
         run_serve(args)
         return
     from mortontrie_amr_amd import cases as C, amr
-    case = C.sphere_amr(options=2)
+    if args.workload == "markers":
+        # with the reference's f_neq rescaling at the interfaces (options=2) this
+        # flow diverges after ~1700 steps; the convective rescaling keeps it finite
+        case = C.sphere_markers(n_markers=6000, les_C=0.1, options=3)
+        workload = MARKER_WORKLOAD
+        if args.steps == 4000:
+            args.steps = 5000
+    else:
+        case = C.sphere_amr(options=2)
+        workload = WORKLOAD
     if args.counters:
         counter_run(case, 4, 5, 256)
         return
@@ -308,7 +326,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": WORKLOAD,
+            "workload": workload,
             "coarse": list(case["dims0"]), "sphere_radius": 8, "levels": Lf + 1,
             "updates_per_level": per_level, "updates_per_coarse_step": upd_step,
             "tau0": case["tau0"], "parallelism": "single_gpu",
@@ -323,8 +341,11 @@ def main():
         "nan_count": r["nan"],
         "drag_coefficient": round(r["cd"], 5),
         "mass_drift": (r["mass1"] - r["mass0"]) / r["mass0"],
-        "interface_option": "explode_both",
+        "interface_option": "explode_both" + ("+convective_xfer" if case["options"] & 1 else ""),
     }
+    if "markers" in case:
+        out["config"].update(markers=int(case["markers"]["dv"].size), les_C=case["les_C"],
+                             solid_boundary="direct forcing by Lagrangian markers")
     if not args.no_variants:
         var = {}
         for name, prec in (("fp32_shifted", amr.FP32_SHIFTED), ("fp16_storage", amr.FP16_STORAGE)):

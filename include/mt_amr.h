@@ -136,8 +136,10 @@ int mt_case_set_members(mt_case* c, const mt_member_desc* members, int32_t n);
  * velocity ub[3].  Every finest sub-step of mt_step applies explicit direct
  * forcing to the post-collision populations: u_k and rho_k are the weighted sums
  * of the cells' moments, F_k = dV_k rho_k (ub - u_k), each cell's populations
- * shift by feq(rho, u + sum_k w F_k / rho) - feq(rho, u); -sum_k F_k adds to the
- * force mt_ib_force reports.  Before mt_case_finalize; single-member cases.    */
+ * shift by feq(rho, u + sum_k w F_k / rho) - feq(rho, u).  sum_k F_k adds to
+ * the force mt_ib_force reports: the body's reaction -sum_k F_k with the sign of
+ * the reported velocities flipped back, as in the link-wise force (a drag along
+ * the physical flow is positive).  Before mt_case_finalize; single-member cases. */
 int mt_case_upload_markers(mt_case* c, int64_t n, const uint64_t* keys27, const double* weights27,
                            const double* dV, const double* ub3);
 int mt_case_level_info(mt_case* c, int level, int64_t* n_cells, int64_t* n_bricks);

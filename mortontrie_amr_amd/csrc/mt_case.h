@@ -112,6 +112,8 @@ struct Case {
   std::vector<double> h_mk_dv;
   double mk_ub[3] = {0, 0, 0};
   int32_t* mk_slot = nullptr;       // [n][27]
+  int32_t* mk_band = nullptr;       // [n][27] band index of each kernel cell
+  double* band_mom = nullptr;       // [4][n_band] rho, u of the band cells (this call)
   double* mk_w = nullptr;           // [n][27]
   double* mk_dv = nullptr;          // [n]
   double* mk_F = nullptr;           // [n][3] this call's marker forces

@@ -1427,40 +1427,66 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros
 
 // ------------------------------------------------------------------ markers
 // Explicit direct forcing of the finest level by Lagrangian markers (the
-// definition in mt_amr.h).  k_mk_interp: one thread per marker, its velocity
-// and density from the 27 weighted cells' moments of fcol.  k_mk_spread: one
-// thread per cell of the support band, the momentum of its markers (ascending
-// marker order) turned into the exact-difference population shift.
+// definition in mt_amr.h), four kernels per finest sub-step:
+//   k_mk_moments  one thread per band cell (a cell in some marker's support):
+//                 its density and velocity from fcol;
+//   k_mk_interp   one 32-lane half-wave per marker, lane j < 27 its j-th kernel
+//                 cell: the weighted moments summed by a shuffle tree, then
+//                 F_k = dV_k rho_k (ub - u_k);
+//   k_mk_spread   one thread per band cell: the momentum of its markers
+//                 (ascending marker order) turned into the exact-difference
+//                 population shift;
+//   k_mk_force    the body force sum_k F_k in a fixed order.
 template <int Q, typename S>
-__global__ __launch_bounds__(256) void k_mk_interp(const S* fcol, int64_t ns, const int32_t* slot, const double* w,
-                                                   const double* dv, int64_t n, double ubx, double uby, double ubz,
-                                                   double* F) {
+__global__ __launch_bounds__(256) void k_mk_moments(const S* fcol, int64_t ns, const int32_t* band, int64_t nb,
+                                                    double* mom) {
   typedef typename RealOf<S>::type R;
-  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n) return;
-  double rk = 0, ux = 0, uy = 0, uz = 0;
-  for (int j = 0; j < 27; ++j) {
-    const double wj = w[m * 27 + j];
-    if (wj == 0.0) continue;
-    const int sl = slot[m * 27 + j];
-    R f[Q];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const int sl = band[i];
+  R f[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f[q] = load_pop<Q, S, R>(fcol, ns, q, sl);
-    R rho, a, b, c;
-    cell_macros<Q, R>(f, rho, a, b, c);
-    rk += wj * (double)rho;
-    ux += wj * (double)a;
-    uy += wj * (double)b;
-    uz += wj * (double)c;
+  for (int q = 0; q < Q; ++q) f[q] = load_pop<Q, S, R>(fcol, ns, q, sl);
+  R rho, ux, uy, uz;
+  cell_macros<Q, R>(f, rho, ux, uy, uz);
+  mom[i] = (double)rho;
+  mom[nb + i] = (double)ux;
+  mom[2 * nb + i] = (double)uy;
+  mom[3 * nb + i] = (double)uz;
+}
+
+__global__ __launch_bounds__(256) void k_mk_interp(const double* mom, int64_t nb, const int32_t* mk_band,
+                                                   const double* w, const double* dv, int64_t n, double ubx,
+                                                   double uby, double ubz, double* F) {
+  const int64_t m = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
+  const int j = threadIdx.x & 31;
+  double r = 0, a = 0, b = 0, c = 0;
+  if (m < n && j < 27) {
+    // unused entries carry weight 0 and a valid band index
+    const double wj = w[m * 27 + j];
+    const int32_t i = mk_band[m * 27 + j];
+    r = wj * mom[i];
+    a = wj * mom[nb + i];
+    b = wj * mom[2 * nb + i];
+    c = wj * mom[3 * nb + i];
   }
-  F[3 * m] = dv[m] * rk * (ubx - ux);
-  F[3 * m + 1] = dv[m] * rk * (uby - uy);
-  F[3 * m + 2] = dv[m] * rk * (ubz - uz);
+  for (int off = 16; off > 0; off >>= 1) {
+    r += __shfl_down(r, off, 32);
+    a += __shfl_down(a, off, 32);
+    b += __shfl_down(b, off, 32);
+    c += __shfl_down(c, off, 32);
+  }
+  if (m < n && j == 0) {
+    F[3 * m] = dv[m] * r * (ubx - a);
+    F[3 * m + 1] = dv[m] * r * (uby - b);
+    F[3 * m + 2] = dv[m] * r * (ubz - c);
+  }
 }
 
 template <int Q, typename S>
 __global__ __launch_bounds__(256) void k_mk_spread(S* fcol, int64_t ns, const int32_t* band, const int32_t* ptr,
-                                                   const int32_t* mk, const double* bw, int64_t nb, const double* F) {
+                                                   const int32_t* mk, const double* bw, int64_t nb,
+                                                   const double* mom, const double* F) {
   typedef typename RealOf<S>::type R;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nb) return;
@@ -1473,28 +1499,29 @@ __global__ __launch_bounds__(256) void k_mk_spread(S* fcol, int64_t ns, const in
     mz += we * F[3 * k + 2];
   }
   const int sl = band[i];
-  R f[Q];
-#pragma unroll
-  for (int q = 0; q < Q; ++q) f[q] = load_pop<Q, S, R>(fcol, ns, q, sl);
-  R rho, ux, uy, uz;
-  cell_macros<Q, R>(f, rho, ux, uy, uz);
+  // the moments were computed in R from the same fcol: the round trip through
+  // double is exact
+  const R rho = (R)mom[i], ux = (R)mom[nb + i], uy = (R)mom[2 * nb + i], uz = (R)mom[3 * nb + i];
   const R vx = ux + (R)(mx / (double)rho), vy = uy + (R)(my / (double)rho), vz = uz + (R)(mz / (double)rho);
   const R uu = ux * ux + uy * uy + uz * uz, vv = vx * vx + vy * vy + vz * vz;
 #pragma unroll
-  for (int q = 0; q < Q; ++q)
-    store_pop<Q, S, R>(fcol, ns, q, sl, f[q] + (feq_c<Q, R>(q, rho, vx, vy, vz, vv) - feq_c<Q, R>(q, rho, ux, uy, uz, uu)));
+  for (int q = 0; q < Q; ++q) {
+    const R fq = load_pop<Q, S, R>(fcol, ns, q, sl);
+    store_pop<Q, S, R>(fcol, ns, q, sl, fq + (feq_c<Q, R>(q, rho, vx, vy, vz, vv) - feq_c<Q, R>(q, rho, ux, uy, uz, uu)));
+  }
 }
 
-// the body force -sum_k F_k of this call into acc[0..2] (count in acc[3]), in
-// a fixed order: 256 strided partial sums, then a fixed tree
-__global__ __launch_bounds__(256) void k_mk_force(const double* F, int64_t n, double* acc) {
-  __shared__ double red[3][256];
+// the body force of this call, sum_k F_k (the frame of the link-wise force,
+// whose velocities are the reported ones), into acc[0..2] (count in acc[3]):
+// 1024 strided partial sums, then a fixed tree
+__global__ __launch_bounds__(1024) void k_mk_force(const double* F, int64_t n, double* acc) {
+  __shared__ double red[3][1024];
   double s[3] = {0, 0, 0};
-  for (int64_t m = threadIdx.x; m < n; m += 256)
-    for (int a = 0; a < 3; ++a) s[a] -= F[3 * m + a];
+  for (int64_t m = threadIdx.x; m < n; m += 1024)
+    for (int a = 0; a < 3; ++a) s[a] += F[3 * m + a];
   for (int a = 0; a < 3; ++a) red[a][threadIdx.x] = s[a];
   __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
+  for (int h = 512; h > 0; h >>= 1) {
     if ((int)threadIdx.x < h)
       for (int a = 0; a < 3; ++a) red[a][threadIdx.x] += red[a][threadIdx.x + h];
     __syncthreads();
@@ -1511,13 +1538,14 @@ __global__ __launch_bounds__(256) void k_mk_force(const double* F, int64_t n, do
 int launch_markers(Case& c, int buf, hipStream_t st) {
   if (!c.n_mk) return MT_OK;
   DevLevel& l = c.lv[c.Lmax];
-  MT_DISPATCH(c, (k_mk_interp<Q_, S_><<<nblk(c.n_mk), 256, 0, st>>>((const S_*)l.fcol[buf], l.n_slots, c.mk_slot,
-                                                                     c.mk_w, c.mk_dv, c.n_mk, c.mk_ub[0], c.mk_ub[1],
-                                                                     c.mk_ub[2], c.mk_F)));
+  MT_DISPATCH(c, (k_mk_moments<Q_, S_><<<nblk(c.n_band), 256, 0, st>>>((const S_*)l.fcol[buf], l.n_slots,
+                                                                       c.band_slot, c.n_band, c.band_mom)));
+  k_mk_interp<<<nblk(32 * c.n_mk), 256, 0, st>>>(c.band_mom, c.n_band, c.mk_band, c.mk_w, c.mk_dv, c.n_mk,
+                                                 c.mk_ub[0], c.mk_ub[1], c.mk_ub[2], c.mk_F);
   MT_DISPATCH(c, (k_mk_spread<Q_, S_><<<nblk(c.n_band), 256, 0, st>>>((S_*)l.fcol[buf], l.n_slots, c.band_slot,
                                                                       c.band_ptr, c.band_mk, c.band_w, c.n_band,
-                                                                      c.mk_F)));
-  k_mk_force<<<1, 256, 0, st>>>(c.mk_F, c.n_mk, c.force_acc);
+                                                                      c.band_mom, c.mk_F)));
+  k_mk_force<<<1, 1024, 0, st>>>(c.mk_F, c.n_mk, c.force_acc);
   MT_HIP(hipGetLastError());
   return MT_OK;
 }
@@ -1575,12 +1603,32 @@ int build_markers(Case& c) {
     bw.push_back(es[i].w);
   }
   ptr.push_back((int32_t)es.size());
+  const std::vector<int32_t>& band_sorted = band;  // ascending slots
+  // band index of every kernel entry (unused entries: the marker's first cell)
+  std::vector<int32_t> mb(ne);
+  for (int64_t m = 0; m < n; ++m) {
+    int32_t first = -1;
+    for (int j = 0; j < 27; ++j) {
+      const int64_t e = m * 27 + j;
+      if (c.h_mk_w[e] == 0.0) continue;
+      mb[e] = (int32_t)(std::lower_bound(band_sorted.begin(), band_sorted.end(), hs[e]) - band_sorted.begin());
+      if (first < 0) first = mb[e];
+    }
+    if (first < 0) {
+      set_error("marker " + std::to_string(m) + " has no kernel cell with a nonzero weight");
+      return MT_ERR_ARG;
+    }
+    for (int j = 0; j < 27; ++j)
+      if (c.h_mk_w[m * 27 + j] == 0.0) mb[m * 27 + j] = first;
+  }
   c.n_band = (int64_t)band.size();
   c.band_slot = c.alloc<int32_t>(c.n_band);
   c.band_ptr = c.alloc<int32_t>(c.n_band + 1);
   c.band_mk = c.alloc<int32_t>((int64_t)mk.size());
   c.band_w = c.alloc<double>((int64_t)bw.size());
-  if (!c.band_slot || !c.band_ptr || !c.band_mk || !c.band_w) {
+  c.band_mom = c.alloc<double>(4 * c.n_band);
+  c.mk_band = c.alloc<int32_t>(ne);
+  if (!c.band_slot || !c.band_ptr || !c.band_mk || !c.band_w || !c.band_mom || !c.mk_band) {
     set_error("out of device memory");
     return MT_ERR_HIP;
   }
@@ -1588,6 +1636,7 @@ int build_markers(Case& c) {
   MT_HIP(hipMemcpyAsync(c.band_ptr, ptr.data(), 4 * ptr.size(), hipMemcpyHostToDevice, c.stream));
   MT_HIP(hipMemcpyAsync(c.band_mk, mk.data(), 4 * mk.size(), hipMemcpyHostToDevice, c.stream));
   MT_HIP(hipMemcpyAsync(c.band_w, bw.data(), 8 * bw.size(), hipMemcpyHostToDevice, c.stream));
+  MT_HIP(hipMemcpyAsync(c.mk_band, mb.data(), 4 * ne, hipMemcpyHostToDevice, c.stream));
   MT_HIP(hipStreamSynchronize(c.stream));
   return MT_OK;
 }

@@ -348,8 +348,10 @@ struct Case {
   // cells' moments (taken from fcol), the marker force density is
   // F_k = dV_k rho_k (ub - u_k), each cell receives the momentum
   // dm(x) = sum_k w_k(x) F_k and its populations shift by the exact-difference
-  // term feq(rho, u + dm/rho) - feq(rho, u) (Kupershtokh's method).  The body
-  // takes -sum_k F_k (logged like a treat() force).
+  // term feq(rho, u + dm/rho) - feq(rho, u) (Kupershtokh's method).  The force
+  // logged like a treat() force is sum_k F_k: the body's reaction -sum_k F_k
+  // expressed, like the link-wise force, with the reported velocities' sign
+  // flipped back (the populations move along -e_q).
   void marker_correct() {
     if (!n_mk) return;
     Level& l = lv[Lmax];
@@ -395,9 +397,9 @@ struct Case {
     }
     double fx = 0, fy = 0, fz = 0;
     for (int64_t m = 0; m < n_mk; ++m) {
-      fx -= F[m][0];
-      fy -= F[m][1];
-      fz -= F[m][2];
+      fx += F[m][0];
+      fy += F[m][1];
+      fz += F[m][2];
     }
     force_log.push_back({fx, fy, fz});
   }

@@ -89,9 +89,10 @@ def gain(o, before):
 
 def test_single_marker_force_in_linear_flow():
     """One marker in a linear velocity field: the force is dV rho (ub - u(X))
-    exactly (the kernel interpolates linear fields), the body takes its
-    negative, the fluid gains its momentum and keeps its mass, and only the
-    marker's 27 cells change."""
+    exactly (the kernel interpolates linear fields) and is the logged force (the
+    body's reaction with the reported velocities' sign flipped back), the fluid
+    gains its momentum and keeps its mass, and only the marker's 27 cells
+    change."""
     X = np.array([5.3, 6.2, 5.8])
     ub = np.array([0.01, -0.02, 0.015])
     dv, rho = 0.7, 1.03
@@ -105,7 +106,7 @@ def test_single_marker_force_in_linear_flow():
     F_expect = dv * rho * (ub - (a + B @ X))
     log = o.force_log()
     assert log.shape == (1, 3)
-    assert np.abs(log[0] + F_expect).max() < 1e-16
+    assert np.abs(log[0] - F_expect).max() < 1e-16
     dm, dp = gain(o, before)
     # rounding of 27 x 19 differences of O(0.1) values; a lost term is O(1e-3)
     assert abs(dm) < 1e-14
@@ -146,7 +147,7 @@ def test_overlapping_markers_superpose():
     F = sum(case["markers"]["dv"][m] * 1.0 * (0 - (a + B @ X[m])) for m in range(2))
     _, dp = gain(o, before)
     assert np.abs(dp - F).max() < 1e-14
-    assert np.abs(o.force_log()[0] + F).max() < 1e-16
+    assert np.abs(o.force_log()[0] - F).max() < 1e-16
 
 
 @pytest.mark.parametrize("steps", [2])
@@ -159,5 +160,5 @@ def test_sphere_markers_oracle_runs(steps):
     log = o.force_log()
     assert log.shape == (4 * steps, 3)
     assert np.isfinite(log).all()
-    # the inflow moves along -x in the reported frame: the body is pushed along it
-    assert np.sign(log[-1, 0]) == np.sign(c["cd"]["U"])
+    # a drag along the physical flow (+x) is positive, as for the link-wise force
+    assert log[-1, 0] > 0
