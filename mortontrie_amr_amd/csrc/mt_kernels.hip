@@ -389,46 +389,70 @@ __device__ __forceinline__ void ib_links_cell(R* f, const Args& a, const int* N,
   const R rho_b = ld_real<R>(a.v.rho_in, s);
   const uint32_t wm = a.wmask[s];
   R wxx = 0, wyy = 0, wzz = 0, wxy = 0, wxz = 0, wyz = 0, vx = 0, vy = 0, vz = 0;
-  static_for<1, Q>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    constexpr int qi = qinv(q);
-    if ((wm >> q) & 1) {
-      const double dist = drow[qi - 1];
-      const R finc = load_pop<Q, S, R>(fin, ns, qi, s);
-      R fs = finc;
-      if ((pm >> qi) & 1) fs = load_pop<Q, S, R>(fin, ns, q, nbr_slot_c<-kEX[q], -kEY[q], -kEZ[q]>(N, lx, ly, lz));
-      const R d = (R)dist;
-      R fn;
-      if (a.ib_method == MT_IB_FH) {
-        if (d < (R)0.5) {
-          fn = ((R)1 - (R)2 * d) * finc + (R)2 * d * fs;
-        } else {
-          // D3Q19 weight of inv(q) (|e_inv(q)| = |e_q|), zero beyond q = 18
-          constexpr double w = qi < 19 ? (qnorm1(q) == 1 ? 1. / 18. : 1. / 36.) : 0.;
-          const R i2d = (R)1 / ((R)2 * d);
-          fn = i2d * fs + ((R)1 - i2d) * ((R)w * rho_b);
+  // Wall links in chunks of CH directions: the chunk's loads (fraction,
+  // fcol_inv(q)(x), fcol_q(x - e_q)) are issued together, then used, so a cell
+  // waits for memory once per chunk instead of once per wall link.
+  constexpr int CH = 6;
+  static_for<0, (Q - 1 + CH - 1) / CH>([&](auto cc) {
+    constexpr int c0 = 1 + decltype(cc)::value * CH;
+    double dist_c[CH];
+    R finc_c[CH], fs_c[CH];
+    static_for<0, CH>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      constexpr int q = c0 + j;
+      if constexpr (q < Q) {
+        constexpr int qi = qinv(q);
+        if ((wm >> q) & 1) {
+          dist_c[j] = drow[qi - 1];
+          finc_c[j] = load_pop<Q, S, R>(fin, ns, qi, s);
+          fs_c[j] = ((pm >> qi) & 1)
+                        ? load_pop<Q, S, R>(fin, ns, q, nbr_slot_c<-kEX[q], -kEY[q], -kEZ[q]>(N, lx, ly, lz))
+                        : finc_c[j];
         }
-      } else {
-        fn = ((R)1 - d) * finc + d * fs;
       }
-      f[q] = fn;
-      const double m = (double)finc + (double)fn;
-      fx -= m * kEX[q];
-      fy -= m * kEY[q];
-      fz -= m * kEZ[q];
-      if (q < 19) {
-        constexpr double w = qnorm1(q) == 1 ? 1. / 18. : 1. / 36.;
-        wxx += (R)(w * kEX[q] * kEX[q]);
-        wyy += (R)(w * kEY[q] * kEY[q]);
-        wzz += (R)(w * kEZ[q] * kEZ[q]);
-        wxy += (R)(w * kEX[q] * kEY[q]);
-        wxz += (R)(w * kEX[q] * kEZ[q]);
-        wyz += (R)(w * kEY[q] * kEZ[q]);
-        vx += (R)(w * kEX[q]);
-        vy += (R)(w * kEY[q]);
-        vz += (R)(w * kEZ[q]);
+    });
+    static_for<0, CH>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      constexpr int q = c0 + j;
+      if constexpr (q < Q) {
+        constexpr int qi = qinv(q);
+        if ((wm >> q) & 1) {
+          const R finc = finc_c[j];
+          const R fs = fs_c[j];
+          const R d = (R)dist_c[j];
+          R fn;
+          if (a.ib_method == MT_IB_FH) {
+            if (d < (R)0.5) {
+              fn = ((R)1 - (R)2 * d) * finc + (R)2 * d * fs;
+            } else {
+              // D3Q19 weight of inv(q) (|e_inv(q)| = |e_q|), zero beyond q = 18
+              constexpr double w = qi < 19 ? (qnorm1(q) == 1 ? 1. / 18. : 1. / 36.) : 0.;
+              const R i2d = (R)1 / ((R)2 * d);
+              fn = i2d * fs + ((R)1 - i2d) * ((R)w * rho_b);
+            }
+          } else {
+            fn = ((R)1 - d) * finc + d * fs;
+          }
+          f[q] = fn;
+          const double m = (double)finc + (double)fn;
+          fx -= m * kEX[q];
+          fy -= m * kEY[q];
+          fz -= m * kEZ[q];
+          if (q < 19) {
+            constexpr double w = qnorm1(q) == 1 ? 1. / 18. : 1. / 36.;
+            wxx += (R)(w * kEX[q] * kEX[q]);
+            wyy += (R)(w * kEY[q] * kEY[q]);
+            wzz += (R)(w * kEZ[q] * kEZ[q]);
+            wxy += (R)(w * kEX[q] * kEY[q]);
+            wxz += (R)(w * kEX[q] * kEZ[q]);
+            wyz += (R)(w * kEY[q] * kEZ[q]);
+            vx += (R)(w * kEX[q]);
+            vy += (R)(w * kEY[q]);
+            vz += (R)(w * kEZ[q]);
+          }
+        }
       }
-    }
+    });
   });
   // density clamp through f_0 (ComplexBoundary_Manager.cpp:150-158)
   R ra = 0, m0 = 0, m1 = 0, m2 = 0;
@@ -481,7 +505,7 @@ __device__ __forceinline__ void ib_links_cell(R* f, const Args& a, const int* N,
 // shuffles; the group's first lane rescales the average around its He-Luo
 // equilibrium and runs the F2C cell's calculateMacros + collide, which the
 // reference does at the start of the coarse level's next sub-step.
-template <int Q, typename S, bool COAL>
+template <int Q, typename S, bool COAL, bool WET, bool IB>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_fused(FusedArgs a) {
   typedef typename RealOf<S>::type R;
   const int lane = threadIdx.x;
@@ -504,18 +528,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   }
   double fx = 0, fy = 0, fz = 0;  // momentum exchange of this lane's treated cell
   const int s = (b << 6) | lane;
+  // the cell's tables, loaded together up front (one memory round trip before
+  // the pull instead of one per table)
   const uint16_t k = a.v.kind[s];
+  const uint32_t pmx = a.v.pmask[s];
+  const uint32_t tm = a.tmask ? a.tmask[s] : 0u;
+  const int row0 = IB ? a.dis_row[s] : -1;
   {
     int N[27];
 #pragma unroll
     for (int o = 0; o < 27; ++o) N[o] = __builtin_amdgcn_readfirstlane(a.v.bnbr[(int64_t)b * 27 + o]);
-    const int row = (a.do_ib && (k & MT_KIND_DIS) && (k & MT_KIND_FLUID)) ? a.dis_row[s] : -1;
+    // keep the cell's loads together with the brick's: the compiler would
+    // otherwise sink each into the branch that first uses it, one memory round
+    // trip after another
+    asm volatile("" ::"v"(k), "v"(pmx), "v"(tm), "v"(row0), "s"(p_tau), "s"(p_les));
+    const int row = (IB && (k & MT_KIND_DIS) && (k & MT_KIND_FLUID)) ? row0 : -1;
     const bool treated = row >= 0;
     if (k & (MT_KIND_FLUID | MT_KIND_C2F)) {
       int lx, ly, lz;
       brick_coords(lane, lx, ly, lz);
-      const uint32_t pmx = a.v.pmask[s];
-      const uint32_t tm = a.tmask ? a.tmask[s] : 0u;
       const S* xb = (const S*)a.xbuf;
       f[0] = load_pop<Q, S, R>((tm & 1u) ? xb : fin, ns, 0, s);
       // pull f_q(x) <- fcol_q(x + e_q); a missing source keeps the cell's own f_q
@@ -539,7 +570,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 #pragma unroll
         for (int q = 0; q < Q; ++q) f[q] = snap_pop<Q, S, R>(q, f[q]);
       }
-      if (a.do_bc && (k & 0x3F00)) {
+      if (WET && (k & 0x3F00)) {
         const int32_t* bn = a.v.bnbr;
         const uint32_t* pmk = a.v.pmask;
         auto nf = [&](int dx, int dy, int dz, int q) -> R {
@@ -596,7 +627,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
       }
     }
   }
-  if (a.do_ib) {
+  if (IB) {
     // this call's force: one running slot per brick, fixed block -> brick map
     for (int off = 32; off > 0; off >>= 1) {
       fx += __shfl_down(fx, off, 64);
@@ -1410,10 +1441,22 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros
     c.ev_pool.pop_back();
   }
   int r = MT_OK;
-  if (coal)
-    MT_DISPATCH(c, (r = launch_events(k_fused<Q_, S_, true>, nb, 64, c.st[L], a, e0, e1)));
-  else
-    MT_DISPATCH(c, (r = launch_events(k_fused<Q_, S_, false>, nb, 64, c.st[L], a, e0, e1)));
+  // the domain boundary (level 0 only), the coalescence (finer levels only) and
+  // the solid boundary (finest level only) are compiled into separate
+  // instantiations, so no level's kernel carries the registers of another's
+  if (coal && a.do_bc) {
+    set_error("coalescence and domain boundary in one fused update");
+    return MT_ERR_STATE;
+  }
+  const int variant = (coal ? 1 : 0) | (a.do_bc ? 2 : 0) | (a.do_ib ? 4 : 0);
+  switch (variant) {
+    case 0: MT_DISPATCH(c, (r = launch_events(k_fused<Q_, S_, false, false, false>, nb, 64, c.st[L], a, e0, e1))); break;
+    case 1: MT_DISPATCH(c, (r = launch_events(k_fused<Q_, S_, true, false, false>, nb, 64, c.st[L], a, e0, e1))); break;
+    case 2: MT_DISPATCH(c, (r = launch_events(k_fused<Q_, S_, false, true, false>, nb, 64, c.st[L], a, e0, e1))); break;
+    case 4: MT_DISPATCH(c, (r = launch_events(k_fused<Q_, S_, false, false, true>, nb, 64, c.st[L], a, e0, e1))); break;
+    case 5: MT_DISPATCH(c, (r = launch_events(k_fused<Q_, S_, true, false, true>, nb, 64, c.st[L], a, e0, e1))); break;
+    default: MT_DISPATCH(c, (r = launch_events(k_fused<Q_, S_, false, true, true>, nb, 64, c.st[L], a, e0, e1))); break;
+  }
   if (r) return r;
   MT_HIP(hipGetLastError());
   if (c.timing) c.ev_pending.push_back({L, {e0, e1}});
