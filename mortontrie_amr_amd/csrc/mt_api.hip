@@ -1,7 +1,10 @@
 // mt_api.hip -- the C-ABI (include/mt_amr.h) and the coarse-step schedule.
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "mt_case.h"
@@ -10,6 +13,104 @@ namespace mt {
 
 static thread_local std::string g_err;
 void set_error(const std::string& s) { g_err = s; }
+
+// ---------------------------------------------------------------- caches
+namespace {
+struct Pool {
+  std::mutex mu;
+  // cached blocks by size; a block carries the stream it was freed on
+  // (nullptr: reusable by anyone)
+  std::multimap<size_t, std::pair<void*, hipStream_t>> free_blocks;
+  std::unordered_map<void*, size_t> size_of;
+  std::vector<hipStream_t> streams;
+};
+Pool& pool() {
+  static Pool* p = new Pool();  // lives until the process ends
+  return *p;
+}
+size_t block_size(size_t n) {
+  if (n == 0) n = 1;
+  if (n <= (size_t(1) << 20)) return (n + 4095) & ~size_t(4095);
+  return (n + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1);
+}
+void release_cached_locked(Pool& P) {
+  for (auto& b : P.free_blocks) {
+    P.size_of.erase(b.second.first);
+    hipFree(b.second.first);
+  }
+  P.free_blocks.clear();
+}
+}  // namespace
+
+hipError_t dev_malloc(void** p, size_t bytes, hipStream_t s) {
+  Pool& P = pool();
+  const size_t n = block_size(bytes);
+  {
+    std::lock_guard<std::mutex> g(P.mu);
+    for (auto it = P.free_blocks.lower_bound(n); it != P.free_blocks.end() && it->first <= n + n / 8; ++it) {
+      if (it->second.second == nullptr || it->second.second == s) {
+        *p = it->second.first;
+        P.free_blocks.erase(it);
+        return hipSuccess;
+      }
+    }
+  }
+  hipError_t e = hipMalloc(p, n);
+  if (e == hipErrorOutOfMemory) {
+    // give the cached blocks back and try once more
+    hipDeviceSynchronize();
+    std::lock_guard<std::mutex> g(P.mu);
+    release_cached_locked(P);
+    (void)hipGetLastError();
+    e = hipMalloc(p, n);
+  }
+  if (e == hipSuccess) {
+    std::lock_guard<std::mutex> g(P.mu);
+    P.size_of[*p] = n;
+  }
+  return e;
+}
+
+void dev_free(void* p, hipStream_t s) {
+  if (!p) return;
+  Pool& P = pool();
+  std::lock_guard<std::mutex> g(P.mu);
+  auto it = P.size_of.find(p);
+  if (it == P.size_of.end()) {
+    hipFree(p);
+    return;
+  }
+  P.free_blocks.emplace(it->second, std::make_pair(p, s));
+}
+
+void pool_quiesce(hipStream_t s) {
+  Pool& P = pool();
+  std::lock_guard<std::mutex> g(P.mu);
+  for (auto& b : P.free_blocks)
+    if (b.second.second == s) b.second.second = nullptr;
+}
+
+hipError_t stream_acquire(hipStream_t* s) {
+  Pool& P = pool();
+  {
+    std::lock_guard<std::mutex> g(P.mu);
+    if (!P.streams.empty()) {
+      *s = P.streams.back();
+      P.streams.pop_back();
+      return hipSuccess;
+    }
+  }
+  return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+
+// the stream must have drained
+void stream_release(hipStream_t s) {
+  if (!s) return;
+  pool_quiesce(s);
+  Pool& P = pool();
+  std::lock_guard<std::mutex> g(P.mu);
+  P.streams.push_back(s);
+}
 
 #define CHECK_CASE(c)                                   \
   do {                                                  \
@@ -227,8 +328,8 @@ int mt_case_create(const mt_case_desc* d, mt_case** out) {
   m.les_C = d->les_C;
   m.rho0 = d->rho0;
   c->h_mdesc.assign(1, m);
-  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  for (int L = 1; L <= c->Lmax && e == hipSuccess; ++L) e = hipStreamCreateWithFlags(&c->lstream[L], hipStreamNonBlocking);
+  hipError_t e = stream_acquire(&c->stream);
+  for (int L = 1; L <= c->Lmax && e == hipSuccess; ++L) e = stream_acquire(&c->lstream[L]);
   for (int L = 0; L <= c->Lmax && e == hipSuccess; ++L) {
     e = hipEventCreateWithFlags(&c->ev_ready[L], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_done[L], hipEventDisableTiming);
@@ -246,20 +347,23 @@ int mt_case_create(const mt_case_desc* d, mt_case** out) {
 
 int mt_case_destroy(mt_case* c) {
   if (!c) return MT_OK;
+  // every stream of the case drains before its memory and streams are reused
   if (c->stream) hipStreamSynchronize(c->stream);
+  for (int L = 0; L < kMaxLevels; ++L)
+    if (c->lstream[L]) hipStreamSynchronize(c->lstream[L]);
   for (auto& g : c->graph_exec) hipGraphExecDestroy(g.second);
   for (auto e : c->ev_pool) hipEventDestroy(e);
   for (auto& p : c->ev_pending) {
     hipEventDestroy(p.second.first);
     hipEventDestroy(p.second.second);
   }
-  for (void* p : c->allocs) hipFree(p);
+  for (void* p : c->allocs) dev_free(p, nullptr);
   for (int L = 0; L < kMaxLevels; ++L) {
     if (c->ev_ready[L]) hipEventDestroy(c->ev_ready[L]);
     if (c->ev_done[L]) hipEventDestroy(c->ev_done[L]);
-    if (c->lstream[L]) hipStreamDestroy(c->lstream[L]);
+    stream_release(c->lstream[L]);
   }
-  if (c->stream) hipStreamDestroy(c->stream);
+  stream_release(c->stream);
   delete c;
   return MT_OK;
 }
@@ -408,8 +512,8 @@ int mt_find(mt_case* c, int level, const uint64_t* keys, int64_t n, int32_t* slo
   if (n <= 0) return MT_OK;
   uint64_t* dk = nullptr;
   int32_t* ds = nullptr;
-  MT_HIP(hipMalloc((void**)&dk, 8 * n));
-  MT_HIP(hipMalloc((void**)&ds, 4 * n));
+  MT_HIP(dev_malloc((void**)&dk, 8 * n, c->stream));
+  MT_HIP(dev_malloc((void**)&ds, 4 * n, c->stream));
   MT_HIP(hipMemcpyAsync(dk, keys, 8 * n, hipMemcpyHostToDevice, c->stream));
   int r = find_slots(*c, level, dk, n, ds);
   if (r == MT_OK) {
@@ -418,8 +522,8 @@ int mt_find(mt_case* c, int level, const uint64_t* keys, int64_t n, int32_t* slo
     if (e != hipSuccess) { set_error(hipGetErrorString(e)); r = MT_ERR_HIP; }
   }
   hipStreamSynchronize(c->stream);
-  hipFree(dk);
-  hipFree(ds);
+  dev_free(dk, c->stream);
+  dev_free(ds, c->stream);
   return r;
 }
 
@@ -433,9 +537,9 @@ static int field_io(mt_case* c, int level, int field, const uint64_t* keys, int6
   uint64_t* dk = nullptr;
   int32_t* ds = nullptr;
   double* dv = nullptr;
-  MT_HIP(hipMalloc((void**)&dk, 8 * n));
-  MT_HIP(hipMalloc((void**)&ds, 4 * n));
-  MT_HIP(hipMalloc((void**)&dv, 8 * n * rows));
+  MT_HIP(dev_malloc((void**)&dk, 8 * n, c->stream));
+  MT_HIP(dev_malloc((void**)&ds, 4 * n, c->stream));
+  MT_HIP(dev_malloc((void**)&dv, 8 * n * rows, c->stream));
   MT_HIP(hipMemcpyAsync(dk, keys, 8 * n, hipMemcpyHostToDevice, c->stream));
   int r = find_slots(*c, level, dk, n, ds);
   std::vector<int32_t> hs(n);
@@ -481,9 +585,9 @@ static int field_io(mt_case* c, int level, int field, const uint64_t* keys, int6
     }
   }
   hipStreamSynchronize(c->stream);
-  hipFree(dk);
-  hipFree(ds);
-  hipFree(dv);
+  dev_free(dk, c->stream);
+  dev_free(ds, c->stream);
+  dev_free(dv, c->stream);
   return r;
 }
 

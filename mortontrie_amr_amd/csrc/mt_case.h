@@ -171,14 +171,29 @@ struct Case {
   size_t real_bytes() const { return prec == MT_FP64 ? 8 : 4; }
 
   template <typename T>
-  T* alloc(int64_t n) {
-    void* p = nullptr;
-    if (n <= 0) n = 1;
-    if (hipMalloc(&p, sizeof(T) * n) != hipSuccess) return nullptr;
-    allocs.push_back(p);
-    return (T*)p;
-  }
+  T* alloc(int64_t n);
 };
+
+// Process-wide caches of device memory and streams (mt_api.hip).  A serving
+// loop builds and destroys cases continually, and hipMalloc / hipFree /
+// hipStreamCreate cost from tens of microseconds to milliseconds per call.
+// A block freed with a stream is reused only by allocations on that stream
+// (stream order) until pool_quiesce(stream), called once the stream has
+// drained; a block freed with no stream is reusable at once.
+hipError_t dev_malloc(void** p, size_t bytes, hipStream_t s);
+void dev_free(void* p, hipStream_t s);
+void pool_quiesce(hipStream_t s);
+hipError_t stream_acquire(hipStream_t* s);
+void stream_release(hipStream_t s);
+
+template <typename T>
+T* Case::alloc(int64_t n) {
+  void* p = nullptr;
+  if (n <= 0) n = 1;
+  if (dev_malloc(&p, sizeof(T) * n, nullptr) != hipSuccess) return nullptr;
+  allocs.push_back(p);
+  return (T*)p;
+}
 
 void set_error(const std::string& s);
 #define MT_HIP(x)                                                                   \

@@ -1601,7 +1601,7 @@ int build_markers(Case& c) {
   const int64_t n = c.n_mk, ne = 27 * n;
   DevLevel& l = c.lv[c.Lmax];
   uint64_t* dk = nullptr;
-  MT_HIP(hipMalloc((void**)&dk, 8 * ne));
+  MT_HIP(dev_malloc((void**)&dk, 8 * ne, c.stream));
   c.mk_slot = c.alloc<int32_t>(ne);
   c.mk_w = c.alloc<double>(ne);
   c.mk_dv = c.alloc<double>(n);
@@ -1620,7 +1620,7 @@ int build_markers(Case& c) {
   MT_HIP(hipMemcpyAsync(hs.data(), c.mk_slot, 4 * ne, hipMemcpyDeviceToHost, c.stream));
   MT_HIP(hipMemcpyAsync(kind.data(), l.kind, 2 * l.n_slots, hipMemcpyDeviceToHost, c.stream));
   MT_HIP(hipStreamSynchronize(c.stream));
-  MT_HIP(hipFree(dk));
+  dev_free(dk, c.stream);
   struct E { int32_t slot, mk; double w; };
   std::vector<E> es;
   for (int64_t m = 0; m < n; ++m)

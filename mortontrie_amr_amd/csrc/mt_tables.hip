@@ -106,7 +106,7 @@ static int scan_exclusive(Case& c, const int* d_in, int* d_out, int64_t n, int64
   if (nb < 1) nb = 1;
   int* bsum = nullptr;
   int* tot = nullptr;
-  MT_HIP(hipMalloc((void**)&bsum, sizeof(int) * (nb + 1)));
+  MT_HIP(dev_malloc((void**)&bsum, sizeof(int) * (nb + 1), c.stream));
   tot = bsum + nb;
   k_scan_blocksum<<<nb, 256, 0, c.stream>>>(d_in, n, bsum);
   k_scan_bsum<<<1, 256, 0, c.stream>>>(bsum, nb, tot);
@@ -114,7 +114,7 @@ static int scan_exclusive(Case& c, const int* d_in, int* d_out, int64_t n, int64
   int t = 0;
   MT_HIP(hipMemcpyAsync(&t, tot, sizeof(int), hipMemcpyDeviceToHost, c.stream));
   MT_HIP(hipStreamSynchronize(c.stream));
-  MT_HIP(hipFree(bsum));
+  dev_free(bsum, c.stream);
   *h_total = t;
   return MT_OK;
 }
@@ -165,11 +165,11 @@ static int radix_sort(Case& c, const uint64_t* d_keys, int64_t n, uint64_t max_k
   int* hist = nullptr;
   int64_t n_tiles = (n + kRsTile - 1) / kRsTile;
   if (n_tiles < 1) n_tiles = 1;
-  MT_HIP(hipMalloc((void**)&ka, 8 * (n + 1)));
-  MT_HIP(hipMalloc((void**)&kb, 8 * (n + 1)));
-  MT_HIP(hipMalloc((void**)&va, 4 * (n + 1)));
-  MT_HIP(hipMalloc((void**)&vb, 4 * (n + 1)));
-  MT_HIP(hipMalloc((void**)&hist, sizeof(int) * 256 * n_tiles));
+  MT_HIP(dev_malloc((void**)&ka, 8 * (n + 1), c.stream));
+  MT_HIP(dev_malloc((void**)&kb, 8 * (n + 1), c.stream));
+  MT_HIP(dev_malloc((void**)&va, 4 * (n + 1), c.stream));
+  MT_HIP(dev_malloc((void**)&vb, 4 * (n + 1), c.stream));
+  MT_HIP(dev_malloc((void**)&hist, sizeof(int) * 256 * n_tiles, c.stream));
   MT_HIP(hipMemcpyAsync(ka, d_keys, 8 * n, hipMemcpyDeviceToDevice, c.stream));
   k_iota<<<(n + 255) / 256 + 1, 256, 0, c.stream>>>(va, n);
   int bits = 0;
@@ -186,9 +186,9 @@ static int radix_sort(Case& c, const uint64_t* d_keys, int64_t n, uint64_t max_k
     std::swap(va, vb);
   }
   MT_HIP(hipStreamSynchronize(c.stream));
-  MT_HIP(hipFree(kb));
-  MT_HIP(hipFree(vb));
-  MT_HIP(hipFree(hist));
+  dev_free(kb, c.stream);
+  dev_free(vb, c.stream);
+  dev_free(hist, c.stream);
   *ks = ka;
   *vs = va;
   return MT_OK;
@@ -499,7 +499,7 @@ static int copy_rows(Case& dst, DevLevel& ld, Case& src, DevLevel& ls, int32_t s
     hd.push_back((T*)r.second);
   }
   T** dev = nullptr;
-  MT_HIP(hipMalloc((void**)&dev, 2 * sizeof(T*) * rows.size()));
+  MT_HIP(dev_malloc((void**)&dev, 2 * sizeof(T*) * rows.size(), dst.stream));
   MT_HIP(hipMemcpyAsync(dev, hs.data(), sizeof(T*) * rows.size(), hipMemcpyHostToDevice, dst.stream));
   MT_HIP(hipMemcpyAsync(dev + rows.size(), hd.data(), sizeof(T*) * rows.size(), hipMemcpyHostToDevice, dst.stream));
   k_copy_member<T><<<ls.n_bricks, 64, 0, dst.stream>>>(ls.bcode, ls.bmember, sm, ld.bcode, ld.n_bricks, d[0], d[1], d[2],
@@ -507,7 +507,7 @@ static int copy_rows(Case& dst, DevLevel& ld, Case& src, DevLevel& ls, int32_t s
                                                        d_err);
   MT_HIP(hipGetLastError());
   MT_HIP(hipStreamSynchronize(dst.stream));
-  MT_HIP(hipFree(dev));
+  dev_free(dev, dst.stream);
   return MT_OK;
 }
 
@@ -521,7 +521,7 @@ int copy_member(Case& dst, int32_t dm, Case& src, int32_t sm) {
   const MemberParams& PS = src.h_mem[sm];
   const MemberParams& PD = dst.h_mem[dm];
   int* d_err = nullptr;
-  MT_HIP(hipMalloc((void**)&d_err, 4));
+  MT_HIP(dev_malloc((void**)&d_err, 4, dst.stream));
   MT_HIP(hipMemsetAsync(d_err, 0, 4, dst.stream));
   MT_HIP(hipStreamSynchronize(src.stream));
   int r = MT_OK;
@@ -573,7 +573,7 @@ int copy_member(Case& dst, int32_t dm, Case& src, int32_t sm) {
       r = MT_ERR_ARG;
     }
   }
-  hipFree(d_err);
+  dev_free(d_err, dst.stream);
   return r;
 }
 
@@ -669,7 +669,7 @@ static int order_active(Case& c, int L) {
 
 static int compact_list(Case& c, const int* d_flag, int64_t n, int32_t** out, int64_t* count) {
   int* pos = nullptr;
-  MT_HIP(hipMalloc((void**)&pos, sizeof(int) * (n + 1)));
+  MT_HIP(dev_malloc((void**)&pos, sizeof(int) * (n + 1), c.stream));
   int64_t tot;
   int r = scan_exclusive(c, d_flag, pos, n, &tot);
   if (r) return r;
@@ -677,7 +677,7 @@ static int compact_list(Case& c, const int* d_flag, int64_t n, int32_t** out, in
   if (!*out) { set_error("hipMalloc failed"); return MT_ERR_HIP; }
   k_emit<<<nblk(n), 256, 0, c.stream>>>(d_flag, pos, n, *out);
   MT_HIP(hipStreamSynchronize(c.stream));
-  MT_HIP(hipFree(pos));
+  dev_free(pos, c.stream);
   *count = tot;
   return MT_OK;
 }
@@ -709,8 +709,8 @@ int build_tables(Case& c) {
     uint64_t *d_keys = nullptr, *ks = nullptr;
     uint16_t* d_kinds = nullptr;
     int32_t* vs = nullptr;
-    MT_HIP(hipMalloc((void**)&d_keys, 8 * n));
-    MT_HIP(hipMalloc((void**)&d_kinds, 2 * n));
+    MT_HIP(dev_malloc((void**)&d_keys, 8 * n, c.stream));
+    MT_HIP(dev_malloc((void**)&d_kinds, 2 * n, c.stream));
     MT_HIP(hipMemcpyAsync(d_keys, l.h_keys.data(), 8 * n, hipMemcpyHostToDevice, c.stream));
     MT_HIP(hipMemcpyAsync(d_kinds, l.h_kinds.data(), 2 * n, hipMemcpyHostToDevice, c.stream));
     uint64_t mx = *std::max_element(l.h_keys.begin(), l.h_keys.end());
@@ -718,8 +718,8 @@ int build_tables(Case& c) {
     if (r) return r;
     int shift = 3 * (c.Lmax - L);
     int *head = nullptr, *bidx = nullptr;
-    MT_HIP(hipMalloc((void**)&head, 4 * n));
-    MT_HIP(hipMalloc((void**)&bidx, 4 * n));
+    MT_HIP(dev_malloc((void**)&head, 4 * n, c.stream));
+    MT_HIP(dev_malloc((void**)&bidx, 4 * n, c.stream));
     k_codes<<<nblk(n), 256, 0, c.stream>>>(ks, n, shift, head, d_err);
     int64_t nb;
     r = scan_exclusive(c, head, bidx, n, &nb);
@@ -758,12 +758,12 @@ int build_tables(Case& c) {
     k_pmask<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.bnbr, l.kind, l.n_slots, c.Q, l.pmask);
     MT_HIP(hipGetLastError());
     MT_HIP(hipStreamSynchronize(c.stream));
-    MT_HIP(hipFree(d_keys));
-    MT_HIP(hipFree(d_kinds));
-    MT_HIP(hipFree(ks));
-    MT_HIP(hipFree(vs));
-    MT_HIP(hipFree(head));
-    MT_HIP(hipFree(bidx));
+    dev_free(d_keys, c.stream);
+    dev_free(d_kinds, c.stream);
+    dev_free(ks, c.stream);
+    dev_free(vs, c.stream);
+    dev_free(head, c.stream);
+    dev_free(bidx, c.stream);
   }
   // dis (finest level)
   int Lm = c.Lmax;
@@ -775,8 +775,8 @@ int build_tables(Case& c) {
     uint64_t *d_k = nullptr, *ks = nullptr;
     int32_t* vs = nullptr;
     double* d_v = nullptr;
-    MT_HIP(hipMalloc((void**)&d_k, 8 * nd));
-    MT_HIP(hipMalloc((void**)&d_v, 8 * nd * m));
+    MT_HIP(dev_malloc((void**)&d_k, 8 * nd, c.stream));
+    MT_HIP(dev_malloc((void**)&d_v, 8 * nd * m, c.stream));
     MT_HIP(hipMemcpyAsync(d_k, c.h_dis_keys.data(), 8 * nd, hipMemcpyHostToDevice, c.stream));
     MT_HIP(hipMemcpyAsync(d_v, c.h_dis_vals.data(), 8 * nd * m, hipMemcpyHostToDevice, c.stream));
     uint64_t mx = *std::max_element(c.h_dis_keys.begin(), c.h_dis_keys.end());
@@ -794,17 +794,17 @@ int build_tables(Case& c) {
     d_dis_sorted = c.alloc<uint64_t>(nd);
     MT_HIP(hipMemcpyAsync(d_dis_sorted, ks, 8 * nd, hipMemcpyDeviceToDevice, c.stream));
     MT_HIP(hipStreamSynchronize(c.stream));
-    MT_HIP(hipFree(d_k));
-    MT_HIP(hipFree(d_v));
-    MT_HIP(hipFree(ks));
-    MT_HIP(hipFree(vs));
+    dev_free(d_k, c.stream);
+    dev_free(d_v, c.stream);
+    dev_free(ks, c.stream);
+    dev_free(vs, c.stream);
   }
   // lists
   for (int L = 0; L <= c.Lmax; ++L) {
     DevLevel& l = c.lv[L];
     int* flag = nullptr;
     int64_t nmax = std::max(l.n_slots, l.n_bricks);
-    MT_HIP(hipMalloc((void**)&flag, 4 * (nmax + 1)));
+    MT_HIP(dev_malloc((void**)&flag, 4 * (nmax + 1), c.stream));
     k_flag_brick<<<nblk(l.n_bricks), 256, 0, c.stream>>>(l.kind, l.n_bricks, MT_KIND_FLUID | MT_KIND_C2F, flag);
     int r = compact_list(c, flag, l.n_bricks, &l.active, &l.n_active);
     if (r) return r;
@@ -823,7 +823,7 @@ int build_tables(Case& c) {
       if (r) return r;
     }
     MT_HIP(hipStreamSynchronize(c.stream));
-    MT_HIP(hipFree(flag));
+    dev_free(flag, c.stream);
     if (L < c.Lmax && (l.n_f2c || l.n_gf2c)) {
       l.f2c_child = c.alloc<int32_t>(l.n_f2c * 8);
       l.f2c_child_write = c.alloc<int32_t>(l.n_f2c * 8);
@@ -868,7 +868,7 @@ int build_tables(Case& c) {
         return MT_ERR_HIP;
       }
       int* d_bad = nullptr;
-      MT_HIP(hipMalloc((void**)&d_bad, 4));
+      MT_HIP(dev_malloc((void**)&d_bad, 4, c.stream));
       MT_HIP(hipMemsetAsync(d_bad, 0, 4, c.stream));
       k_fill_i32<<<nblk(lf.n_slots / 8), 256, 0, c.stream>>>(lf.cpar, lf.n_slots / 8, -1);
       k_cpar<<<nblk(l.n_f2c), 256, 0, c.stream>>>(l.f2c, l.f2c_child, l.n_f2c, lf.kind, lf.dis_row, lf.cpar, d_bad);
@@ -876,7 +876,7 @@ int build_tables(Case& c) {
       int hbad = 1;
       MT_HIP(hipMemcpyAsync(&hbad, d_bad, 4, hipMemcpyDeviceToHost, c.stream));
       MT_HIP(hipStreamSynchronize(c.stream));
-      MT_HIP(hipFree(d_bad));
+      dev_free(d_bad, c.stream);
       l.coal_in_fused = hbad == 0;
     }
     // explosions inside the fused updates: needs the in-kernel coalescence on
@@ -886,13 +886,13 @@ int build_tables(Case& c) {
     DevLevel& lm = c.lv[c.Lmax];
     if (xin && lm.n_ibl) {
       int* d_hit = nullptr;
-      MT_HIP(hipMalloc((void**)&d_hit, 4 * lm.n_ibl));
+      MT_HIP(dev_malloc((void**)&d_hit, 4 * lm.n_ibl, c.stream));
       k_ib_touch<<<nblk(lm.n_ibl), 256, 0, c.stream>>>(lm.ibl, lm.n_ibl, lm.bnbr, lm.kind, d_hit);
       MT_HIP(hipGetLastError());
       std::vector<int> h(lm.n_ibl);
       MT_HIP(hipMemcpyAsync(h.data(), d_hit, 4 * lm.n_ibl, hipMemcpyDeviceToHost, c.stream));
       MT_HIP(hipStreamSynchronize(c.stream));
-      MT_HIP(hipFree(d_hit));
+      dev_free(d_hit, c.stream);
       for (int v : h) xin = xin && v == 0;
     }
     for (int L = 0; L <= c.Lmax && xin; ++L) {
@@ -900,12 +900,12 @@ int build_tables(Case& c) {
       if (L >= 1) {
         l.tmask = c.alloc<uint32_t>(l.n_slots);
         int* flag = nullptr;
-        MT_HIP(hipMalloc((void**)&flag, 4 * (l.n_slots + 1)));
+        MT_HIP(dev_malloc((void**)&flag, 4 * (l.n_slots + 1), c.stream));
         k_flag_target<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.kind, l.n_slots, flag);
         int r = compact_list(c, flag, l.n_slots, &l.xt_list, &l.n_xt);
         if (r) return r;
         MT_HIP(hipStreamSynchronize(c.stream));
-        MT_HIP(hipFree(flag));
+        dev_free(flag, c.stream);
         if (!l.tmask) {
           set_error("out of device memory");
           return MT_ERR_HIP;

@@ -4,12 +4,14 @@ Requests (case dictionaries with a solid boundary) arrive over time.  The
 server keeps every running request as a member of one packed case, so all of
 them share each level sub-step's launch.  It advances the packed case in
 chunks of `chunk` coarse steps; between chunks it
-  * retires requests that finished (max_steps reached, or the drag averaged
+  * marks requests that finished (max_steps reached, or the drag averaged
     over the last chunk moved by less than `tol` from the previous chunk's),
-  * admits arrived requests while their device footprint fits `hbm_budget`,
-  * repacks when membership changed, carrying each running member's state over
-    on the device (mt_case_copy_member), so a member computes exactly what it
-    would compute alone.
+  * every `admit_every` chunks (or at once when nothing is left running)
+    admits arrived requests while their device footprint fits `hbm_budget`
+    and repacks, dropping the finished members and carrying each running
+    member's state over on the device (mt_case_copy_member), so a member
+    computes exactly what it would compute alone.  Between repacks a finished
+    member keeps stepping with the batch; its result is already final.
 Latency is completion time minus arrival time, on the server's wall clock.
 """
 from __future__ import annotations
@@ -56,8 +58,9 @@ class Server:
     (each chunk takes that long, idle waits are skipped) for reproducible tests."""
 
     def __init__(self, precision=amr.FP32_SHIFTED, chunk=100, hbm_budget=200 << 30, chunk_seconds=None,
-                 slot_dims=(100, 48, 48)):
+                 slot_dims=(100, 48, 48), admit_every=1):
         self.precision, self.chunk, self.budget = precision, chunk, hbm_budget
+        self.admit_every = admit_every
         self.chunk_seconds = chunk_seconds
         self.vtime = 0.0
         # every running request keeps one slot (a fixed box along x) for its life,
@@ -113,43 +116,48 @@ class Server:
         """Serve `requests` (sorted by arrival, seconds from now); returns them finished."""
         t0 = time.perf_counter()
         pending = deque(sorted(requests, key=lambda r: r.arrival))
-        running, done = [], []
-        changed = False
-        while pending or running:
+        running, done = [], []     # running: the batch's members, finished or not
+        since = self.admit_every   # chunks since the last repack
+        while pending or any(r.done is None for r in running):
             now = self._now(t0)
-            used = sum(footprint_bytes(r.case, self.precision) for r in running)
-            while pending and pending[0].arrival <= now:
-                need = footprint_bytes(pending[0].case, self.precision)
-                if running and used + need > self.budget:
-                    break
-                r = pending.popleft()
-                r.admitted = now
-                running.append(r)
-                used += need
-                changed = True
+            active = [r for r in running if r.done is None]
+            arrived = bool(pending) and pending[0].arrival <= now
+            if (arrived and (since >= self.admit_every or not active)) or (running and not active):
+                used = sum(footprint_bytes(r.case, self.precision) for r in active)
+                admitted = []
+                while pending and pending[0].arrival <= now:
+                    need = footprint_bytes(pending[0].case, self.precision)
+                    if (active or admitted) and used + need > self.budget:
+                        break
+                    r = pending.popleft()
+                    r.admitted = now
+                    admitted.append(r)
+                    used += need
+                running = active + admitted
+                if running:
+                    self._repack(running)
+                elif self.batch is not None:
+                    self.batch.close()
+                    self.batch, self.members = None, []
+                since = 0
             if not running:
                 self._wait(max(0.0, pending[0].arrival - now))
                 continue
-            if changed:
-                self._repack(running)
-                changed = False
             self.batch.step(self.chunk)
+            since += 1
             sums, n = self.batch.ib_force_members(reset=True)
             self.chunks += 1
             if self.chunk_seconds is not None:
                 self.vtime += self.chunk_seconds
             now = self._now(t0)
-            still = []
             for i, r in enumerate(self.members):
+                if r.done is not None:
+                    continue
                 r.steps += self.chunk
                 r.cd.append(r.cd_of(sums[i, 0] / max(n, 1)))
                 if r.finished():
                     r.done = now
                     done.append(r)
-                    changed = True
-                else:
-                    still.append(r)
-            running = still
         if self.batch is not None:
             self.batch.close()
             self.batch, self.members = None, []
