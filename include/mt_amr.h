@@ -180,9 +180,11 @@ int mt_ib_treat(mt_case* c, double* force3);
 int mt_step(mt_case* c, int n_coarse);
 int mt_set_graphs(mt_case* c, int on);
 /* Launch-configuration autotuner (no counterpart in the reference): tries the
- * fused kernel's block -> brick maps and level concurrency, each for `steps`
- * coarse steps of the real run (+1 untimed), keeps the fastest.  Results do not
- * depend on the choice.  ms_per_step[4] may be NULL.                         */
+ * fused kernel's block -> brick maps and level concurrency, twice each, for
+ * `steps` coarse steps of the real run (+1 untimed) per try, and keeps the
+ * fastest (the run advances 8 (steps + 1) coarse steps).  Results do not
+ * depend on the choice.  ms_per_step[4] (may be NULL) gets each candidate's
+ * faster try.                                                                 */
 int mt_autotune(mt_case* c, int steps, double* ms_per_step);
 /* per-kernel event timing of mt_step (off by default; forces graphs off):
  * ms[level] accumulates the fused update kernel's time, calls[level] its count */

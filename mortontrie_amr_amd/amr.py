@@ -255,7 +255,7 @@ class AmrCase:
         _check(load().mt_case_copy_member(self.h, member, src.h, src_member))
 
     def autotune(self, steps=5):
-        """Pick the fastest launch configuration (advances the run by 4 (steps + 1) coarse steps);
+        """Pick the fastest launch configuration (advances the run by 8 (steps + 1) coarse steps);
         returns each candidate's ms per coarse step."""
         ms = np.zeros(4)
         _check(load().mt_autotune(self.h, steps, ms.ctypes.data_as(_dp)))

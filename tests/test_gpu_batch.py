@@ -87,7 +87,7 @@ def test_autotune_keeps_results():
     ms = a.autotune(2)
     assert np.all(ms > 0)
     a.step(3)
-    b.step(4 * 3 + 3)
+    b.step(8 * 3 + 3)
     for L in range(case["Lmax"] + 1):
         assert np.array_equal(a.f(L), b.f(L))
         assert np.array_equal(a.fcol(L), b.fcol(L))
