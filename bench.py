@@ -157,14 +157,15 @@ def run_batch(args):
     for m in members:
         upd_step += sum((c["fluid"] + c["c2f"]) << L for L, c in enumerate(C.level_counts(m)))
     g = amr.AmrCase(packed, precision=amr.FP32_SHIFTED, graphs=True)
-    tune = g.autotune(3)
+    tune_steps = 3
+    tune = g.autotune(tune_steps)
     g.step(args.warmup)
     g.sync()
     t0 = time.perf_counter()
     g.step(args.steps)
     g.sync()
     secs = time.perf_counter() - t0
-    total = args.warmup + args.steps + 4 * 4
+    total = args.warmup + args.steps + 8 * (tune_steps + 1)  # the autotuner steps the run too
     # members 1 (two-level) and 0 alone, same number of steps: identical fields
     same = []
     for i in (0, 1):
