@@ -647,13 +647,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   const int par = a.cpar[(b << 3) | (lane >> 3)];
   if (__ballot(par >= 0) == 0) return;  // uniform over the wave
   // every lane of a group with a parent is a streamed C2F cell holding its
-  // post-stream f; the sums run over all 64 lanes
-  const int b0 = lane & ~7;
+  // post-stream f; the sums run over all 64 lanes, as a butterfly within each
+  // aligned group of 8 (((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)), the same bits in
+  // every lane of the group since the additions commute exactly)
   __shared__ R lavg[8][Q];
   for (int q = 0; q < Q; ++q) {
-    R sum = __shfl(f[q], b0, 64);
-#pragma unroll
-    for (int t = 1; t < 8; ++t) sum += __shfl(f[q], b0 + t, 64);
+    R sum = f[q];
+    sum += __shfl_xor(sum, 1, 64);
+    sum += __shfl_xor(sum, 2, 64);
+    sum += __shfl_xor(sum, 4, 64);
     if ((lane & 7) == 0) lavg[lane >> 3][q] = sum / (R)8.0;
   }
   const int g = lane >> 3;
