@@ -223,6 +223,19 @@ def run_serve(args):
     t0 = time.perf_counter()
     done_sat = srv_sat.run(sat)
     sat_s = time.perf_counter() - t0
+    # the same capacity run through the library's own loop (mt_server_*)
+    nat = serve.NativeServer(precision=amr.FP32_SHIFTED, chunk=100, admit_every=4)
+    t0 = time.perf_counter()
+    for r in sat:
+        nat.submit(r.id, r.case, r.max_steps, r.tol)
+    nat_done = []
+    while True:
+        nr, nw = nat.step()
+        nat_done += nat.poll()
+        if nr == 0 and nw == 0:
+            break
+    nat_s = time.perf_counter() - t0
+    nat.close()
     t0 = time.perf_counter()
     plain = {r.id: serve.run_alone(r, precision=amr.FP64) for r in reqs}
     plain_s = time.perf_counter() - t0
@@ -253,6 +266,7 @@ def run_serve(args):
         "arrival_bound_cases_per_s": round(len(reqs) / max(r.arrival for r in reqs), 3),
         "capacity_cases_per_s": round(len(done_sat) / sat_s, 3),
         "capacity_speedup_vs_plain_fp64": round((len(done_sat) / sat_s) / (len(reqs) / plain_s), 3),
+        "native_server_capacity_cases_per_s": round(len(nat_done) / nat_s, 3),
         "drag_rel_error_last_chunk_max": float(np.nanmax(errs)) if np.isfinite(errs).any() else None,
         "drag_rel_error_last_chunk_median": float(np.nanmedian(errs)) if np.isfinite(errs).any() else None,
         "drag_rel_error_mean_max": float(np.nanmax(merrs)) if np.isfinite(merrs).any() else None,

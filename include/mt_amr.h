@@ -206,6 +206,31 @@ int mt_ib_force_members(mt_case* c, double* sum3, int32_t n_members, int64_t* co
  * requests arrive or finish.                                                  */
 int mt_case_copy_member(mt_case* dst, int32_t dst_member, mt_case* src, int32_t src_member);
 
+/* Continuous-batching server (no counterpart in the reference, which runs one
+ * flow per process; these are the submit / poll entry points of a serving
+ * loop).  A request is a single-member case built with mt_case_create,
+ * mt_case_upload_level and mt_case_upload_dis and not finalized; the server
+ * takes it over at submission (the handle is freed then, also on an error).
+ * Requests must share Q, Lmax, collision, IB method and options and fit a
+ * slot of 96 x 48 x 48 level-0 cells.  mt_server_step runs one chunk of
+ * `chunk_steps` coarse steps over every running request, packed into one case;
+ * every `admit_every` chunks (or at once when no request is left running) it
+ * admits waiting requests in submission order while their device footprint
+ * fits `hbm_budget` bytes.  A request finishes at max_steps or when its drag
+ * coefficient over the last chunk moved by less than drag_tol from the
+ * previous chunk's; Cd = 2 Fx / (U^2 pi d^2 / 4) with U = cd_U, d = cd_d, and
+ * each request computes exactly what it computes alone.  mt_server_poll hands
+ * out finished requests (id, steps, last chunk's Cd) in completion order.   */
+typedef struct mt_server mt_server;
+int mt_server_create(int32_t precision, int32_t chunk_steps, int32_t admit_every, int64_t hbm_budget,
+                     mt_server** out);
+int mt_server_submit(mt_server* s, mt_case* request, int64_t id, int32_t max_steps, double drag_tol,
+                     double cd_U, double cd_d);
+int mt_server_step(mt_server* s, int32_t* n_running, int32_t* n_waiting);
+int mt_server_poll(mt_server* s, int64_t* ids, int32_t* steps, double* cd_last, int32_t max_results,
+                   int32_t* n_results);
+int mt_server_destroy(mt_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,7 +51,8 @@ EXPORTS = [
     "mt_collide", "mt_stream", "mt_explode", "mt_coalesce", "mt_apply_bc", "mt_ib_treat", "mt_step",
     "mt_set_graphs", "mt_set_timing", "mt_get_timing", "mt_sync", "mt_reduce", "mt_ib_force",
     "mt_case_set_members", "mt_ib_force_members", "mt_autotune", "mt_case_copy_member",
-    "mt_case_upload_markers",
+    "mt_case_upload_markers", "mt_server_create", "mt_server_submit", "mt_server_step", "mt_server_poll",
+    "mt_server_destroy",
 ]
 
 _lib = None
@@ -95,6 +96,13 @@ def load():
     L.mt_autotune.argtypes = [C.c_void_p, C.c_int, _dp]
     L.mt_case_copy_member.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]
     L.mt_case_upload_markers.argtypes = [C.c_void_p, C.c_int64, _u64p, _dp, _dp, _dp]
+    L.mt_server_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.POINTER(C.c_void_p)]
+    L.mt_server_submit.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_double,
+                                   C.c_double]
+    L.mt_server_step.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.mt_server_poll.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), _dp, C.c_int32,
+                                 C.POINTER(C.c_int32)]
+    L.mt_server_destroy.argtypes = [C.c_void_p]
     _lib = L
     return L
 
@@ -112,7 +120,8 @@ def _u64(a):
 class AmrCase:
     """A device-resident AMR case built from a `cases` dictionary."""
 
-    def __init__(self, case, precision=FP64, graphs=True):
+    def __init__(self, case, precision=FP64, graphs=True, finalize=True):
+        """finalize=False stops after the uploads (a request for mt_server_submit, see detach())."""
         L = load()
         self.case = case
         self.Q = case["Q"]
@@ -163,8 +172,14 @@ class AmrCase:
                 raise MtError("markers: keys and w are [n][27], dv [n], ub [3]")
             _check(L.mt_case_upload_markers(self.h, dvm.size, pmk, w.ctypes.data_as(_dp),
                                             dvm.ctypes.data_as(_dp), ub.ctypes.data_as(_dp)))
-        _check(L.mt_case_finalize(self.h))
-        _check(L.mt_set_graphs(self.h, 1 if graphs else 0))
+        if finalize:
+            _check(L.mt_case_finalize(self.h))
+            _check(L.mt_set_graphs(self.h, 1 if graphs else 0))
+
+    def detach(self):
+        """Give up the handle (its new owner frees it)."""
+        h, self.h = self.h, None
+        return h
 
     def close(self):
         if getattr(self, "h", None) is not None and _lib is not None:

@@ -286,7 +286,6 @@ static int drain_timing(Case& c) {
 
 using namespace mt;
 
-struct mt_case : public mt::Case {};
 
 extern "C" {
 

@@ -231,3 +231,6 @@ int launch_scatter(Case& c, int L, int field, const int32_t* d_slots, int64_t n,
 int launch_reduce(Case& c, int L, double* h_out);
 
 }  // namespace mt
+
+// the C-ABI handle is the case itself
+struct mt_case : public mt::Case {};

@@ -185,3 +185,47 @@ def poisson_requests(n, rate=20.0, seed=1, max_steps=2000, tol=1e-4):
         t += float(rng.exponential(1.0 / rate))
         out.append(Request(i, C.sphere_request(rng), t, max_steps, tol))
     return out
+
+
+class NativeServer:
+    """The same serving loop inside the library (mt_server_*): submit request
+    cases, call step() until nothing runs or waits, collect with poll()."""
+
+    def __init__(self, precision=amr.FP32_SHIFTED, chunk=100, admit_every=1, hbm_budget=200 << 30):
+        import ctypes as C
+        self._C = C
+        L = amr.load()
+        h = C.c_void_p()
+        amr._check(L.mt_server_create(precision, chunk, admit_every, hbm_budget, C.byref(h)))
+        self.h = h
+        self.precision = precision
+
+    def submit(self, rid, case, max_steps=2000, tol=1e-4):
+        L = amr.load()
+        req = amr.AmrCase(case, precision=self.precision, finalize=False).detach()
+        cd = case["cd"]
+        amr._check(L.mt_server_submit(self.h, req, rid, max_steps, tol, cd["U"], cd["d"]))
+
+    def step(self):
+        C = self._C
+        nr, nw = C.c_int32(), C.c_int32()
+        amr._check(amr.load().mt_server_step(self.h, C.byref(nr), C.byref(nw)))
+        return nr.value, nw.value
+
+    def poll(self, max_results=256):
+        C = self._C
+        ids = (C.c_int64 * max_results)()
+        steps = (C.c_int32 * max_results)()
+        cd = (C.c_double * max_results)()
+        n = C.c_int32()
+        amr._check(amr.load().mt_server_poll(self.h, ids, steps, cd, max_results, C.byref(n)))
+        return [(ids[i], steps[i], cd[i]) for i in range(n.value)]
+
+    def close(self):
+        if getattr(self, "h", None) is not None and amr._lib is not None:
+            amr._lib.mt_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+

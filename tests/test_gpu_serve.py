@@ -93,3 +93,45 @@ def test_cases_rebuilt_on_cached_memory_compute_the_same():
             assert np.array_equal(a.f(L), want[L])
         b.close()
         a.close()
+
+
+def test_native_server_matches_alone():
+    """The library's own serving loop (mt_server_*): requests submitted while
+    others run finish with the steps and drag they give alone."""
+    rng = np.random.default_rng(8)
+    reqs = [small_request(i, 0.0, rng, max_steps=300 + 100 * (i % 2)) for i in range(4)]
+    srv = serve.NativeServer(precision=amr.FP32_SHIFTED, chunk=100, admit_every=2)
+    srv.submit(0, reqs[0].case, reqs[0].max_steps, 0.0)
+    srv.submit(1, reqs[1].case, reqs[1].max_steps, 0.0)
+    got = {}
+    srv.step()
+    srv.submit(2, reqs[2].case, reqs[2].max_steps, 0.0)
+    srv.step()
+    srv.submit(3, reqs[3].case, reqs[3].max_steps, 0.0)
+    for _ in range(50):
+        nr, nw = srv.step()
+        for rid, steps, cd in srv.poll():
+            got[rid] = (steps, cd)
+        if nr == 0 and nw == 0:
+            break
+    for rid, steps, cd in srv.poll():
+        got[rid] = (steps, cd)
+    srv.close()
+    assert sorted(got) == [0, 1, 2, 3]
+    for r in reqs:
+        alone = serve.run_alone(r, precision=amr.FP32_SHIFTED, chunk=100)
+        assert got[r.id][0] == alone.steps
+        assert np.isclose(got[r.id][1], alone.cd[-1], rtol=1e-9, atol=1e-12), r.id
+
+
+def test_native_server_rejects_bad_requests():
+    srv = serve.NativeServer(precision=amr.FP32_SHIFTED)
+    big = C.sphere_amr((128, 32, 32), radius=4.0, center=(20.0, 16.0, 16.0), shells=(3.0,), options=2)
+    with pytest.raises(amr.MtError, match="slot"):
+        srv.submit(0, big, 100, 0.0)
+    rng = np.random.default_rng(9)
+    srv.submit(1, small_request(1, 0.0, rng).case, 100, 0.0)
+    other = C.sphere_amr((48, 24, 24), radius=3.0, center=(16.0, 12.0, 12.0), shells=(), options=2)
+    with pytest.raises(amr.MtError, match="share"):
+        srv.submit(2, other, 100, 0.0)
+    srv.close()
