@@ -86,6 +86,7 @@ struct Server {
   std::vector<std::unique_ptr<Request>> results;
   mt_case* batch = nullptr;
   int since = 1 << 30;
+  std::string broken;  // set when a repack failed; every later call reports it
 };
 
 }  // namespace mt
@@ -234,7 +235,16 @@ int mt_server_create(int32_t precision, int32_t chunk_steps, int32_t admit_every
 
 int mt_server_submit(mt_server* s, mt_case* req, int64_t id, int32_t max_steps, double drag_tol, double cd_U,
                      double cd_d) {
-  if (!s || !req) { set_error("null server or request"); return MT_ERR_ARG; }
+  if (!s || !req) {
+    if (req) mt_case_destroy(req);
+    set_error("null server or request");
+    return MT_ERR_ARG;
+  }
+  if (!s->broken.empty()) {
+    mt_case_destroy(req);
+    set_error(s->broken);
+    return MT_ERR_STATE;
+  }
   Case& c = *req;
   auto bad = [&](const char* why) {
     mt_case_destroy(req);
@@ -278,6 +288,7 @@ int mt_server_submit(mt_server* s, mt_case* req, int64_t id, int32_t max_steps, 
 
 int mt_server_step(mt_server* s, int32_t* n_running, int32_t* n_waiting) {
   if (!s) { set_error("null server"); return MT_ERR_ARG; }
+  if (!s->broken.empty()) { set_error(s->broken); return MT_ERR_STATE; }
   int64_t active = 0, used = 0;
   for (auto& r : s->running)
     if (!r->done) {
@@ -299,7 +310,15 @@ int mt_server_step(mt_server* s, int32_t* n_running, int32_t* n_waiting) {
       s->waiting.pop_front();
     }
     int rc = repack(*s, std::move(next), old);
-    if (rc != MT_OK) return rc;
+    if (rc != MT_OK) {
+      // the running requests left with the failed repack: the server stops
+      s->broken = std::string("server stopped: a repack failed: ") + mt_last_error();
+      s->running.clear();
+      if (s->batch) mt_case_destroy(s->batch);
+      s->batch = nullptr;
+      set_error(s->broken);
+      return rc;
+    }
   }
   if (s->batch) {
     TRYS(mt_step(s->batch, s->chunk));
