@@ -220,7 +220,8 @@ int mt_case_copy_member(mt_case* dst, int32_t dst_member, mt_case* src, int32_t 
  * coefficient over the last chunk moved by less than drag_tol from the
  * previous chunk's; Cd = 2 Fx / (U^2 pi d^2 / 4) with U = cd_U, d = cd_d, and
  * each request computes exactly what it computes alone.  mt_server_poll hands
- * out finished requests (id, steps, last chunk's Cd) in completion order.   */
+ * out finished requests (id, steps, last chunk's Cd) in completion order.  A
+ * repack that fails stops the server: later calls return MT_ERR_STATE.      */
 typedef struct mt_server mt_server;
 int mt_server_create(int32_t precision, int32_t chunk_steps, int32_t admit_every, int64_t hbm_budget,
                      mt_server** out);

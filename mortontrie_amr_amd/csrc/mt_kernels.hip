@@ -405,9 +405,7 @@ __device__ __forceinline__ void ib_links_cell(R* f, const Args& a, const int* N,
         if ((wm >> q) & 1) {
           dist_c[j] = drow[qi - 1];
           finc_c[j] = load_pop<Q, S, R>(fin, ns, qi, s);
-          fs_c[j] = ((pm >> qi) & 1)
-                        ? load_pop<Q, S, R>(fin, ns, q, nbr_slot_c<-kEX[q], -kEY[q], -kEZ[q]>(N, lx, ly, lz))
-                        : finc_c[j];
+          fs_c[j] = f[q];  // the pull loaded fcol_q(x - e_q) (or fcol_inv(q)(x)) for a wall link
         }
       }
     });
@@ -534,6 +532,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   const uint32_t pmx = a.v.pmask[s];
   const uint32_t tm = a.tmask ? a.tmask[s] : 0u;
   const int row0 = IB ? a.dis_row[s] : -1;
+  const uint32_t wm0 = IB ? a.wmask[s] : 0u;
   {
     int N[27];
 #pragma unroll
@@ -541,7 +540,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     // keep the cell's loads together with the brick's: the compiler would
     // otherwise sink each into the branch that first uses it, one memory round
     // trip after another
-    asm volatile("" ::"v"(k), "v"(pmx), "v"(tm), "v"(row0), "s"(p_tau), "s"(p_les));
+    asm volatile("" ::"v"(k), "v"(pmx), "v"(tm), "v"(row0), "v"(wm0), "s"(p_tau), "s"(p_les));
     const int row = (IB && (k & MT_KIND_DIS) && (k & MT_KIND_FLUID)) ? row0 : -1;
     const bool treated = row >= 0;
     if (k & (MT_KIND_FLUID | MT_KIND_C2F)) {
@@ -561,6 +560,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         const S* pb = ((tm >> q) & 1) ? xb : fin;
         const S* src = p ? pb + (int64_t)q * ns + t
                          : (treated ? fin + (int64_t)qinv(q) * ns + s : fst + (int64_t)q * ns + s);
+        if (IB && treated && ((wm0 >> q) & 1)) {
+          // a wall link's population is interpolated from fcol_q(x - e_q) (or
+          // fcol_inv(q)(x) when x - e_q is missing): load that here, with the
+          // pull, instead of the source inside the wall
+          constexpr int qi = qinv(q);
+          src = ((pmx >> qi) & 1) ? fin + (int64_t)q * ns + nbr_slot_c<-kEX[q], -kEY[q], -kEZ[q]>(N, lx, ly, lz)
+                                  : fin + (int64_t)qi * ns + s;
+        }
         R v = (R)Store<S>::ld(src);
         if (Store<S>::shifted) v += (R)wq<Q>(q);
         f[q] = v;
