@@ -392,7 +392,7 @@ __device__ __forceinline__ void ib_links_cell(R* f, const Args& a, const int* N,
   // Wall links in chunks of CH directions: the chunk's loads (fraction,
   // fcol_inv(q)(x), fcol_q(x - e_q)) are issued together, then used, so a cell
   // waits for memory once per chunk instead of once per wall link.
-  constexpr int CH = 6;
+  constexpr int CH = 13;
   static_for<0, (Q - 1 + CH - 1) / CH>([&](auto cc) {
     constexpr int c0 = 1 + decltype(cc)::value * CH;
     double dist_c[CH];
