@@ -188,6 +188,15 @@ static int substep(Case& c, int L, bool child_last, bool write_f, bool nee, bool
   if (L < c.Lmax) {
     MT_HIP(hipEventRecord(c.ev_ready[L], s));
     MT_HIP(hipStreamWaitEvent(c.st[L + 1], c.ev_ready[L], 0));
+    // The previous sub-step's finer pair coalesced into this level's F2C
+    // cells, so it must finish before this sub-step's update of level L.
+    // Waiting here, after the fork, lets the next finer pair follow the
+    // previous one directly on its stream, without a round trip through this
+    // level's stream.
+    if (c.join_pending[L]) {
+      MT_HIP(hipStreamWaitEvent(s, c.ev_done[L + 1], 0));
+      c.join_pending[L] = false;
+    }
     const CoalReq cr = {write_macros ? 1 : 0};
     TRY(run_level(c, L + 1, child_last, l.coal_in_fused ? &cr : nullptr));
   }
@@ -196,18 +205,31 @@ static int substep(Case& c, int L, bool child_last, bool write_f, bool nee, bool
   if (L == c.Lmax) TRY(launch_markers(c, 1 - l.pf, s));
   if (L < c.Lmax) {
     MT_HIP(hipEventRecord(c.ev_done[L + 1], c.st[L + 1]));
-    MT_HIP(hipStreamWaitEvent(s, c.ev_done[L + 1], 0));
-    if (!l.coal_in_fused) TRY(launch_coalesce(c, L, true, write_macros));
+    if (l.coal_in_fused) {
+      c.join_pending[L] = true;
+    } else {
+      MT_HIP(hipStreamWaitEvent(s, c.ev_done[L + 1], 0));
+      TRY(launch_coalesce(c, L, true, write_macros));
+    }
   }
   l.pf ^= 1;
   l.pr ^= 1;
+  return MT_OK;
+}
+// the join a level still owes its finer pair, at the end of its sub-steps
+static int join_finer(Case& c, int L) {
+  if (L < c.Lmax && c.join_pending[L]) {
+    MT_HIP(hipStreamWaitEvent(c.st[L], c.ev_done[L + 1], 0));
+    c.join_pending[L] = false;
+  }
   return MT_OK;
 }
 static int run_level(Case& c, int L, bool last, const CoalReq* coal) {
   const bool res0 = (L == 0) && has_reservoir(c);
   if (L == 0) {
     // level 0 has one sub-step per coarse step
-    return substep(c, 0, last, last || res0, res0, last || res0, 0, true, nullptr);
+    TRY(substep(c, 0, last, last || res0, res0, last || res0, 0, true, nullptr));
+    return join_finer(c, 0);
   }
   // Sub-step B: the reference does not explode (LBM_Simulator.cpp:152-217).
   // With MT_OPT_EXPLODE_BOTH it explodes again from the same parent fcol (the
@@ -217,7 +239,8 @@ static int run_level(Case& c, int L, bool last, const CoalReq* coal) {
   const bool both = (c.d.options & MT_OPT_EXPLODE_BOTH) != 0;
   // (with c.xin the pair's targets are already in place: no explosion kernel)
   TRY(substep(c, L, false, false, false, false, c.xin ? 0 : (both ? 2 : 1), !both, nullptr));
-  return substep(c, L, last, last, false, last, 0, !both, coal);
+  TRY(substep(c, L, last, last, false, last, 0, !both, coal));
+  return join_finer(c, L);
 }
 
 static void use_level_streams(Case& c, bool on) {
@@ -236,16 +259,24 @@ static int parity_key(const Case& c, bool last) {
   return k;
 }
 
-static int coarse_step(Case& c, bool last) {
-  if (!c.graphs || c.timing) return run_level(c, 0, last, nullptr);
-  int key = parity_key(c, last);
+// k consecutive coarse steps (the last of them `last`), replayed as one HIP
+// graph: the step boundaries inside a graph cost no host launch, and a graph
+// per (parities, k, last) is captured once.
+constexpr int kStepsPerGraph = 4;
+static int coarse_steps(Case& c, int k, bool last) {
+  if (!c.graphs || c.timing) {
+    for (int j = 0; j < k; ++j) TRY(run_level(c, 0, last && j == k - 1, nullptr));
+    return MT_OK;
+  }
+  const int key = parity_key(c, last) | (k << 24);
   auto it = c.graph_exec.find(key);
   int pf[kMaxLevels], pr[kMaxLevels];
   for (int L = 0; L <= c.Lmax; ++L) { pf[L] = c.lv[L].pf; pr[L] = c.lv[L].pr; }
   if (it == c.graph_exec.end()) {
     hipGraph_t g;
     MT_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-    int r = run_level(c, 0, last, nullptr);
+    int r = MT_OK;
+    for (int j = 0; j < k && r == MT_OK; ++j) r = run_level(c, 0, last && j == k - 1, nullptr);
     hipError_t e = hipStreamEndCapture(c.stream, &g);
     if (r) return r;
     if (e != hipSuccess) { set_error(std::string("graph capture: ") + hipGetErrorString(e)); return MT_ERR_HIP; }
@@ -257,10 +288,10 @@ static int coarse_step(Case& c, bool last) {
     for (int L = 0; L <= c.Lmax; ++L) { c.lv[L].pf = pf[L]; c.lv[L].pr = pr[L]; }
   }
   MT_HIP(hipGraphLaunch(it->second, c.stream));
-  // parities advance as in the captured schedule: level L flips 2^L times
-  // per coarse step except level 0 which flips once
+  // parities advance as in the captured schedule: per coarse step level L
+  // flips 2^L times, level 0 once
   for (int L = 0; L <= c.Lmax; ++L) {
-    int flips = L == 0 ? 1 : (1 << L);
+    const int flips = k * (L == 0 ? 1 : (1 << L));
     c.lv[L].pf = pf[L] ^ (flips & 1);
     c.lv[L].pr = pr[L] ^ (flips & 1);
   }
@@ -672,7 +703,11 @@ int mt_step(mt_case* c, int n) {
   }
   use_level_streams(*c, true);
   int r = MT_OK;
-  for (int i = 0; i < n && r == MT_OK; ++i) r = coarse_step(*c, i == n - 1);
+  for (int i = 0; i < n && r == MT_OK;) {
+    const int k = std::min(kStepsPerGraph, n - i);
+    r = coarse_steps(*c, k, i + k == n);
+    i += k;
+  }
   use_level_streams(*c, false);
   if (r != MT_OK) return r;
   if (c->timing) TRY(drain_timing(*c));

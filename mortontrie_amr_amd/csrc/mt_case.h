@@ -150,6 +150,7 @@ struct Case {
   hipStream_t st[kMaxLevels] = {nullptr};
   hipEvent_t ev_ready[kMaxLevels] = {nullptr};  // level L's current fcol final for its children
   hipEvent_t ev_done[kMaxLevels] = {nullptr};   // level L's sub-step tree finished
+  bool join_pending[kMaxLevels] = {false};       // level L has not yet waited for ev_done[L + 1]
   std::vector<void*> allocs;
   // IB force
   double* force_part = nullptr;  // per-block partials of the finest fused kernel
