@@ -216,10 +216,19 @@ static int substep(Case& c, int L, bool child_last, bool write_f, bool nee, bool
   l.pr ^= 1;
   return MT_OK;
 }
-// the join a level still owes its finer pair, at the end of its sub-steps
-static int join_finer(Case& c, int L) {
-  if (L < c.Lmax && c.join_pending[L]) {
-    MT_HIP(hipStreamWaitEvent(c.st[L], c.ev_done[L + 1], 0));
+// The joins the levels still owe their finer pairs, finest first, so that
+// c.stream (level 0's) ends after everything issued.  A level's own next
+// update waits for its finer pair (the coalescence into its F2C cells) inside
+// `substep`, after the fork; nothing coarser needs more than the finer level's
+// own updates (a finer level's work writes only its own data and the F2C cells
+// of the level above it), so between coarse steps a join is carried over and
+// only the end of the issued steps closes every stream.
+static int join_all(Case& c) {
+  for (int L = c.Lmax - 1; L >= 0; --L) {
+    if (c.st[L] != c.st[L + 1]) {
+      MT_HIP(hipEventRecord(c.ev_done[L + 1], c.st[L + 1]));
+      MT_HIP(hipStreamWaitEvent(c.st[L], c.ev_done[L + 1], 0));
+    }
     c.join_pending[L] = false;
   }
   return MT_OK;
@@ -228,8 +237,7 @@ static int run_level(Case& c, int L, bool last, const CoalReq* coal) {
   const bool res0 = (L == 0) && has_reservoir(c);
   if (L == 0) {
     // level 0 has one sub-step per coarse step
-    TRY(substep(c, 0, last, last || res0, res0, last || res0, 0, true, nullptr));
-    return join_finer(c, 0);
+    return substep(c, 0, last, last || res0, res0, last || res0, 0, true, nullptr);
   }
   // Sub-step B: the reference does not explode (LBM_Simulator.cpp:152-217).
   // With MT_OPT_EXPLODE_BOTH it explodes again from the same parent fcol (the
@@ -239,8 +247,7 @@ static int run_level(Case& c, int L, bool last, const CoalReq* coal) {
   const bool both = (c.d.options & MT_OPT_EXPLODE_BOTH) != 0;
   // (with c.xin the pair's targets are already in place: no explosion kernel)
   TRY(substep(c, L, false, false, false, false, c.xin ? 0 : (both ? 2 : 1), !both, nullptr));
-  TRY(substep(c, L, last, last, false, last, 0, !both, coal));
-  return join_finer(c, L);
+  return substep(c, L, last, last, false, last, 0, !both, coal);
 }
 
 static void use_level_streams(Case& c, bool on) {
@@ -266,7 +273,7 @@ constexpr int kStepsPerGraph = 4;
 static int coarse_steps(Case& c, int k, bool last) {
   if (!c.graphs || c.timing) {
     for (int j = 0; j < k; ++j) TRY(run_level(c, 0, last && j == k - 1, nullptr));
-    return MT_OK;
+    return join_all(c);
   }
   const int key = parity_key(c, last) | (k << 24);
   auto it = c.graph_exec.find(key);
@@ -277,6 +284,7 @@ static int coarse_steps(Case& c, int k, bool last) {
     MT_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     int r = MT_OK;
     for (int j = 0; j < k && r == MT_OK; ++j) r = run_level(c, 0, last && j == k - 1, nullptr);
+    if (r == MT_OK) r = join_all(c);
     hipError_t e = hipStreamEndCapture(c.stream, &g);
     if (r) return r;
     if (e != hipSuccess) { set_error(std::string("graph capture: ") + hipGetErrorString(e)); return MT_ERR_HIP; }

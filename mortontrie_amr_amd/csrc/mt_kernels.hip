@@ -347,6 +347,22 @@ struct FusedArgs {
   void* cx_dst;                        // with COAL: this level's targets of the next pair
 };
 
+// A value from another lane of the wave through a DPP lane permutation (CTRL:
+// quad_perm [1,0,3,2] = 0xB1 reads lane ^ 1, [2,3,0,1] = 0x4E lane ^ 2,
+// row_half_mirror = 0x141 lane 7 - i within each aligned group of 8).  Every
+// lane must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_get(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_get(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
 // AMR_transDDF_explosion (LBM_Manager.cpp:1059-1129) of one parent held in a
 // lane: its post-collision fcol rescaled around the He-Luo equilibrium of its
 // own moments, written to its (up to 8, -1 = none) targets.
@@ -654,15 +670,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   const int par = a.cpar[(b << 3) | (lane >> 3)];
   if (__ballot(par >= 0) == 0) return;  // uniform over the wave
   // every lane of a group with a parent is a streamed C2F cell holding its
-  // post-stream f; the sums run over all 64 lanes, as a butterfly within each
-  // aligned group of 8 (((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)), the same bits in
-  // every lane of the group since the additions commute exactly)
+  // post-stream f; the sums run over all 64 lanes (all active here), as a
+  // butterfly within each aligned group of 8 (((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)),
+  // the same bits in every lane of the group since the additions commute
+  // exactly; the last step pairs lane i with lane 7 - i, whose partial sum is
+  // that of lane i ^ 4)
   __shared__ R lavg[8][Q];
   for (int q = 0; q < Q; ++q) {
     R sum = f[q];
-    sum += __shfl_xor(sum, 1, 64);
-    sum += __shfl_xor(sum, 2, 64);
-    sum += __shfl_xor(sum, 4, 64);
+    sum += dpp_get<0xB1>(sum);
+    sum += dpp_get<0x4E>(sum);
+    sum += dpp_get<0x141>(sum);
     if ((lane & 7) == 0) lavg[lane >> 3][q] = sum / (R)8.0;
   }
   const int g = lane >> 3;
