@@ -179,15 +179,30 @@ int mt_ib_treat(mt_case* c, double* force3);
  *      fused per-level kernels, captured as HIP graphs when graphs are on.    */
 int mt_step(mt_case* c, int n_coarse);
 int mt_set_graphs(mt_case* c, int on);
+/* How mt_step issues the levels' fused updates (no counterpart in the
+ * reference; every schedule computes the same bits):
+ *   MT_SCHED_MERGED   one launch per finest sub-step on one stream, carrying
+ *                     each coarser level's update whose span of finest
+ *                     sub-steps ends with it (the default; where explosions do
+ *                     not run inside the updates, as MT_SCHED_SERIAL);
+ *   MT_SCHED_STREAMS  each level on its own stream, a level's update
+ *                     concurrent with its finer pair, events between streams;
+ *   MT_SCHED_SERIAL   the levels' updates one after another on one stream.   */
+#define MT_SCHED_MERGED 0
+#define MT_SCHED_STREAMS 1
+#define MT_SCHED_SERIAL 2
+int mt_set_schedule(mt_case* c, int schedule);
 /* Launch-configuration autotuner (no counterpart in the reference): tries the
- * fused kernel's block -> brick maps and level concurrency, twice each, for
- * `steps` coarse steps of the real run (+1 untimed) per try, and keeps the
- * fastest (the run advances 8 (steps + 1) coarse steps).  Results do not
- * depend on the choice.  ms_per_step[4] (may be NULL) gets each candidate's
- * faster try.                                                                 */
+ * fused kernel's block -> brick maps and the schedules MT_SCHED_STREAMS and
+ * MT_SCHED_MERGED, twice each, for `steps` coarse steps of the real run (+1
+ * untimed) per try, and keeps the fastest (the run advances 8 (steps + 1)
+ * coarse steps).  Results do not depend on the choice.  ms_per_step[4] (may
+ * be NULL) gets each candidate's faster try, in the order (map 0, streams),
+ * (map 1, streams), (map 0, merged), (map 1, merged).                         */
 int mt_autotune(mt_case* c, int steps, double* ms_per_step);
 /* per-kernel event timing of mt_step (off by default; forces graphs off):
- * ms[level] accumulates the fused update kernel's time, calls[level] its count */
+ * ms[level] accumulates the fused update kernel's time, calls[level] its count;
+ * a launch that carries several levels' updates counts under the finest.     */
 int mt_set_timing(mt_case* c, int on);
 int mt_get_timing(mt_case* c, double* ms, int64_t* calls, int max_levels);
 int mt_sync(mt_case* c);

@@ -49,7 +49,7 @@ EXPORTS = [
     "mt_last_error", "mt_case_create", "mt_case_destroy", "mt_case_upload_level", "mt_case_upload_dis",
     "mt_case_finalize", "mt_case_level_info", "mt_find", "mt_get_field", "mt_set_field", "mt_macros",
     "mt_collide", "mt_stream", "mt_explode", "mt_coalesce", "mt_apply_bc", "mt_ib_treat", "mt_step",
-    "mt_set_graphs", "mt_set_timing", "mt_get_timing", "mt_sync", "mt_reduce", "mt_ib_force",
+    "mt_set_graphs", "mt_set_schedule", "mt_set_timing", "mt_get_timing", "mt_sync", "mt_reduce", "mt_ib_force",
     "mt_case_set_members", "mt_ib_force_members", "mt_autotune", "mt_case_copy_member",
     "mt_case_upload_markers", "mt_server_create", "mt_server_submit", "mt_server_step", "mt_server_poll",
     "mt_server_destroy",
@@ -86,6 +86,7 @@ def load():
     L.mt_ib_treat.argtypes = [C.c_void_p, _dp]
     L.mt_step.argtypes = [C.c_void_p, C.c_int]
     L.mt_set_graphs.argtypes = [C.c_void_p, C.c_int]
+    L.mt_set_schedule.argtypes = [C.c_void_p, C.c_int]
     L.mt_set_timing.argtypes = [C.c_void_p, C.c_int]
     L.mt_get_timing.argtypes = [C.c_void_p, _dp, _i64p, C.c_int]
     L.mt_sync.argtypes = [C.c_void_p]
@@ -264,6 +265,13 @@ class AmrCase:
 
     def set_graphs(self, on):
         _check(load().mt_set_graphs(self.h, 1 if on else 0))
+
+    SCHEDULES = {"merged": 0, "streams": 1, "serial": 2}
+
+    def set_schedule(self, name):
+        """How mt_step issues the levels' updates: "merged" (one launch per finest
+        sub-step, the default), "streams" (a stream per level) or "serial"."""
+        _check(load().mt_set_schedule(self.h, self.SCHEDULES[name]))
 
     def copy_member_from(self, member, src, src_member):
         """This case's member `member` takes the state of `src`'s member `src_member`."""

@@ -250,8 +250,72 @@ static int run_level(Case& c, int L, bool last, const CoalReq* coal) {
   return substep(c, L, last, last, false, last, 0, !both, coal);
 }
 
+// One coarse step as 2^Lmax launches on one stream (Case::merge_levels): launch
+// t carries the finest level's sub-step t and every coarser level L's sub-step
+// whose span of 2^(Lmax-L) finest sub-steps ends with t, with the flags the
+// recursion in run_level gives it.  Level L's update j then runs beside the
+// last update of its finer pair j (the stream schedule runs it beside the
+// whole pair), after its finer pair j-1 (an earlier span) and before its finer
+// pair j+1 (the next span): the orders the stream schedule's fork and join
+// keep, without any wait between streams.  Any launch of the span would keep
+// them; the last one (the finest level's sub-step B, whose coalescing bricks
+// leave the longest tail) measured fastest, with the coarser levels' blocks
+// dispatched first.  A level's buffers flip when its span ends, as in
+// run_level, where a level flips after its finer pair.  Needs the explosions
+// and coalescences inside the updates (Case::xin).
+static bool merged(const Case& c) { return c.merge_levels && c.xin && c.Lmax >= 1; }
+
+static int merged_step(Case& c, bool last) {
+  const int Lm = c.Lmax;
+  const bool res0 = has_reservoir(c);
+  const bool both = (c.d.options & MT_OPT_EXPLODE_BOTH) != 0;
+  CoalReq cr[kMaxLevels];
+  LevelUpdate u[kMaxLevels];
+  for (int t = 0; t < (1 << Lm); ++t) {
+    int n = 0;
+    bool nee = false;
+    for (int L = 0; L <= Lm; ++L) {
+      const int span = 1 << (Lm - L);
+      if (t % span != span - 1) continue;
+      const int j = t / span;  // level L's sub-step within the coarse step
+      LevelUpdate& x = u[n++];
+      x.L = L;
+      x.coal = nullptr;
+      if (L == 0) {
+        x.write_f = x.write_macros = last || res0;
+        x.c2f_fcol = true;
+        nee = res0;
+      } else {
+        // sub-step B (odd j) of the level's last pair in the last step writes f
+        // and the maps; B coalesces into level L-1 with the maps flag of
+        // level L-1's current sub-step
+        const bool b = (j & 1) != 0;
+        x.write_f = x.write_macros = b && last && (j >> 1) == (1 << (L - 1)) - 1;
+        x.c2f_fcol = !both;
+        if (b) {
+          const int jp = j >> 1;
+          const bool wm = (L == 1) ? (last || res0) : ((jp & 1) != 0 && last && (jp >> 1) == (1 << (L - 2)) - 1);
+          cr[L].write_macros = wm ? 1 : 0;
+          x.coal = &cr[L];
+        }
+      }
+    }
+    TRY(launch_fused_levels(c, n, u, nee));
+    TRY(launch_markers(c, 1 - c.lv[Lm].pf, c.st[Lm]));
+    // the levels whose span ends here flip (their finer pair's updates in the
+    // span address the buffers of the sub-step in progress)
+    for (int i = 0; i < n; ++i) {
+      c.lv[u[i].L].pf ^= 1;
+      c.lv[u[i].L].pr ^= 1;
+    }
+  }
+  return MT_OK;
+}
+
+static int one_step(Case& c, bool last) { return merged(c) ? merged_step(c, last) : run_level(c, 0, last, nullptr); }
+
 static void use_level_streams(Case& c, bool on) {
-  on = on && c.level_streams;
+  on = on && c.level_streams && !merged(c);
   for (int L = 0; L <= c.Lmax; ++L) c.st[L] = (on && L > 0) ? c.lstream[L] : c.stream;
 }
 
@@ -272,7 +336,7 @@ static int parity_key(const Case& c, bool last) {
 constexpr int kStepsPerGraph = 4;
 static int coarse_steps(Case& c, int k, bool last) {
   if (!c.graphs || c.timing) {
-    for (int j = 0; j < k; ++j) TRY(run_level(c, 0, last && j == k - 1, nullptr));
+    for (int j = 0; j < k; ++j) TRY(one_step(c, last && j == k - 1));
     return join_all(c);
   }
   const int key = parity_key(c, last) | (k << 24);
@@ -283,7 +347,7 @@ static int coarse_steps(Case& c, int k, bool last) {
     hipGraph_t g;
     MT_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     int r = MT_OK;
-    for (int j = 0; j < k && r == MT_OK; ++j) r = run_level(c, 0, last && j == k - 1, nullptr);
+    for (int j = 0; j < k && r == MT_OK; ++j) r = one_step(c, last && j == k - 1);
     if (r == MT_OK) r = join_all(c);
     hipError_t e = hipStreamEndCapture(c.stream, &g);
     if (r) return r;
@@ -728,7 +792,7 @@ int mt_step(mt_case* c, int n) {
 // computes the same values, so the tuning steps are ordinary steps.  The
 // fastest candidate is kept; ms_out[i] (may be NULL) receives candidate i's
 // time per coarse step, in the order (map 0, streams), (map 1, streams),
-// (map 0, one stream), (map 1, one stream).
+// (map 0, merged), (map 1, merged).
 int mt_autotune(mt_case* c, int steps, double* ms_out) {
   CHECK_FINAL(c);
   if (steps <= 0) { set_error("steps must be positive"); return MT_ERR_ARG; }
@@ -747,6 +811,7 @@ int mt_autotune(mt_case* c, int steps, double* ms_out) {
     const int i = k & 3;
     c->xmap = i & 1;
     c->level_streams = i < 2;
+    c->merge_levels = i >= 2;
     drop_graphs(*c);
     r = mt_step(c, 1);
     if (r != MT_OK) break;
@@ -769,6 +834,7 @@ int mt_autotune(mt_case* c, int steps, double* ms_out) {
   hipEventDestroy(e1);
   c->xmap = best_i & 1;
   c->level_streams = best_i < 2;
+  c->merge_levels = best_i >= 2;
   drop_graphs(*c);
   c->timing = timing;
   if (r == MT_ERR_HIP) set_error("autotune: HIP event failure");
@@ -778,6 +844,17 @@ int mt_autotune(mt_case* c, int steps, double* ms_out) {
 int mt_set_graphs(mt_case* c, int on) {
   CHECK_CASE(c);
   c->graphs = on != 0;
+  return MT_OK;
+}
+int mt_set_schedule(mt_case* c, int schedule) {
+  CHECK_CASE(c);
+  if (schedule < MT_SCHED_MERGED || schedule > MT_SCHED_SERIAL) {
+    set_error("unknown schedule");
+    return MT_ERR_ARG;
+  }
+  c->merge_levels = schedule == MT_SCHED_MERGED;
+  c->level_streams = schedule == MT_SCHED_STREAMS;
+  drop_graphs(*c);
   return MT_OK;
 }
 int mt_set_timing(mt_case* c, int on) {

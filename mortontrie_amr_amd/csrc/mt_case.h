@@ -131,6 +131,10 @@ struct Case {
   // launch configuration (results do not depend on it; mt_autotune picks it)
   int xmap = 0;               // fused kernel block -> brick map (FusedArgs::xmap)
   bool level_streams = true;  // levels on concurrent streams in mt_step
+  // with explosions inside the updates (xin): mt_step issues one launch per
+  // finest sub-step on one stream, carrying every coarser level's update whose
+  // span ends with it (takes precedence over level_streams)
+  bool merge_levels = true;
   bool wet_any = false;  // some member has a Zou-He / outflow face
   bool res_any = false;  // some member has a reservoir face
   // Explosions inside the fused updates (with MT_OPT_EXPLODE_BOTH).  Both
@@ -224,6 +228,17 @@ int launch_bc(Case& c);
 int launch_ib(Case& c, bool reduce_into_acc);
 int launch_fused(Case& c, int L, bool write_f, bool bc_nee_after, bool write_macros = true, bool c2f_fcol = true,
                  const CoalReq* coal = nullptr);
+// one level's part of a launch of several levels' fused updates
+struct LevelUpdate {
+  int L;
+  bool write_f;
+  bool write_macros;
+  bool c2f_fcol;
+  const CoalReq* coal;
+};
+// The fused updates of n levels in one launch on the finest one's stream (every
+// level's stream must be that one), then the reservoir NEE if nee_after.
+int launch_fused_levels(Case& c, int n, const LevelUpdate* u, bool nee_after);
 bool level_has_ib(const Case& c, int L);
 int build_markers(Case& c);
 int launch_markers(Case& c, int buf, hipStream_t st);

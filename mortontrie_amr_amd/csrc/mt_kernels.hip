@@ -24,6 +24,12 @@
 
 namespace mt {
 
+#define TRY_K(x)                  \
+  do {                            \
+    int r_ = (x);                 \
+    if (r_ != MT_OK) return r_;   \
+  } while (0)
+
 constexpr uint16_t kHasPop = MT_KIND_FLUID | MT_KIND_F2C | MT_KIND_C2F | MT_KIND_GHOST_C2F | MT_KIND_SOLID | MT_KIND_BCRES;
 
 struct LvArgs {
@@ -519,13 +525,16 @@ __device__ __forceinline__ void ib_links_cell(R* f, const Args& a, const int* N,
 // shuffles; the group's first lane rescales the average around its He-Luo
 // equilibrium and runs the F2C cell's calculateMacros + collide, which the
 // reference does at the start of the coarse level's next sub-step.
+//
+// fused_block is the update of block `blk` of a launch of `nb` blocks (a
+// multiple of 8) for one level; k_fused launches one level, k_fused_levels
+// several levels side by side.
 template <int Q, typename S, bool COAL, bool WET, bool IB>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_fused(FusedArgs a) {
+__device__ __forceinline__ void fused_block(const FusedArgs& a, const uint32_t blk, const int64_t nb) {
   typedef typename RealOf<S>::type R;
   const int lane = threadIdx.x;
-  const int64_t nb = gridDim.x;
   const int64_t per_xcd = (nb + 7) >> 3;
-  const int64_t bi = a.xmap ? (int64_t)blockIdx.x : (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  const int64_t bi = a.xmap ? (int64_t)blk : (int64_t)(blk & 7) * per_xcd + (blk >> 3);
   const S* fin = (const S*)a.v.fcol_in;
   S* fout = (S*)a.v.fcol_out;
   S* fst = (S*)a.v.f;
@@ -728,6 +737,41 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   if (par >= 0 && !(k & MT_KIND_DIS)) {
 #pragma unroll
     for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)a.cx_dst, ns, q, s, lavg[g][q]);
+  }
+}
+
+template <int Q, typename S, bool COAL, bool WET, bool IB>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_fused(FusedArgs a) {
+  fused_block<Q, S, COAL, WET, IB>(a, blockIdx.x, gridDim.x);
+}
+
+// Several levels' fused updates in one launch: segment i owns the blocks
+// [end[i-1], end[i]) (multiples of 8, so each segment keeps its XCD map) and
+// runs the instantiation variant[i] = COAL | WET << 1 | IB << 2.  A coarse
+// step issued this way is one chain of launches on one stream, with no
+// cross-stream waits.
+struct LevelsArgs {
+  int n;
+  int variant[kMaxLevels];
+  uint32_t end[kMaxLevels];
+  FusedArgs seg[kMaxLevels];
+};
+
+template <int Q, typename S>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_fused_levels(LevelsArgs m) {
+  const uint32_t blk = blockIdx.x;
+  int i = 0;
+  while (i + 1 < m.n && blk >= m.end[i]) ++i;
+  const uint32_t b0 = i ? m.end[i - 1] : 0u;
+  const FusedArgs& a = m.seg[i];
+  const int64_t nb = (int64_t)(m.end[i] - b0);
+  switch (m.variant[i]) {
+    case 0: fused_block<Q, S, false, false, false>(a, blk - b0, nb); break;
+    case 1: fused_block<Q, S, true, false, false>(a, blk - b0, nb); break;
+    case 2: fused_block<Q, S, false, true, false>(a, blk - b0, nb); break;
+    case 4: fused_block<Q, S, false, false, true>(a, blk - b0, nb); break;
+    case 5: fused_block<Q, S, true, false, true>(a, blk - b0, nb); break;
+    default: break;  // (a level with both boundaries is a single-level case)
   }
 }
 
@@ -1384,10 +1428,11 @@ static int launch_events(Kern kern, int64_t nb, int bs, hipStream_t st, Args& a,
   return MT_OK;
 }
 
-int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros, bool c2f_fcol,
-                 const CoalReq* coal) {
+// The arguments of level L's fused update in its current parities, its
+// instantiation (COAL | WET << 1 | IB << 2) and its grid.
+static int fused_args(Case& c, int L, bool write_f, bool write_macros, bool c2f_fcol, const CoalReq* coal,
+                      FusedArgs& a, int& variant, int64_t& nb) {
   DevLevel& l = c.lv[L];
-  FusedArgs a;
   a.v = lvargs(c, L, l.pf, 1 - l.pf, l.pr, 1 - l.pr);
   a.active = l.active;
   a.n_active = l.n_active;
@@ -1451,23 +1496,8 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros
     a.c_write_macros = coal->write_macros;
   }
   // grid: a multiple of 8 blocks so the XCD remap covers every brick index
-  int64_t nb = ((l.n_active + 7) / 8) * 8;
+  nb = ((l.n_active + 7) / 8) * 8;
   if (nb < 8) nb = 8;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (c.timing) {
-    if (c.ev_pool.size() < 2) {
-      for (int i = 0; i < 64; ++i) {
-        hipEvent_t e;
-        MT_HIP(hipEventCreate(&e));
-        c.ev_pool.push_back(e);
-      }
-    }
-    e0 = c.ev_pool.back();
-    c.ev_pool.pop_back();
-    e1 = c.ev_pool.back();
-    c.ev_pool.pop_back();
-  }
-  int r = MT_OK;
   // the domain boundary (level 0 only), the coalescence (finer levels only) and
   // the solid boundary (finest level only) are compiled into separate
   // instantiations, so no level's kernel carries the registers of another's
@@ -1475,7 +1505,37 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros
     set_error("coalescence and domain boundary in one fused update");
     return MT_ERR_STATE;
   }
-  const int variant = (coal ? 1 : 0) | (a.do_bc ? 2 : 0) | (a.do_ib ? 4 : 0);
+  variant = (coal ? 1 : 0) | (a.do_bc ? 2 : 0) | (a.do_ib ? 4 : 0);
+  return MT_OK;
+}
+
+// a start/stop event pair for timing mode (null when not timing)
+static int timing_events(Case& c, hipEvent_t& e0, hipEvent_t& e1) {
+  e0 = e1 = nullptr;
+  if (!c.timing) return MT_OK;
+  if (c.ev_pool.size() < 2) {
+    for (int i = 0; i < 64; ++i) {
+      hipEvent_t e;
+      MT_HIP(hipEventCreate(&e));
+      c.ev_pool.push_back(e);
+    }
+  }
+  e0 = c.ev_pool.back();
+  c.ev_pool.pop_back();
+  e1 = c.ev_pool.back();
+  c.ev_pool.pop_back();
+  return MT_OK;
+}
+
+int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros, bool c2f_fcol,
+                 const CoalReq* coal) {
+  FusedArgs a;
+  int variant = 0;
+  int64_t nb = 0;
+  TRY_K(fused_args(c, L, write_f, write_macros, c2f_fcol, coal, a, variant, nb));
+  hipEvent_t e0, e1;
+  TRY_K(timing_events(c, e0, e1));
+  int r = MT_OK;
   switch (variant) {
     case 0: MT_DISPATCH(c, (r = launch_events(k_fused<Q_, S_, false, false, false>, nb, 64, c.st[L], a, e0, e1))); break;
     case 1: MT_DISPATCH(c, (r = launch_events(k_fused<Q_, S_, true, false, false>, nb, 64, c.st[L], a, e0, e1))); break;
@@ -1491,6 +1551,51 @@ int launch_fused(Case& c, int L, bool write_f, bool nee_after, bool write_macros
     // reservoir NEE reads the pre-sub-step maps (rho/u[pr]) and the f just written
     int r = launch_nee(c);
     if (r) return r;
+  }
+  return MT_OK;
+}
+
+int launch_fused_levels(Case& c, int n, const LevelUpdate* u, bool nee_after) {
+  if (n < 1 || n > kMaxLevels) {
+    set_error("fused update of an invalid number of levels");
+    return MT_ERR_STATE;
+  }
+  LevelsArgs m;
+  m.n = n;
+  int64_t total = 0;
+  for (int i = 0; i < n; ++i) {
+    int64_t nb = 0;
+    TRY_K(fused_args(c, u[i].L, u[i].write_f, u[i].write_macros, u[i].c2f_fcol, u[i].coal, m.seg[i], m.variant[i],
+                     nb));
+    if (m.variant[i] == 6) {
+      set_error("a level with both the domain and the solid boundary in a launch of several levels");
+      return MT_ERR_STATE;
+    }
+    total += nb;
+    if (total >= (int64_t)1 << 31) {
+      set_error("fused update grid too large");
+      return MT_ERR_STATE;
+    }
+    m.end[i] = (uint32_t)total;
+  }
+  for (int i = n; i < kMaxLevels; ++i) {
+    m.variant[i] = 0;
+    m.end[i] = (uint32_t)total;
+  }
+  hipEvent_t e0, e1;
+  TRY_K(timing_events(c, e0, e1));
+  // every level's stream is c.stream here; the launch is timed under the finest
+  // level taking part
+  int Lf = 0;
+  for (int i = 0; i < n; ++i) Lf = std::max(Lf, u[i].L);
+  int r = MT_OK;
+  MT_DISPATCH(c, (r = launch_events(k_fused_levels<Q_, S_>, total, 64, c.st[Lf], m, e0, e1)));
+  if (r) return r;
+  MT_HIP(hipGetLastError());
+  if (c.timing) c.ev_pending.push_back({Lf, {e0, e1}});
+  if (nee_after) {
+    int r2 = launch_nee(c);
+    if (r2) return r2;
   }
   return MT_OK;
 }
