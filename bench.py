@@ -45,7 +45,7 @@ METRIC = "MLUPS & % HBM roofline (FP64 vs FP32-shifted); served cases/s + p50/p9
 WORKLOAD = "sphere_amr_3level_d3q27_cumulant_fp64"
 MARKER_WORKLOAD = "sphere_markers6000_3level_d3q27_cumulant_les0.1_fp64"
 CAVITY_WORKLOAD = "cavity64_d3q19_lbgk_zouhe_fp64"
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_fused_sphere_amr_fp64_hbm_traffic.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_fused_levels_sphere_amr_fp64_hbm_traffic.md")
 
 
 def bytes_per_update(Q, precision):

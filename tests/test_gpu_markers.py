@@ -74,6 +74,28 @@ def test_sphere_markers_graph_replay_and_split_steps():
     b.close()
 
 
+@pytest.mark.parametrize("precision", [amr.FP32_SHIFTED, amr.FP16_STORAGE])
+def test_sphere_markers_schedules_same_bits_reduced_precision(precision):
+    """In FP32 and FP16 storage too, the merged schedule, per-level streams and
+    the serial order give the same bits (fields and marker force sums)."""
+    case = small_marker_sphere()
+    runs = []
+    for sched in ("merged", "streams", "serial"):
+        g = amr.AmrCase(case, precision=precision)
+        g.set_schedule(sched)
+        g.step(2)
+        g.step(3)
+        runs.append(g)
+    a = runs[0]
+    for b in runs[1:]:
+        for L in range(case["Lmax"] + 1):
+            assert np.array_equal(a.f(L), b.f(L))
+            assert np.array_equal(a.fcol(L), b.fcol(L))
+        assert np.array_equal(a.ib_force()[0], b.ib_force()[0])
+    for g in runs:
+        g.close()
+
+
 def test_markers_on_interface_rejected():
     """A marker whose kernel reaches a cell that is not a streamed fluid cell of
     the finest level (here the C2F layer) is rejected at finalize."""
