@@ -198,6 +198,33 @@ def test_schedules_compute_the_same_bits(graphs):
         g.close()
 
 
+def test_reservoir_sphere_merged_schedule():
+    """The reference's layout (LBGK, reservoir faces with NEE, FH + MDF) with both-
+    sub-step explosions and a finest shell wide enough for explosions inside the
+    updates: the merged schedule runs (every launch is timed under the finest
+    level) with the reservoir NEE after the launch that carries level 0, matches
+    the oracle, and equals the per-level streams bit for bit."""
+    case = C.reference_sphere((32, 24, 24), radius=3.0, center=(12, 12, 12), shells=(5.5, 2.0))
+    case["options"] = 2
+    Lmax = case["Lmax"]
+    g, o = amr.AmrCase(case), O.OracleCase(case)
+    g.set_timing(True)
+    g.step(6)
+    ms, calls = g.timing()
+    g.set_timing(False)
+    assert all(calls[L] == 0 for L in range(Lmax)) and calls[Lmax] == 6 * 2 ** Lmax
+    o.step(6)
+    compare(g, o, case)
+    b = amr.AmrCase(case)
+    b.set_schedule("streams")
+    b.step(6)
+    for L in range(Lmax + 1):
+        assert np.array_equal(g.f(L), b.f(L)) and np.array_equal(g.fcol(L), b.fcol(L))
+    assert np.array_equal(g.ib_force()[0], b.ib_force()[0])
+    g.close()
+    b.close()
+
+
 def test_reference_sphere_layout():
     """The configuration the compiled reference runs: LBGK, reservoir faces with
     NEE, FH + MDF on the finest of 3 levels."""
