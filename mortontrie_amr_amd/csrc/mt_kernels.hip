@@ -1602,7 +1602,7 @@ int launch_fused_levels(Case& c, int n, const LevelUpdate* u, bool nee_after) {
 
 // ------------------------------------------------------------------ markers
 // Explicit direct forcing of the finest level by Lagrangian markers (the
-// definition in mt_amr.h), four kernels per finest sub-step:
+// definition in mt_amr.h), three kernels per finest sub-step:
 //   k_mk_moments  one thread per band cell (a cell in some marker's support):
 //                 its density and velocity from fcol;
 //   k_mk_interp   one 32-lane half-wave per marker, lane j < 27 its j-th kernel
@@ -1610,8 +1610,9 @@ int launch_fused_levels(Case& c, int n, const LevelUpdate* u, bool nee_after) {
 //                 F_k = dV_k rho_k (ub - u_k);
 //   k_mk_spread   one thread per band cell: the momentum of its markers
 //                 (ascending marker order) turned into the exact-difference
-//                 population shift;
-//   k_mk_force    the body force sum_k F_k in a fixed order.
+//                 population shift; an extra block 0 sums the body force
+//                 sum_k F_k in a fixed order beside them (mk_force_sum; F is
+//                 final when the kernel starts).
 template <int Q, typename S>
 __global__ __launch_bounds__(256) void k_mk_moments(const S* fcol, int64_t ns, const int32_t* band, int64_t nb,
                                                     double* mom) {
@@ -1658,12 +1659,55 @@ __global__ __launch_bounds__(256) void k_mk_interp(const double* mom, int64_t nb
   }
 }
 
+// the body force of this call, sum_k F_k (the frame of the link-wise force,
+// whose velocities are the reported ones), into acc[0..2] (count in acc[3]) by
+// one block of 256 threads: 1024 strided partial sums, then a fixed pairwise
+// tree (the first two levels pair partial sums that one thread holds)
+__device__ void mk_force_sum(const double* F, int64_t n, double* acc) {
+  __shared__ double red[3][256];
+  const int t = threadIdx.x;
+  double s[4][3];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    s[p][0] = s[p][1] = s[p][2] = 0;
+    for (int64_t m = t + 256 * p; m < n; m += 1024) {
+      s[p][0] += F[3 * m];
+      s[p][1] += F[3 * m + 1];
+      s[p][2] += F[3 * m + 2];
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    s[0][a] += s[2][a];  // partial t + 512
+    s[1][a] += s[3][a];  // partial t + 768 into t + 256
+    red[a][t] = s[0][a] + s[1][a];
+  }
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h)
+      for (int a = 0; a < 3; ++a) red[a][t] += red[a][t + h];
+    __syncthreads();
+  }
+  if (t == 0) {
+    for (int a = 0; a < 3; ++a) {
+      acc[a] += red[a][0];
+      acc[4 + a] = red[a][0];
+    }
+    acc[3] += 1.0;
+  }
+}
+
 template <int Q, typename S>
 __global__ __launch_bounds__(256) void k_mk_spread(S* fcol, int64_t ns, const int32_t* band, const int32_t* ptr,
                                                    const int32_t* mk, const double* bw, int64_t nb,
-                                                   const double* mom, const double* F) {
+                                                   const double* mom, const double* F, int64_t n_mk, double* acc) {
   typedef typename RealOf<S>::type R;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // block 0 sums the body force beside the band's blocks 1, 2, ...
+  if (blockIdx.x == 0) {
+    mk_force_sum(F, n_mk, acc);
+    return;
+  }
+  const int64_t i = (int64_t)(blockIdx.x - 1) * blockDim.x + threadIdx.x;
   if (i >= nb) return;
   double mx = 0, my = 0, mz = 0;
   for (int32_t e = ptr[i]; e < ptr[i + 1]; ++e) {
@@ -1686,30 +1730,6 @@ __global__ __launch_bounds__(256) void k_mk_spread(S* fcol, int64_t ns, const in
   }
 }
 
-// the body force of this call, sum_k F_k (the frame of the link-wise force,
-// whose velocities are the reported ones), into acc[0..2] (count in acc[3]):
-// 1024 strided partial sums, then a fixed tree
-__global__ __launch_bounds__(1024) void k_mk_force(const double* F, int64_t n, double* acc) {
-  __shared__ double red[3][1024];
-  double s[3] = {0, 0, 0};
-  for (int64_t m = threadIdx.x; m < n; m += 1024)
-    for (int a = 0; a < 3; ++a) s[a] += F[3 * m + a];
-  for (int a = 0; a < 3; ++a) red[a][threadIdx.x] = s[a];
-  __syncthreads();
-  for (int h = 512; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h)
-      for (int a = 0; a < 3; ++a) red[a][threadIdx.x] += red[a][threadIdx.x + h];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    for (int a = 0; a < 3; ++a) {
-      acc[a] += red[a][0];
-      acc[4 + a] = red[a][0];
-    }
-    acc[3] += 1.0;
-  }
-}
-
 int launch_markers(Case& c, int buf, hipStream_t st) {
   if (!c.n_mk) return MT_OK;
   DevLevel& l = c.lv[c.Lmax];
@@ -1717,10 +1737,9 @@ int launch_markers(Case& c, int buf, hipStream_t st) {
                                                                        c.band_slot, c.n_band, c.band_mom)));
   k_mk_interp<<<nblk(32 * c.n_mk), 256, 0, st>>>(c.band_mom, c.n_band, c.mk_band, c.mk_w, c.mk_dv, c.n_mk,
                                                  c.mk_ub[0], c.mk_ub[1], c.mk_ub[2], c.mk_F);
-  MT_DISPATCH(c, (k_mk_spread<Q_, S_><<<nblk(c.n_band), 256, 0, st>>>((S_*)l.fcol[buf], l.n_slots, c.band_slot,
+  MT_DISPATCH(c, (k_mk_spread<Q_, S_><<<nblk(c.n_band) + 1, 256, 0, st>>>((S_*)l.fcol[buf], l.n_slots, c.band_slot,
                                                                       c.band_ptr, c.band_mk, c.band_w, c.n_band,
-                                                                      c.band_mom, c.mk_F)));
-  k_mk_force<<<1, 1024, 0, st>>>(c.mk_F, c.n_mk, c.force_acc);
+                                                                      c.band_mom, c.mk_F, c.n_mk, c.force_acc)));
   MT_HIP(hipGetLastError());
   return MT_OK;
 }
