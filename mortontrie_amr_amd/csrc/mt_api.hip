@@ -202,7 +202,7 @@ static int substep(Case& c, int L, bool child_last, bool write_f, bool nee, bool
   }
   if (explode) TRY(launch_explode(c, L, explode == 2));
   TRY(launch_fused(c, L, write_f, nee, write_macros, c2f_fcol, coal));
-  if (L == c.Lmax) TRY(launch_markers(c, 1 - l.pf, s));
+  if (L == c.Lmax) TRY(launch_markers(c, 1 - l.pf, s, true));
   if (L < c.Lmax) {
     MT_HIP(hipEventRecord(c.ev_done[L + 1], c.st[L + 1]));
     if (l.coal_in_fused) {
@@ -301,7 +301,7 @@ static int merged_step(Case& c, bool last) {
       }
     }
     TRY(launch_fused_levels(c, n, u, nee));
-    TRY(launch_markers(c, 1 - c.lv[Lm].pf, c.st[Lm]));
+    TRY(launch_markers(c, 1 - c.lv[Lm].pf, c.st[Lm], true));
     // the levels whose span ends here flip (their finer pair's updates in the
     // span address the buffers of the sub-step in progress)
     for (int i = 0; i < n; ++i) {

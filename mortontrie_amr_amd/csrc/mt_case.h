@@ -118,6 +118,7 @@ struct Case {
   double* mk_dv = nullptr;          // [n]
   double* mk_F = nullptr;           // [n][3] this call's marker forces
   int32_t* band_slot = nullptr;     // cells in some marker's support
+  int32_t* band_row = nullptr;      // [finest n_slots] band index of the slot, -1 none
   int64_t n_band = 0;
   int32_t* band_ptr = nullptr;      // [n_band + 1]
   int32_t* band_mk = nullptr;       // markers of each cell, ascending
@@ -241,7 +242,9 @@ struct LevelUpdate {
 int launch_fused_levels(Case& c, int n, const LevelUpdate* u, bool nee_after);
 bool level_has_ib(const Case& c, int L);
 int build_markers(Case& c);
-int launch_markers(Case& c, int buf, hipStream_t st);
+// moments_done: the finest fused update has already left the band's moments
+// (FusedArgs::band_row)
+int launch_markers(Case& c, int buf, hipStream_t st, bool moments_done = false);
 int launch_gather(Case& c, int L, int field, const int32_t* d_slots, int64_t n, double* d_out, int buf);
 int launch_scatter(Case& c, int L, int field, const int32_t* d_slots, int64_t n, const double* d_in, int buf);
 int launch_reduce(Case& c, int L, double* h_out);

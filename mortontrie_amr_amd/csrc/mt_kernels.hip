@@ -351,6 +351,11 @@ struct FusedArgs {
   void* gx_dst;                        // level L+1 targets of the next pair
   int64_t ns_f;
   void* cx_dst;                        // with COAL: this level's targets of the next pair
+  // markers (finest level): the band cells' density and velocity of the new
+  // fcol, [4][n_band], as k_mk_moments computes them
+  const int32_t* __restrict__ band_row;
+  double* band_mom;
+  int64_t n_band;
 };
 
 // A value from another lane of the wave through a DPP lane permutation (CTRL:
@@ -641,6 +646,21 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const uint32_t b
         collide_cell<Q, R>(f, rho, ux, uy, uz, (R)p_tau, (R)p_les, a.model);
 #pragma unroll
         for (int q = 0; q < Q; ++q) store_pop<Q, S, R>(fout, ns, q, s, f[q]);
+        // (loaded here, not with the cell's tables: a value live across the
+        // pull costs the step more than the load's latency on marker cases)
+        const int brow = a.band_row ? a.band_row[s] : -1;
+        if (brow >= 0) {
+          // a marker band cell: the moments of the fcol just stored (rounded
+          // through the storage format, as a load would return them)
+#pragma unroll
+          for (int q = 0; q < Q; ++q) f[q] = snap_pop<Q, S, R>(q, f[q]);
+          R br, bx, by, bz;
+          cell_macros<Q, R>(f, br, bx, by, bz);
+          a.band_mom[brow] = (double)br;
+          a.band_mom[a.n_band + brow] = (double)bx;
+          a.band_mom[2 * a.n_band + brow] = (double)by;
+          a.band_mom[3 * a.n_band + brow] = (double)bz;
+        }
         if (a.gxrow) {
           // a ghost_F2C parent explodes its next collision into the finer
           // level's ghost_C2F targets of the next pair
@@ -1465,6 +1485,14 @@ static int fused_args(Case& c, int L, bool write_f, bool write_macros, bool c2f_
   a.gx_dst = nullptr;
   a.ns_f = 0;
   a.cx_dst = nullptr;
+  a.band_row = nullptr;
+  a.band_mom = nullptr;
+  a.n_band = 0;
+  if (L == c.Lmax && c.n_mk && c.band_row) {
+    a.band_row = c.band_row;
+    a.band_mom = c.band_mom;
+    a.n_band = c.n_band;
+  }
   if (c.xin) {
     if (L >= 1) {
       a.tmask = l.tmask;
@@ -1604,7 +1632,8 @@ int launch_fused_levels(Case& c, int n, const LevelUpdate* u, bool nee_after) {
 // Explicit direct forcing of the finest level by Lagrangian markers (the
 // definition in mt_amr.h), three kernels per finest sub-step:
 //   k_mk_moments  one thread per band cell (a cell in some marker's support):
-//                 its density and velocity from fcol;
+//                 its density and velocity from fcol (after a fused update of
+//                 the finest level, that update has left them already);
 //   k_mk_interp   one 32-lane half-wave per marker, lane j < 27 its j-th kernel
 //                 cell: the weighted moments summed by a shuffle tree, then
 //                 F_k = dV_k rho_k (ub - u_k);
@@ -1730,11 +1759,12 @@ __global__ __launch_bounds__(256) void k_mk_spread(S* fcol, int64_t ns, const in
   }
 }
 
-int launch_markers(Case& c, int buf, hipStream_t st) {
+int launch_markers(Case& c, int buf, hipStream_t st, bool moments_done) {
   if (!c.n_mk) return MT_OK;
   DevLevel& l = c.lv[c.Lmax];
-  MT_DISPATCH(c, (k_mk_moments<Q_, S_><<<nblk(c.n_band), 256, 0, st>>>((const S_*)l.fcol[buf], l.n_slots,
-                                                                       c.band_slot, c.n_band, c.band_mom)));
+  if (!moments_done)
+    MT_DISPATCH(c, (k_mk_moments<Q_, S_><<<nblk(c.n_band), 256, 0, st>>>((const S_*)l.fcol[buf], l.n_slots,
+                                                                         c.band_slot, c.n_band, c.band_mom)));
   k_mk_interp<<<nblk(32 * c.n_mk), 256, 0, st>>>(c.band_mom, c.n_band, c.mk_band, c.mk_w, c.mk_dv, c.n_mk,
                                                  c.mk_ub[0], c.mk_ub[1], c.mk_ub[2], c.mk_F);
   MT_DISPATCH(c, (k_mk_spread<Q_, S_><<<nblk(c.n_band) + 1, 256, 0, st>>>((S_*)l.fcol[buf], l.n_slots, c.band_slot,
@@ -1822,7 +1852,9 @@ int build_markers(Case& c) {
   c.band_w = c.alloc<double>((int64_t)bw.size());
   c.band_mom = c.alloc<double>(4 * c.n_band);
   c.mk_band = c.alloc<int32_t>(ne);
-  if (!c.band_slot || !c.band_ptr || !c.band_mk || !c.band_w || !c.band_mom || !c.mk_band) {
+  const int64_t ns_f = c.lv[c.Lmax].n_slots;
+  c.band_row = c.alloc<int32_t>(ns_f);
+  if (!c.band_slot || !c.band_ptr || !c.band_mk || !c.band_w || !c.band_mom || !c.mk_band || !c.band_row) {
     set_error("out of device memory");
     return MT_ERR_HIP;
   }
@@ -1831,6 +1863,9 @@ int build_markers(Case& c) {
   MT_HIP(hipMemcpyAsync(c.band_mk, mk.data(), 4 * mk.size(), hipMemcpyHostToDevice, c.stream));
   MT_HIP(hipMemcpyAsync(c.band_w, bw.data(), 8 * bw.size(), hipMemcpyHostToDevice, c.stream));
   MT_HIP(hipMemcpyAsync(c.mk_band, mb.data(), 4 * ne, hipMemcpyHostToDevice, c.stream));
+  std::vector<int32_t> row(ns_f, -1);
+  for (int64_t i = 0; i < c.n_band; ++i) row[band[i]] = (int32_t)i;
+  MT_HIP(hipMemcpyAsync(c.band_row, row.data(), 4 * ns_f, hipMemcpyHostToDevice, c.stream));
   MT_HIP(hipStreamSynchronize(c.stream));
   return MT_OK;
 }
