@@ -57,6 +57,11 @@ struct DevLevel {
   // work lists
   int32_t* active = nullptr;  // bricks holding lat_f or C2F cells
   int64_t n_active = 0;
+  // the same bricks in the order of a sub-step that coalesces into the coarser
+  // level (coalescing bricks first), with each one's position in `active`
+  // (its force slot)
+  int32_t* active_b = nullptr;
+  int32_t* slot_b = nullptr;
   int32_t* f2c = nullptr;     // F2C slots of this level (coarse side)
   int64_t n_f2c = 0;
   int32_t* f2c_child = nullptr;        // [n_f2c][8] child slots on level+1

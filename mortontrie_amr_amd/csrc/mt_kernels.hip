@@ -319,6 +319,7 @@ __device__ __forceinline__ void block_reduce3(double& x, double& y, double& z, d
 struct FusedArgs {
   LvArgs v;
   const int32_t* __restrict__ active;
+  const int32_t* __restrict__ slot_of;  // force slot of each list position (null: the position)
   int64_t n_active;
   int xmap;  // block -> brick map: 0 one contiguous range per XCD, 1 in order
   int model;
@@ -688,9 +689,10 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const uint32_t b
     }
     if (lane == 0) {
       if (fx != 0.0 || fy != 0.0 || fz != 0.0) {
-        a.force_slots[3 * bi] += fx;
-        a.force_slots[3 * bi + 1] += fy;
-        a.force_slots[3 * bi + 2] += fz;
+        const int64_t fs = a.slot_of ? (int64_t)a.slot_of[bi] : bi;
+        a.force_slots[3 * fs] += fx;
+        a.force_slots[3 * fs + 1] += fy;
+        a.force_slots[3 * fs + 2] += fz;
       }
       if (bi == 0) a.force_acc[7] += 1.0;  // treat() calls folded into the slots
     }
@@ -1454,7 +1456,8 @@ static int fused_args(Case& c, int L, bool write_f, bool write_macros, bool c2f_
                       FusedArgs& a, int& variant, int64_t& nb) {
   DevLevel& l = c.lv[L];
   a.v = lvargs(c, L, l.pf, 1 - l.pf, l.pr, 1 - l.pr);
-  a.active = l.active;
+  a.active = (coal && l.active_b) ? l.active_b : l.active;
+  a.slot_of = (coal && l.active_b) ? l.slot_b : nullptr;
   a.n_active = l.n_active;
   a.xmap = c.xmap;
   a.model = c.d.collision;

@@ -625,6 +625,8 @@ int build_members(Case& c) {
 // update has a longer dependent tail go first within their range, so they do
 // not finish last: treated solid-boundary cells, then coalescence groups, then
 // ghost_F2C parents (explosion stores), then the rest; Morton order otherwise.
+// A level that coalesces into the coarser one gets a second order for the
+// sub-step that does (coalescence groups first), over the same XCD ranges.
 static int order_active(Case& c, int L) {
   DevLevel& l = c.lv[L];
   const int64_t na = l.n_active;
@@ -644,25 +646,50 @@ static int order_active(Case& c, int L) {
     MT_HIP(hipMemcpyAsync(gx.data(), l.gxrow, 4 * l.n_slots, hipMemcpyDeviceToHost, c.stream));
   }
   MT_HIP(hipStreamSynchronize(c.stream));
-  auto cls = [&](int32_t b) {
-    int k = 3;
+  // bit 0: treated cells, bit 1: coalescence groups, bit 2: ghost_F2C parents
+  auto flags = [&](int32_t b) {
+    int k = 0;
     for (int i = 0; i < 64; ++i) {
       const int64_t s = (int64_t)b * 64 + i;
-      if ((kind[s] & MT_KIND_DIS) && (kind[s] & MT_KIND_FLUID) && drow[s] >= 0) return 0;
-      if (!cpar.empty() && (i & 7) == 0 && cpar[s >> 3] >= 0) k = std::min(k, 1);
-      if (!gx.empty() && gx[s] >= 0) k = std::min(k, 2);
+      if ((kind[s] & MT_KIND_DIS) && (kind[s] & MT_KIND_FLUID) && drow[s] >= 0) k |= 1;
+      if (!cpar.empty() && (i & 7) == 0 && cpar[s >> 3] >= 0) k |= 2;
+      if (!gx.empty() && gx[s] >= 0) k |= 4;
     }
     return k;
   };
+  auto rank_a = [](int k) { return (k & 1) ? 0 : (k & 2) ? 1 : (k & 4) ? 2 : 3; };
+  // in a sub-step that coalesces, a coalescing brick's tail (the parents'
+  // rescale, collision and explosion) is the longest
+  auto rank_b = [](int k) { return (k & 2) ? 0 : (k & 1) ? 1 : (k & 4) ? 2 : 3; };
   const int64_t per = ((na + 7) / 8 * 8) / 8;
+  std::vector<int32_t> act_b(na), slot_b(na);
   for (int64_t x = 0; x * per < na; ++x) {
-    auto b = act.begin() + x * per, e = act.begin() + std::min<int64_t>(na, (x + 1) * per);
+    const int64_t lo = x * per, hi = std::min<int64_t>(na, (x + 1) * per);
     std::vector<std::pair<int, int32_t>> v;
-    for (auto it = b; it != e; ++it) v.push_back({cls(*it), *it});
-    std::stable_sort(v.begin(), v.end(), [](auto& p, auto& q) { return p.first < q.first; });
-    for (size_t i = 0; i < v.size(); ++i) b[i] = v[i].second;
+    for (int64_t i = lo; i < hi; ++i) v.push_back({flags(act[i]), act[i]});
+    std::stable_sort(v.begin(), v.end(), [&](auto& p, auto& q) { return rank_a(p.first) < rank_a(q.first); });
+    std::vector<int64_t> ib(v.size());
+    for (size_t i = 0; i < v.size(); ++i) {
+      act[lo + i] = v[i].second;
+      ib[i] = (int64_t)i;
+    }
+    std::stable_sort(ib.begin(), ib.end(), [&](int64_t p, int64_t q) { return rank_b(v[p].first) < rank_b(v[q].first); });
+    for (size_t i = 0; i < v.size(); ++i) {
+      act_b[lo + i] = v[ib[i]].second;
+      slot_b[lo + i] = (int32_t)(lo + ib[i]);
+    }
   }
   MT_HIP(hipMemcpyAsync(l.active, act.data(), 4 * na, hipMemcpyHostToDevice, c.stream));
+  if (l.cpar) {
+    l.active_b = c.alloc<int32_t>(na);
+    l.slot_b = c.alloc<int32_t>(na);
+    if (!l.active_b || !l.slot_b) {
+      set_error("out of device memory");
+      return MT_ERR_HIP;
+    }
+    MT_HIP(hipMemcpyAsync(l.active_b, act_b.data(), 4 * na, hipMemcpyHostToDevice, c.stream));
+    MT_HIP(hipMemcpyAsync(l.slot_b, slot_b.data(), 4 * na, hipMemcpyHostToDevice, c.stream));
+  }
   MT_HIP(hipStreamSynchronize(c.stream));
   return MT_OK;
 }
