@@ -706,19 +706,20 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const uint32_t b
   // the same bits in every lane of the group since the additions commute
   // exactly; the last step pairs lane i with lane 7 - i, whose partial sum is
   // that of lane i ^ 4)
-  __shared__ R lavg[8][Q];
+  // (in place: a lane's own f is not read again; every lane of the group
+  // holds the group's average)
+#pragma unroll
   for (int q = 0; q < Q; ++q) {
     R sum = f[q];
     sum += dpp_get<0xB1>(sum);
     sum += dpp_get<0x4E>(sum);
     sum += dpp_get<0x141>(sum);
-    if ((lane & 7) == 0) lavg[lane >> 3][q] = sum / (R)8.0;
+    f[q] = sum / (R)8.0;
   }
+  __shared__ R lavg[8][Q];
   const int g = lane >> 3;
   if (!(lane & 7) && par >= 0) {
-    R avg[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) avg[q] = lavg[g][q];
+    R* avg = f;
     R rho, ux, uy, uz;
     cell_macros<Q, R>(avg, rho, ux, uy, uz);
     R uu = ux * ux + uy * uy + uz * uz;
