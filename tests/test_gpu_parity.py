@@ -225,6 +225,32 @@ def test_reservoir_sphere_merged_schedule():
     b.close()
 
 
+def test_four_levels_merged_schedule():
+    """Four levels (eight finest sub-steps per coarse step): the merged schedule
+    places each level's update at the end of its span, with the coalescence
+    flags of a grandparent's sub-step, and matches the oracle and the per-level
+    streams bit for bit."""
+    case = C.sphere_amr((40, 32, 32), radius=2.0, center=(16, 16, 16), shells=(7.0, 4.0, 2.0), options=2)
+    Lmax = case["Lmax"]
+    assert Lmax == 3
+    g, o = amr.AmrCase(case), O.OracleCase(case)
+    g.set_timing(True)
+    g.step(2)
+    ms, calls = g.timing()
+    g.set_timing(False)
+    assert all(calls[L] == 0 for L in range(Lmax)) and calls[Lmax] == 2 * 2 ** Lmax
+    o.step(2)
+    compare(g, o, case)
+    b = amr.AmrCase(case)
+    b.set_schedule("streams")
+    b.step(2)
+    for L in range(Lmax + 1):
+        assert np.array_equal(g.f(L), b.f(L)) and np.array_equal(g.fcol(L), b.fcol(L))
+    assert np.array_equal(g.ib_force()[0], b.ib_force()[0])
+    g.close()
+    b.close()
+
+
 def test_reference_sphere_layout():
     """The configuration the compiled reference runs: LBGK, reservoir faces with
     NEE, FH + MDF on the finest of 3 levels."""
