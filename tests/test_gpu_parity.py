@@ -407,7 +407,6 @@ def test_cpp_driver_runs():
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("no g++")
-    out = os.path.join(ROOT, "gpurun_out" if os.path.isdir(os.path.join(ROOT, "gpurun_out")) else "/tmp")
     exe = os.path.join("/tmp", "mt_driver_check")
     subprocess.check_call([gxx, "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"),
                            os.path.join(ROOT, "tests", "cpp", "driver_check.cpp"), "-o", exe,
