@@ -375,14 +375,19 @@ def test_missing_key_raises_lookup_error():
 
 # ---------------------------------------------------------------- full size
 def test_full_sphere_properties():
-    """Full-size 3-level sphere (explosion in both sub-steps): deterministic
-    (two runs bitwise equal), finite, mass bounded; seams-free fused path."""
+    """Full-size 3-level sphere (explosion in both sub-steps, the benchmark case):
+    the merged schedule and the per-level streams, in different step splits,
+    give the same bits (fields and force sums); finite, mass bounded."""
     case = C.sphere_amr(options=2)
     a, b = amr.AmrCase(case), amr.AmrCase(case)
+    b.set_schedule("streams")
     a.step(60)
-    b.step(60)
+    b.step(25)
+    b.step(35)
     for L in range(3):
         assert np.array_equal(a.f(L), b.f(L))
+        assert np.array_equal(a.fcol(L), b.fcol(L))
+    assert np.array_equal(a.ib_force()[0], b.ib_force()[0])
     m, p, nan = a.reduce(2)
     assert nan == 0 and np.isfinite(m)
     cd = a.drag_coefficient()
