@@ -324,6 +324,7 @@ struct Level {
 
 struct Case {
   int Q = 19, Lmax = 0, collision = 0, ib = 0, nee_seq = 1, options = 0;
+  int mdf_sweeps = 8;  // the reference's MDF_ITERS; tests set 0 to read the pre-forcing state
   double tau0 = 0.6, les_C = 0, rho0 = 1;
   double u0[3] = {0, 0, 0};
   int dims0[3] = {0, 0, 0};
@@ -836,7 +837,7 @@ struct Case {
     force_log.push_back({fx, fy, fz});
     if (F) { F[0] = fx; F[1] = fy; F[2] = fz; }
     // multi-direct forcing, 8 sweeps of relaxation 0.5 (:180-225)
-    for (int it = 0; it < 8; ++it) {
+    for (int it = 0; it < mdf_sweeps; ++it) {
       for (auto& d : dis) {
         uint64_t k = d.first;
         if (!l.lat_f_set.count(k)) continue;
@@ -914,6 +915,18 @@ void* mto_create(int Q, int Lmax, double tau0, int collision, double les_C, doub
 }
 void mto_destroy(void* h) { delete (Case*)h; }
 void mto_set_options(void* h, int options) { ((Case*)h)->options = options; }
+int mto_set_mdf_sweeps(void* h, int n) {
+  ((Case*)h)->mdf_sweeps = n;
+  return 0;
+}
+// overrides tau_L after mto_init (tests that run the reference's compiled path
+// use the relaxation times its setUnit computed, LBM_Manager.cpp:763-844)
+int mto_set_tau(void* h, int L, double tau) {
+  Case* c = (Case*)h;
+  if (L < 0 || L > c->Lmax || (int)c->tau.size() <= L) return 1;
+  c->tau[L] = tau;
+  return 0;
+}
 
 int mto_add_level(void* h, int L, const uint64_t* keys, const uint16_t* kinds, int64_t n) {
   Case* c = (Case*)h;

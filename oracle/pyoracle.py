@@ -12,7 +12,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "libmt_oracle.so")
+# MT_ORACLE_LIB points the checker at another build of the oracle (mutation runs)
+LIB_PATH = os.environ.get("MT_ORACLE_LIB") or os.path.join(HERE, "_build", "libmt_oracle.so")
 
 _lib = None
 
@@ -37,6 +38,8 @@ def lib():
                                  _dp, C.c_int, C.c_int]
         L.mto_destroy.argtypes = [C.c_void_p]
         L.mto_set_options.argtypes = [C.c_void_p, C.c_int]
+        L.mto_set_tau.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.mto_set_mdf_sweeps.argtypes = [C.c_void_p, C.c_int]
         L.mto_add_level.argtypes = [C.c_void_p, C.c_int, _u64p, _u16p, C.c_int64]
         L.mto_set_dis.argtypes = [C.c_void_p, _u64p, _dp, C.c_int64]
         L.mto_set_markers.argtypes = [C.c_void_p, C.c_int64, _u64p, _dp, _dp, _dp]
@@ -119,6 +122,16 @@ class OracleCase:
 
     def step(self, n=1):
         lib().mto_step(self.h, n)
+
+    def set_mdf_sweeps(self, n):
+        """Number of multi-direct forcing sweeps treat() runs (the reference's 8)."""
+        lib().mto_set_mdf_sweeps(self.h, int(n))
+
+    def set_tau(self, taus):
+        """Replaces tau_L (after init) with the given per-level values."""
+        for L, t in enumerate(taus):
+            if lib().mto_set_tau(self.h, L, float(t)):
+                raise ValueError(f"no level {L}")
 
     def get(self, level, what, keys=None):
         if keys is None:

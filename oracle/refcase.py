@@ -1,0 +1,184 @@
+"""Runs the reference's own compiled update path on a case (test infrastructure only).
+
+oracle/_ref/libref_amr.so is built by oracle/Makefile.ref from the reference's
+sources where they lie (see that file for the translation units) plus
+oracle/ref_driver.cpp.  Its LBM_Manager and the collision model are process-wide
+singletons that bind the first case they see (header/lbm_kernel/Collision_Manager.hpp:
+33-34, 57-61), so every run happens in a child process of its own: the parent writes
+the case, the initial state and a list of operations to a file, the child loads the
+library, runs them, and writes the reference's state back.
+
+The compiled reference is D3Q19 with three levels (C_Q 19, C_max_level 2), LBGK,
+reservoir faces with NEE, and FH (or distance-weighted) link interpolation on the
+finest level; only cases of that shape can be run.  Its relaxation times come from
+its own setUnit (LBM_Manager.cpp:763-844); the caller gives the oracle the same
+values (OracleCase.set_tau).
+
+Operations (rows of three ints):
+    (0, n, 0)   n coarse steps: LBMkernel(0); calculateMacros()  (LBM_Simulator.cpp:11-17)
+    (1, op, L)  one seam at level L: 0 calculateMacros, 1 collide, 2 stream,
+                3 AMR_transDDF_explosion, 4 AMR_transDDF_coalescence,
+                5 the six AfterStream face strategies, 6 treat()
+    (2, 0, 0)   record the state of every level and the running IB force sums
+    (3, 0, 0)   reset the running IB force sums
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+REF_LIB = os.path.join(HERE, "_ref", "libref_amr.so")
+REF_SRC = "/root/reference"
+
+K_FLUID, K_F2C, K_C2F, K_GHOST_C2F, K_GHOST_F2C, K_SOLID, K_BCRES = 0x1, 0x2, 0x4, 0x8, 0x10, 0x20, 0x40
+# faces (W,E,S,N,B,T bits 0..5) of the reference's 26 lat_bc sets, in the order
+# of its Bdry_Type enum (header/Lat_Manager.hpp:205-233)
+SET_FACES = [[0], [1], [2], [3], [4], [5], [1, 3], [0, 3], [0, 2], [1, 2], [3, 5], [2, 5], [3, 4], [2, 4],
+             [1, 5], [0, 5], [1, 4], [0, 4], [1, 3, 5], [0, 3, 5], [0, 2, 5], [1, 2, 5], [1, 3, 4], [0, 3, 4],
+             [0, 2, 4], [1, 2, 4]]
+MASK_TO_SET = {sum(1 << f for f in fs): i for i, fs in enumerate(SET_FACES)}
+C_DX = 0.1  # background spacing of the reference's sphere configuration (header/Constants.h:193)
+
+STEP, SEAM, SNAP, RESET_FORCE = 0, 1, 2, 3
+MACROS, COLLIDE, STREAM, EXPLODE, COALESCE, BC, TREAT = range(7)
+
+
+def available():
+    return os.path.exists(REF_LIB)
+
+
+def build():
+    """Builds the library when the reference's sources are present."""
+    if os.path.isdir(REF_SRC):
+        subprocess.check_call(["make", "-s", "-j8", "-f", os.path.join(HERE, "Makefile.ref"), f"REF={REF_SRC}"])
+
+
+def run(case, state, ops, ib=1, timeout=600):
+    """Runs ops on the reference from `state` (per level: dict f, fcol, rho, u over
+    the level's keys; None keeps the reference's own initial state).  Returns (params, snapshots): params holds "tau" (per level)
+    and "bc_u" (the six face strategies' wall velocity); each snapshot is a dict
+    {"levels": [per level {f, fcol, rho, u}], "force": [Fx, Fy, Fz, count]}."""
+    if case["Q"] != 19 or case["Lmax"] != 2:
+        raise ValueError("the compiled reference runs D3Q19 cases with three levels")
+    data = {"dis_keys": np.asarray(case["dis_keys"], np.uint64),
+            "dis_vals": np.asarray(case["dis_vals"], np.float64),
+            "ops": np.asarray(ops, np.int64).reshape(-1, 3), "ib": np.int64(ib)}
+    for L, lv in enumerate(case["levels"]):
+        data[f"keys{L}"] = np.asarray(lv["keys"], np.uint64)
+        data[f"kinds{L}"] = np.asarray(lv["kinds"], np.uint16)
+        if state is not None:
+            for k in ("f", "fcol", "rho", "u"):
+                data[f"{k}{L}"] = np.asarray(state[L][k], np.float64)
+    with tempfile.TemporaryDirectory() as tmp:
+        src, dst = os.path.join(tmp, "in.npz"), os.path.join(tmp, "out.npz")
+        np.savez(src, **data)
+        p = subprocess.run([sys.executable, os.path.abspath(__file__), src, dst], stdout=subprocess.DEVNULL,
+                           stderr=subprocess.PIPE, timeout=timeout, cwd=tmp)
+        if p.returncode != 0:
+            raise RuntimeError(f"reference run failed ({p.returncode}): {p.stderr.decode()[-2000:]}")
+        out = np.load(dst)
+        tau = out["tau"]
+        bc_u = out["bc_u"]
+        snaps = []
+        for s in range(int(out["n_snap"])):
+            levels = [{k: out[f"s{s}_{k}{L}"] for k in ("f", "fcol", "rho", "u")} for L in range(3)]
+            snaps.append({"levels": levels, "force": out[f"s{s}_force"]})
+    return {"tau": tau, "bc_u": bc_u}, snaps
+
+
+# ---- child side ----------------------------------------------------------------
+def _child(src, dst):
+    lib = C.CDLL(REF_LIB, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+    dp, u64p = C.POINTER(C.c_double), C.POINTER(C.c_uint64)
+    lib.ref_add.argtypes = [C.c_int, C.c_int, u64p, C.c_int64]
+    lib.ref_add_dis.argtypes = [u64p, dp, C.c_int64]
+    lib.ref_set_dx.argtypes = [dp]
+    lib.ref_construct.argtypes = [C.c_int]
+    lib.ref_tau.argtypes = [dp]
+    lib.ref_get.argtypes = [C.c_int, C.c_int, u64p, C.c_int64, dp]
+    lib.ref_set.argtypes = [C.c_int, C.c_int, u64p, C.c_int64, dp]
+    lib.ref_step.argtypes = [C.c_int]
+    lib.ref_call.argtypes = [C.c_int, C.c_int]
+    lib.ref_force.argtypes = [dp, C.c_int]
+    lib.ref_bc_velocity.argtypes = [dp]
+    assert lib.ref_q() == 19 and lib.ref_lmax() == 2
+    d = np.load(src)
+
+    def kp(a):
+        return a.ctypes.data_as(u64p)
+
+    def fp(a):
+        return a.ctypes.data_as(dp)
+
+    keys = [np.ascontiguousarray(d[f"keys{L}"]) for L in range(3)]
+    for L in range(3):
+        kinds = d[f"kinds{L}"].astype(np.int64)
+        for bit, sid in ((K_FLUID, 0), (K_F2C, 1), (K_C2F, 2), (K_GHOST_C2F, 3), (K_GHOST_F2C, 4), (K_SOLID, 5)):
+            sel = np.ascontiguousarray(keys[L][(kinds & bit) != 0])
+            if sel.size:
+                assert lib.ref_add(sid, L, kp(sel), sel.size) == 0
+        if L == 0:
+            res = (kinds & K_BCRES) != 0
+            masks = (kinds >> 8) & 0x3F
+            for m in np.unique(masks[res]):
+                sel = np.ascontiguousarray(keys[0][res & (masks == m)])
+                assert lib.ref_add(6 + MASK_TO_SET[int(m)], 0, kp(sel), sel.size) == 0
+    dk = np.ascontiguousarray(d["dis_keys"])
+    dv = np.ascontiguousarray(d["dis_vals"])
+    if dk.size:
+        lib.ref_add_dis(kp(dk), fp(dv), dk.size)
+    dx = np.array([C_DX / (1 << L) for L in range(3)])
+    lib.ref_set_dx(fp(dx))
+    assert lib.ref_construct(int(d["ib"])) == 0
+    tau = np.zeros(3)
+    lib.ref_tau(fp(tau))
+    bcu = np.zeros(18)
+    assert lib.ref_bc_velocity(fp(bcu)) == 0
+    for L in range(3 if "f0" in d.files else 0):
+        n = keys[L].size
+        for what, name in ((0, "f"), (1, "fcol"), (2, "rho"), (3, "u")):
+            v = np.ascontiguousarray(d[f"{name}{L}"], np.float64)
+            if what >= 2:  # only cells that carry macros in the reference hold them
+                if what == 2:
+                    ok = ~np.isnan(v)
+                else:
+                    ok = ~np.isnan(v).any(axis=0)
+                kk = np.ascontiguousarray(keys[L][ok])
+                vv = np.ascontiguousarray(v[..., ok])
+                lib.ref_set(L, what, kp(kk), kk.size, fp(vv))
+            else:
+                lib.ref_set(L, what, kp(keys[L]), n, fp(v))
+    out = {"tau": tau, "bc_u": bcu.reshape(6, 3)}
+    ns = 0
+    for code, a, b in d["ops"]:
+        if code == STEP:
+            assert lib.ref_step(int(a)) == 0
+        elif code == SEAM:
+            assert lib.ref_call(int(a), int(b)) == 0
+        elif code == SNAP:
+            for L in range(3):
+                n = keys[L].size
+                for what, name, rows in ((0, "f", 19), (1, "fcol", 19), (2, "rho", 1), (3, "u", 3)):
+                    buf = np.zeros(rows * n)
+                    lib.ref_get(L, what, kp(keys[L]), n, fp(buf))
+                    out[f"s{ns}_{name}{L}"] = buf.reshape(rows, n) if rows > 1 else buf
+            F = np.zeros(4)
+            lib.ref_force(fp(F), 0)
+            out[f"s{ns}_force"] = F
+            ns += 1
+        elif code == RESET_FORCE:
+            F = np.zeros(4)
+            lib.ref_force(fp(F), 1)
+    out["n_snap"] = np.int64(ns)
+    np.savez(dst, **out)
+
+
+if __name__ == "__main__":
+    _child(sys.argv[1], sys.argv[2])
