@@ -148,6 +148,17 @@ int mt_case_level_info(mt_case* c, int level, int64_t* n_cells, int64_t* n_brick
  *      header/Grid_Class.h:116-137): slot index or -1 per key                   */
 int mt_find(mt_case* c, int level, const uint64_t* keys, int64_t n, int32_t* slot_out);
 
+/* ---- mt_find without any allocation.  No counterpart in the reference: for
+ *      callers that cannot allocate device memory (code under HIP graph capture,
+ *      tight serving loops, harnesses that forbid hipMalloc).  Same inputs, output
+ *      and errors as mt_find; it stages the keys and slots in a fixed per-GPU
+ *      device array of MT_FIND_NOALLOC_MAX_KEYS keys (3 MiB, held until the
+ *      process ends) and returns MT_ERR_ARG for larger n.  Every call on a GPU
+ *      shares that array: calls that overlap on one GPU (other threads, other
+ *      cases' streams) race and must be serialised by the caller.              */
+#define MT_FIND_NOALLOC_MAX_KEYS (1 << 18)
+int mt_find_noalloc(mt_case* c, int level, const uint64_t* keys, int64_t n, int32_t* slot_out);
+
 /* ---- state access in the caller's key order (user[L].df / density / velocity,
  *      header/user.h:18-55,154-167); MT_ERR_LOOKUP if a key is absent          */
 int mt_get_field(mt_case* c, int level, int field, const uint64_t* keys, int64_t n, double* out);

@@ -47,7 +47,7 @@ class MemberDesc(C.Structure):
 
 EXPORTS = [
     "mt_last_error", "mt_case_create", "mt_case_destroy", "mt_case_upload_level", "mt_case_upload_dis",
-    "mt_case_finalize", "mt_case_level_info", "mt_find", "mt_get_field", "mt_set_field", "mt_macros",
+    "mt_case_finalize", "mt_case_level_info", "mt_find", "mt_find_noalloc", "mt_get_field", "mt_set_field", "mt_macros",
     "mt_collide", "mt_stream", "mt_explode", "mt_coalesce", "mt_apply_bc", "mt_ib_treat", "mt_step",
     "mt_set_graphs", "mt_set_schedule", "mt_set_timing", "mt_get_timing", "mt_sync", "mt_reduce", "mt_ib_force",
     "mt_case_set_members", "mt_ib_force_members", "mt_autotune", "mt_case_copy_member",
@@ -56,6 +56,10 @@ EXPORTS = [
 ]
 
 _lib = None
+
+
+# MT_FIND_NOALLOC_MAX_KEYS of include/mt_amr.h: the largest call of mt_find_noalloc
+FIND_NOALLOC_MAX_KEYS = 1 << 18
 
 
 class MtError(RuntimeError):
@@ -78,6 +82,7 @@ def load():
     L.mt_case_finalize.argtypes = [C.c_void_p]
     L.mt_case_level_info.argtypes = [C.c_void_p, C.c_int, _i64p, _i64p]
     L.mt_find.argtypes = [C.c_void_p, C.c_int, _u64p, C.c_int64, _i32p]
+    L.mt_find_noalloc.argtypes = [C.c_void_p, C.c_int, _u64p, C.c_int64, _i32p]
     L.mt_get_field.argtypes = [C.c_void_p, C.c_int, C.c_int, _u64p, C.c_int64, _dp]
     L.mt_set_field.argtypes = [C.c_void_p, C.c_int, C.c_int, _u64p, C.c_int64, _dp]
     for n in ("mt_macros", "mt_collide", "mt_stream", "mt_explode", "mt_coalesce"):
@@ -201,6 +206,13 @@ class AmrCase:
         keys, pk = _u64(keys)
         out = np.empty(keys.size, dtype=np.int32)
         _check(load().mt_find(self.h, level, pk, keys.size, out.ctypes.data_as(_i32p)))
+        return out
+
+    def find_noalloc(self, level, keys):
+        """mt_find through the allocation-free entry point (fixed per-GPU scratch)."""
+        keys, pk = _u64(keys)
+        out = np.empty(keys.size, dtype=np.int32)
+        _check(load().mt_find_noalloc(self.h, level, pk, keys.size, out.ctypes.data_as(_i32p)))
         return out
 
     # ---- fields

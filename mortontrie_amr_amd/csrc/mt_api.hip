@@ -385,6 +385,43 @@ static int drain_timing(Case& c) {
   return MT_OK;
 }
 
+// ---------------------------------------------------------------- allocation-free lookup
+// Device memory of mt_find_noalloc: the query keys and their slots for up to
+// MT_FIND_NOALLOC_MAX_KEYS keys, keys first (8 B each), slots after them (4 B each).
+// Each GPU holds its own copy until the process ends.  In host code the name is
+// only a stand-in for the symbol: its device address comes from
+// hipGetSymbolAddress, once per device, and only that address is used.
+static __device__ __align__(256) unsigned char scratch[MT_FIND_NOALLOC_MAX_KEYS * 12];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// the device address of `scratch` on the device `stream` runs on
+int scratch_for_stream(hipStream_t stream, unsigned char** out) {
+  int dev = -1;
+  MT_HIP(hipStreamGetDevice(stream, &dev));
+  if (dev < 0 || dev >= kMaxDevices) { set_error("device index out of range"); return MT_ERR_ARG; }
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = -1;
+    MT_HIP(hipGetDevice(&cur));
+    if (cur != dev) MT_HIP(hipSetDevice(dev));
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p) {
+      set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return MT_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return MT_OK;
+}
+
 }  // namespace mt
 
 using namespace mt;
@@ -628,6 +665,29 @@ int mt_find(mt_case* c, int level, const uint64_t* keys, int64_t n, int32_t* slo
   dev_free(dk, c->stream);
   dev_free(ds, c->stream);
   return r;
+}
+
+int mt_find_noalloc(mt_case* c, int level, const uint64_t* keys, int64_t n, int32_t* slot_out) {
+  CHECK_FINAL(c);
+  CHECK_LEVEL(c, level);
+  if (n < 0) { set_error("mt_find_noalloc: negative key count"); return MT_ERR_ARG; }
+  if (n > MT_FIND_NOALLOC_MAX_KEYS) {
+    set_error("mt_find_noalloc: " + std::to_string(n) + " keys exceed its scratch capacity of " +
+              std::to_string(MT_FIND_NOALLOC_MAX_KEYS) + " keys (use mt_find)");
+    return MT_ERR_ARG;
+  }
+  if (n == 0) return MT_OK;
+  if (!keys || !slot_out) { set_error("mt_find_noalloc: null keys or output"); return MT_ERR_ARG; }
+  unsigned char* base = nullptr;
+  TRY(scratch_for_stream(c->stream, &base));
+  uint64_t* dk = reinterpret_cast<uint64_t*>(base);
+  int32_t* ds = reinterpret_cast<int32_t*>(base + 8 * (size_t)MT_FIND_NOALLOC_MAX_KEYS);
+  // every slot the lookup returns is written by it (found or -1): nothing to clear
+  MT_HIP(hipMemcpyAsync(dk, keys, 8 * n, hipMemcpyHostToDevice, c->stream));
+  TRY(find_slots(*c, level, dk, n, ds));
+  MT_HIP(hipMemcpyAsync(slot_out, ds, 4 * n, hipMemcpyDeviceToHost, c->stream));
+  MT_HIP(hipStreamSynchronize(c->stream));
+  return MT_OK;
 }
 
 static int field_io(mt_case* c, int level, int field, const uint64_t* keys, int64_t n, double* out,
