@@ -222,6 +222,11 @@ int mt_sync(mt_case* c);
 /* mass and momentum over lat_f and F2C of one level, and a NaN count       */
 int mt_reduce(mt_case* c, int level, double* mass, double* momentum3, int64_t* nan_count);
 /* accumulated IB force since the last reset (LBM_Manager.cpp:846-852, 964-979) */
+/* ---- per-member failure scan: the NaN checks of LBMkernel (LBM_Simulator.cpp:33-41,
+ *      52-60, 78-86, 102-111, 128-137) reduced on the device per member of a packed
+ *      case: bad_cells[i] = fluid and F2C cells of member i, all levels, holding a
+ *      non-finite population.  n_members must equal the case's member count.    */
+int mt_scan_members(mt_case* c, int32_t n_members, int64_t* bad_cells);
 int mt_ib_force(mt_case* c, double* sum3, int64_t* count, int reset);
 /* the same per member: sum3[m][3] over member m's cells, fixed order          */
 int mt_ib_force_members(mt_case* c, double* sum3, int32_t n_members, int64_t* count, int reset);
@@ -245,17 +250,23 @@ int mt_case_copy_member(mt_case* dst, int32_t dst_member, mt_case* src, int32_t 
  * fits `hbm_budget` bytes.  A request finishes at max_steps or when its drag
  * coefficient over the last chunk moved by less than drag_tol from the
  * previous chunk's; Cd = 2 Fx / (U^2 pi d^2 / 4) with U = cd_U, d = cd_d, and
- * each request computes exactly what it computes alone.  mt_server_poll hands
- * out finished requests (id, steps, last chunk's Cd) in completion order.  A
- * repack that fails stops the server: later calls return MT_ERR_STATE.      */
+ * each request computes exactly what it computes alone.  After every chunk each
+ * member is scanned for non-finite populations (mt_scan_members): a member with
+ * any, or with a non-finite drag, ends with status MT_REQ_DIVERGED and leaves
+ * the batch at the next step; the others go on unchanged.  mt_server_poll hands
+ * out finished requests (id, steps, last chunk's Cd, status) in completion
+ * order.  A repack or a step that fails stops the server: later calls return
+ * MT_ERR_STATE.                                                             */
+#define MT_REQ_FINISHED 0 /* max_steps reached or the drag settled */
+#define MT_REQ_DIVERGED 1 /* non-finite state or drag: evicted */
 typedef struct mt_server mt_server;
 int mt_server_create(int32_t precision, int32_t chunk_steps, int32_t admit_every, int64_t hbm_budget,
                      mt_server** out);
 int mt_server_submit(mt_server* s, mt_case* request, int64_t id, int32_t max_steps, double drag_tol,
                      double cd_U, double cd_d);
 int mt_server_step(mt_server* s, int32_t* n_running, int32_t* n_waiting);
-int mt_server_poll(mt_server* s, int64_t* ids, int32_t* steps, double* cd_last, int32_t max_results,
-                   int32_t* n_results);
+int mt_server_poll(mt_server* s, int64_t* ids, int32_t* steps, double* cd_last, int32_t* status,
+                   int32_t max_results, int32_t* n_results);
 int mt_server_destroy(mt_server* s);
 
 #ifdef __cplusplus

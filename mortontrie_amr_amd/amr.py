@@ -51,7 +51,8 @@ EXPORTS = [
     "mt_collide", "mt_stream", "mt_explode", "mt_coalesce", "mt_apply_bc", "mt_ib_treat", "mt_step",
     "mt_set_graphs", "mt_set_schedule", "mt_set_timing", "mt_get_timing", "mt_sync", "mt_reduce", "mt_ib_force",
     "mt_case_set_members", "mt_ib_force_members", "mt_autotune", "mt_case_copy_member",
-    "mt_case_upload_markers", "mt_server_create", "mt_server_submit", "mt_server_step", "mt_server_poll",
+    "mt_case_upload_markers", "mt_scan_members", "mt_server_create", "mt_server_submit", "mt_server_step",
+    "mt_server_poll",
     "mt_server_destroy",
 ]
 
@@ -106,8 +107,9 @@ def load():
     L.mt_server_submit.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_double,
                                    C.c_double]
     L.mt_server_step.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
-    L.mt_server_poll.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), _dp, C.c_int32,
-                                 C.POINTER(C.c_int32)]
+    L.mt_server_poll.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), _dp,
+                                 C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]
+    L.mt_scan_members.argtypes = [C.c_void_p, C.c_int32, _i64p]
     L.mt_server_destroy.argtypes = [C.c_void_p]
     _lib = L
     return L
@@ -325,6 +327,13 @@ class AmrCase:
         n = C.c_int64()
         _check(load().mt_ib_force_members(self.h, s.ctypes.data_as(_dp), nm, C.byref(n), 1 if reset else 0))
         return s, n.value
+
+    def scan_members(self):
+        """Per member: fluid/F2C cells (all levels) holding a non-finite population (mt_scan_members)."""
+        nm = len(self.case.get("members") or [0])
+        out = np.zeros(nm, dtype=np.int64)
+        _check(load().mt_scan_members(self.h, nm, out.ctypes.data_as(_i64p)))
+        return out
 
     def drag_coefficients(self, reset=True):
         """Cd of every member of a packed case (nan for members without a solid boundary)."""

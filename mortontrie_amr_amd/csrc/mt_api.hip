@@ -342,22 +342,42 @@ static int coarse_steps(Case& c, int k, bool last) {
   const int key = parity_key(c, last) | (k << 24);
   auto it = c.graph_exec.find(key);
   int pf[kMaxLevels], pr[kMaxLevels];
-  for (int L = 0; L <= c.Lmax; ++L) { pf[L] = c.lv[L].pf; pr[L] = c.lv[L].pr; }
+  bool jp[kMaxLevels];
+  for (int L = 0; L <= c.Lmax; ++L) { pf[L] = c.lv[L].pf; pr[L] = c.lv[L].pr; jp[L] = c.join_pending[L]; }
+  const bool xprimed = c.xprimed, ahead = c.ahead;
+  // capture records the schedule without running it: the host-side state the
+  // recording advanced goes back to where the device still is, also on failure
+  auto restore = [&]() {
+    for (int L = 0; L <= c.Lmax; ++L) { c.lv[L].pf = pf[L]; c.lv[L].pr = pr[L]; c.join_pending[L] = jp[L]; }
+    c.xprimed = xprimed;
+    c.ahead = ahead;
+  };
   if (it == c.graph_exec.end()) {
-    hipGraph_t g;
+    hipGraph_t g = nullptr;
     MT_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     int r = MT_OK;
     for (int j = 0; j < k && r == MT_OK; ++j) r = one_step(c, last && j == k - 1);
     if (r == MT_OK) r = join_all(c);
     hipError_t e = hipStreamEndCapture(c.stream, &g);
-    if (r) return r;
-    if (e != hipSuccess) { set_error(std::string("graph capture: ") + hipGetErrorString(e)); return MT_ERR_HIP; }
+    const bool xprimed_after = c.xprimed, ahead_after = c.ahead;
+    bool jp_after[kMaxLevels];
+    for (int L = 0; L <= c.Lmax; ++L) jp_after[L] = c.join_pending[L];
+    restore();
+    if (r != MT_OK || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      if (r == MT_OK) { set_error(std::string("graph capture: ") + hipGetErrorString(e)); r = MT_ERR_HIP; }
+      return r;
+    }
     hipGraphExec_t ex;
-    MT_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
-    MT_HIP(hipGraphDestroy(g));
+    hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) { set_error(std::string("graph instantiate: ") + hipGetErrorString(ei)); return MT_ERR_HIP; }
     c.graph_exec[key] = ex;
     it = c.graph_exec.find(key);
-    for (int L = 0; L <= c.Lmax; ++L) { c.lv[L].pf = pf[L]; c.lv[L].pr = pr[L]; }
+    // the replay below leaves the flags as the recording did
+    c.xprimed = xprimed_after;
+    c.ahead = ahead_after;
+    for (int L = 0; L <= c.Lmax; ++L) c.join_pending[L] = jp_after[L];
   }
   MT_HIP(hipGraphLaunch(it->second, c.stream));
   // parities advance as in the captured schedule: per coarse step level L
@@ -517,12 +537,20 @@ int mt_case_upload_level(mt_case* c, int level, const uint64_t* keys, const uint
   // an interface cell is either on the coarse side (F2C, ghost_F2C) or on the
   // fine side (C2F, ghost_C2F) of a level boundary, never both
   const uint16_t coarse_side = MT_KIND_F2C | MT_KIND_GHOST_F2C, fine_side = MT_KIND_C2F | MT_KIND_GHOST_C2F;
-  for (int64_t i = 0; i < n; ++i)
+  for (int64_t i = 0; i < n; ++i) {
     if ((kinds[i] & coarse_side) && (kinds[i] & fine_side)) {
       set_error("level " + std::to_string(level) + ": cell " + std::to_string(i) +
                 " is on both sides of a level interface (refinement shells too close)");
       return MT_ERR_ARG;
     }
+    // F2C cells are taken out of lat_f (Lat_Voxelize.cpp:628-632) and C2F cells are
+    // never in it: a fluid cell of either kind would be updated by two levels
+    if ((kinds[i] & MT_KIND_FLUID) && (kinds[i] & (MT_KIND_F2C | MT_KIND_C2F))) {
+      set_error("level " + std::to_string(level) + ": cell " + std::to_string(i) +
+                " is both a fluid cell and an interface (F2C / C2F) cell");
+      return MT_ERR_ARG;
+    }
+  }
   DevLevel& l = c->lv[level];
   l.h_keys.assign(keys, keys + n);
   l.h_kinds.assign(kinds, kinds + n);
@@ -625,7 +653,8 @@ int mt_case_finalize(mt_case* c) {
   c->nee_tmp = c->alloc<double>((int64_t)c->Q * std::max<int64_t>(1, c->lv[0].n_bcres));
   c->red_cap = 5 * 1024;
   c->red_tmp = c->alloc<double>(c->red_cap);
-  if (!c->force_part || !c->force_acc || !c->force_slots || !c->nee_tmp || !c->red_tmp) { set_error("out of device memory"); return MT_ERR_HIP; }
+  c->member_bad = c->alloc<unsigned long long>((int64_t)c->h_mem.size());
+  if (!c->force_part || !c->force_acc || !c->force_slots || !c->nee_tmp || !c->red_tmp || !c->member_bad) { set_error("out of device memory"); return MT_ERR_HIP; }
   MT_HIP(hipMemsetAsync(c->force_acc, 0, 8 * sizeof(double), c->stream));
   MT_HIP(hipMemsetAsync(c->force_slots, 0, 3 * max_fp * sizeof(double), c->stream));
   TRY(init_state(*c));
@@ -954,6 +983,16 @@ int mt_reduce(mt_case* c, int level, double* mass, double* momentum3, int64_t* n
   }
   if (nan_count) *nan_count = (int64_t)out[4];
   return MT_OK;
+}
+
+int mt_scan_members(mt_case* c, int32_t n_members, int64_t* bad_cells) {
+  CHECK_FINAL(c);
+  if (n_members != (int32_t)c->h_mem.size() || !bad_cells) {
+    set_error("mt_scan_members: n_members must be the case's member count and the output non-null");
+    return MT_ERR_ARG;
+  }
+  TRY(settle(*c));
+  return launch_member_scan(*c, bad_cells);
 }
 
 int mt_ib_force(mt_case* c, double* sum3, int64_t* count, int reset) {

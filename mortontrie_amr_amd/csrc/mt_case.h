@@ -171,6 +171,7 @@ struct Case {
   double* nee_tmp = nullptr;     // NEE Jacobi staging
   double* red_tmp = nullptr;     // reduction partials
   int64_t red_cap = 0;
+  unsigned long long* member_bad = nullptr;  // [members] cells with a non-finite population
   // graphs: key = (start parity of level 0) | (final << 1)
   std::map<int, hipGraphExec_t> graph_exec;
   // timing
@@ -253,6 +254,9 @@ int launch_markers(Case& c, int buf, hipStream_t st, bool moments_done = false);
 int launch_gather(Case& c, int L, int field, const int32_t* d_slots, int64_t n, double* d_out, int buf);
 int launch_scatter(Case& c, int L, int field, const int32_t* d_slots, int64_t n, const double* d_in, int buf);
 int launch_reduce(Case& c, int L, double* h_out);
+// per member: fluid and F2C cells of any level holding a non-finite population
+// in either fcol buffer
+int launch_member_scan(Case& c, int64_t* h_counts);
 
 }  // namespace mt
 

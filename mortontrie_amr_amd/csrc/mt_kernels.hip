@@ -1226,6 +1226,33 @@ __global__ __launch_bounds__(256) void k_reduce(LvArgs v, double* part) {
     }
 }
 
+// The NaN scans of LBMkernel (LBM_Simulator.cpp:33-41, 52-60, 78-86, 102-111,
+// 128-137), per member of a packed case: counts the fluid and F2C cells holding a
+// non-finite population in either fcol buffer.  One brick per wave: the wave's
+// ballot is added to its member's counter by one lane.
+template <int Q, typename S>
+__global__ __launch_bounds__(256) void k_member_scan(int64_t ns, const uint16_t* __restrict__ kind,
+                                                     const int32_t* __restrict__ bmember, const S* __restrict__ fa,
+                                                     const S* __restrict__ fb, unsigned long long* counts) {
+  typedef typename RealOf<S>::type R;
+  const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  bool bad = false;
+  if (s < ns && (kind[s] & (MT_KIND_FLUID | MT_KIND_F2C))) {
+    for (int q = 0; q < Q; ++q) {
+      const R a = load_pop<Q, S, R>(fa, ns, q, (int)s);
+      const R b = load_pop<Q, S, R>(fb, ns, q, (int)s);
+      bad |= !isfinite(a) || !isfinite(b);
+    }
+  }
+  const unsigned long long m = __ballot(bad);
+  if ((threadIdx.x & 63) == 0 && s < ns && m) atomicAdd(&counts[bmember[s >> 6]], (unsigned long long)__popcll(m));
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 // ================================================================== launchers
 static inline int64_t nblk(int64_t n, int b = 256) { return n <= 0 ? 1 : (n + b - 1) / b; }
 
@@ -1894,6 +1921,23 @@ int launch_scatter(Case& c, int L, int field, const int32_t* d_slots, int64_t n,
   MT_HIP(hipGetLastError());
   return MT_OK;
 }
+int launch_member_scan(Case& c, int64_t* h_counts) {
+  const int64_t nm = (int64_t)c.h_mem.size();
+  k_fill_u64<<<nblk(nm), 256, 0, c.stream>>>(c.member_bad, nm);
+  MT_HIP(hipGetLastError());
+  for (int L = 0; L <= c.Lmax; ++L) {
+    DevLevel& l = c.lv[L];
+    MT_DISPATCH(c, (k_member_scan<Q_, S_><<<nblk(l.n_slots), 256, 0, c.stream>>>(
+                       l.n_slots, l.kind, l.bmember, (const S_*)l.fcol[0], (const S_*)l.fcol[1], c.member_bad)));
+    MT_HIP(hipGetLastError());
+  }
+  std::vector<unsigned long long> h(nm);
+  MT_HIP(hipMemcpyAsync(h.data(), c.member_bad, sizeof(unsigned long long) * nm, hipMemcpyDeviceToHost, c.stream));
+  MT_HIP(hipStreamSynchronize(c.stream));
+  for (int64_t i = 0; i < nm; ++i) h_counts[i] = (int64_t)h[i];
+  return MT_OK;
+}
+
 int launch_reduce(Case& c, int L, double* h_out) {
   DevLevel& l = c.lv[L];
   LvArgs v = lvargs(c, L, 0, 0, 0, 0);

@@ -16,7 +16,12 @@
 //     when its drag moved by less than the tolerance since the previous chunk;
 //     finished members keep stepping with the batch until the next repack
 //     (their results are final) and wait in the results queue for
-//     mt_server_poll.
+//     mt_server_poll;
+//   * after every chunk the device scans each member for non-finite
+//     populations (mt_scan_members, the reference's NaN checks per member); a
+//     member that has any, or whose drag is not finite, finishes at once with
+//     the status MT_REQ_DIVERGED and is evicted at the next step's repack, which
+//     is then due whatever the admission interval.
 #include <algorithm>
 #include <cmath>
 #include <deque>
@@ -72,6 +77,7 @@ struct Request {
   int32_t steps = 0;
   std::vector<double> cd;
   bool done = false;
+  int32_t status = MT_REQ_FINISHED;
 };
 
 }  // namespace
@@ -86,7 +92,8 @@ struct Server {
   std::vector<std::unique_ptr<Request>> results;
   mt_case* batch = nullptr;
   int since = 1 << 30;
-  std::string broken;  // set when a repack failed; every later call reports it
+  bool evict = false;  // a member diverged: repack at the next step
+  std::string broken;  // set when a repack or a step failed; every later call reports it
 };
 
 }  // namespace mt
@@ -296,7 +303,8 @@ int mt_server_step(mt_server* s, int32_t* n_running, int32_t* n_waiting) {
       used += r->footprint;
     }
   const bool due = !s->waiting.empty() && (s->since >= s->admit_every || active == 0);
-  if (due || (!s->running.empty() && active == 0)) {
+  if (due || s->evict || (!s->running.empty() && active == 0)) {
+    s->evict = false;
     std::vector<Request*> old;
     for (auto& r : s->running) old.push_back(r.get());
     std::vector<std::unique_ptr<Request>> next;
@@ -321,12 +329,23 @@ int mt_server_step(mt_server* s, int32_t* n_running, int32_t* n_waiting) {
     }
   }
   if (s->batch) {
-    TRYS(mt_step(s->batch, s->chunk));
-    s->since += 1;
     const int32_t nm = (int32_t)s->running.size();
     std::vector<double> sums(3 * (size_t)nm);
+    std::vector<int64_t> bad(nm);
     int64_t calls = 0;
-    TRYS(mt_ib_force_members(s->batch, sums.data(), nm, &calls, 1));
+    int rc = mt_step(s->batch, s->chunk);
+    if (rc == MT_OK) rc = mt_ib_force_members(s->batch, sums.data(), nm, &calls, 1);
+    if (rc == MT_OK) rc = mt_scan_members(s->batch, nm, bad.data());
+    if (rc != MT_OK) {
+      // the batch's state is unknown after a failed step: the server stops
+      s->broken = std::string("server stopped: a step failed: ") + mt_last_error();
+      s->running.clear();
+      mt_case_destroy(s->batch);
+      s->batch = nullptr;
+      set_error(s->broken);
+      return rc;
+    }
+    s->since += 1;
     for (int32_t i = 0; i < nm; ++i) {
       Request& r = *s->running[i];
       if (r.done) continue;
@@ -334,12 +353,16 @@ int mt_server_step(mt_server* s, int32_t* n_running, int32_t* n_waiting) {
       const double fx = sums[3 * i] / (double)std::max<int64_t>(calls, 1);
       r.cd.push_back(2.0 * fx / (r.cd_U * r.cd_U * M_PI * r.cd_d * r.cd_d / 4.0));
       const size_t k = r.cd.size();
-      if (r.steps >= r.max_steps || (k >= 2 && std::fabs(r.cd[k - 1] - r.cd[k - 2]) < r.tol)) {
+      const bool diverged = bad[i] > 0 || !std::isfinite(r.cd[k - 1]);
+      if (diverged || r.steps >= r.max_steps || (k >= 2 && std::fabs(r.cd[k - 1] - r.cd[k - 2]) < r.tol)) {
         r.done = true;
+        r.status = diverged ? MT_REQ_DIVERGED : MT_REQ_FINISHED;
+        if (diverged) s->evict = true;
         auto res = std::make_unique<Request>();
         res->id = r.id;
         res->steps = r.steps;
         res->cd = r.cd;
+        res->status = r.status;
         s->results.push_back(std::move(res));
       }
     }
@@ -352,8 +375,8 @@ int mt_server_step(mt_server* s, int32_t* n_running, int32_t* n_waiting) {
   return MT_OK;
 }
 
-int mt_server_poll(mt_server* s, int64_t* ids, int32_t* steps, double* cd_last, int32_t max_results,
-                   int32_t* n_results) {
+int mt_server_poll(mt_server* s, int64_t* ids, int32_t* steps, double* cd_last, int32_t* status,
+                   int32_t max_results, int32_t* n_results) {
   if (!s || !n_results) { set_error("null server or count"); return MT_ERR_ARG; }
   int32_t n = 0;
   while (n < max_results && !s->results.empty()) {
@@ -361,6 +384,7 @@ int mt_server_poll(mt_server* s, int64_t* ids, int32_t* steps, double* cd_last, 
     if (ids) ids[n] = r.id;
     if (steps) steps[n] = r.steps;
     if (cd_last) cd_last[n] = r.cd.empty() ? NAN : r.cd.back();
+    if (status) status[n] = r.status;
     s->results.erase(s->results.begin());
     ++n;
   }

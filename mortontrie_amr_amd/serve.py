@@ -12,6 +12,10 @@ chunks of `chunk` coarse steps; between chunks it
     member's state over on the device (mt_case_copy_member), so a member
     computes exactly what it would compute alone.  Between repacks a finished
     member keeps stepping with the batch; its result is already final.
+  * scans every member for non-finite populations on the device
+    (mt_scan_members, the reference's NaN checks per member); a member that has
+    any, or whose drag is not finite, finishes as diverged (`failed`) and is
+    evicted by a repack at the next chunk, whatever the admission interval.
 Latency is completion time minus arrival time, on the server's wall clock.
 """
 from __future__ import annotations
@@ -42,6 +46,7 @@ class Request:
         self.cd = []
         self.admitted = None
         self.done = None
+        self.failed = False
 
     def cd_of(self, fx_per_call):
         cd = self.case["cd"]
@@ -71,6 +76,7 @@ class Server:
         self.members = []      # running requests, in member order
         self.repacks = 0
         self.chunks = 0
+        self.repack_log = []   # (server time, member ids) of every repack
 
     def _now(self, t0):
         return self.vtime if self.chunk_seconds is not None else time.perf_counter() - t0
@@ -118,11 +124,13 @@ class Server:
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         running, done = [], []     # running: the batch's members, finished or not
         since = self.admit_every   # chunks since the last repack
+        evict = False              # a member diverged in the last chunk
         while pending or any(r.done is None for r in running):
             now = self._now(t0)
             active = [r for r in running if r.done is None]
             arrived = bool(pending) and pending[0].arrival <= now
-            if (arrived and (since >= self.admit_every or not active)) or (running and not active):
+            if (arrived and (since >= self.admit_every or not active)) or (running and not active) or evict:
+                evict = False
                 used = sum(footprint_bytes(r.case, self.precision) for r in active)
                 admitted = []
                 while pending and pending[0].arrival <= now:
@@ -136,6 +144,7 @@ class Server:
                 running = active + admitted
                 if running:
                     self._repack(running)
+                    self.repack_log.append((now, [r.id for r in running]))
                 elif self.batch is not None:
                     self.batch.close()
                     self.batch, self.members = None, []
@@ -146,6 +155,7 @@ class Server:
             self.batch.step(self.chunk)
             since += 1
             sums, n = self.batch.ib_force_members(reset=True)
+            bad = self.batch.scan_members()
             self.chunks += 1
             if self.chunk_seconds is not None:
                 self.vtime += self.chunk_seconds
@@ -155,7 +165,10 @@ class Server:
                     continue
                 r.steps += self.chunk
                 r.cd.append(r.cd_of(sums[i, 0] / max(n, 1)))
-                if r.finished():
+                if bad[i] > 0 or not np.isfinite(r.cd[-1]):
+                    r.failed = True
+                    evict = True
+                if r.failed or r.finished():
                     r.done = now
                     done.append(r)
         if self.batch is not None:
@@ -173,6 +186,9 @@ def run_alone(req, precision=amr.FP64, chunk=100):
         s, n = g.ib_force(reset=True)
         r.steps += chunk
         r.cd.append(r.cd_of(s[0] / max(n, 1)))
+        if g.scan_members()[0] > 0 or not np.isfinite(r.cd[-1]):
+            r.failed = True
+            break
     g.close()
     return r
 
@@ -185,6 +201,10 @@ def poisson_requests(n, rate=20.0, seed=1, max_steps=2000, tol=1e-4):
         t += float(rng.exponential(1.0 / rate))
         out.append(Request(i, C.sphere_request(rng), t, max_steps, tol))
     return out
+
+
+# request statuses of mt_server_poll (include/mt_amr.h)
+REQ_FINISHED, REQ_DIVERGED = 0, 1
 
 
 class NativeServer:
@@ -213,13 +233,15 @@ class NativeServer:
         return nr.value, nw.value
 
     def poll(self, max_results=256):
+        """Finished requests: (id, steps, last chunk's Cd, diverged) in completion order."""
         C = self._C
         ids = (C.c_int64 * max_results)()
         steps = (C.c_int32 * max_results)()
         cd = (C.c_double * max_results)()
+        st = (C.c_int32 * max_results)()
         n = C.c_int32()
-        amr._check(amr.load().mt_server_poll(self.h, ids, steps, cd, max_results, C.byref(n)))
-        return [(ids[i], steps[i], cd[i]) for i in range(n.value)]
+        amr._check(amr.load().mt_server_poll(self.h, ids, steps, cd, st, max_results, C.byref(n)))
+        return [(ids[i], steps[i], cd[i], st[i] == REQ_DIVERGED) for i in range(n.value)]
 
     def close(self):
         if getattr(self, "h", None) is not None and amr._lib is not None:

@@ -471,6 +471,9 @@ struct Case {
         }
       }
     }
+    init_tau();
+  }
+  void init_tau() {
     tau.resize(Lmax + 1);
     for (int L = 0; L <= Lmax; ++L) tau[L] = double(1 << L) * (tau0 - 0.5) + 0.5;
   }
@@ -895,6 +898,630 @@ struct Case {
   }
 };
 
+
+// ---------------------------------------------------------------- dense mode
+// The same algorithm on sorted arrays: every cell set of a level becomes a list
+// of indices into the level's keys in ascending key order (the order the map mode
+// walks), every map lookup becomes an index resolved once from the same Morton
+// arithmetic, and the per-cell arithmetic is the map mode's own functions.  Loops
+// whose cells do not interact (all but the NEE chain and the fixed-order force
+// sums) run in parallel; those two run in the map mode's order.  The result is
+// the map mode's bits (tests compare them) at a speed that reaches full-size runs.
+struct Dense {
+  struct Lev {
+    std::vector<uint64_t> keys;
+    std::unordered_map<uint64_t, int64_t> idx;
+    std::vector<uint16_t> kind;
+    std::vector<double> f, fcol, rho, u, v_old;  // f/fcol [Q][n], u/v_old [3][n]
+    std::vector<uint8_t> has_macro;
+    std::vector<int64_t> lat_f, f2c, c2f, bcres;
+    std::vector<int64_t> strm_cells;  // lat_f then c2f
+    std::vector<int64_t> strm_src;    // [cell][Q]
+    std::vector<int64_t> ex_parent, ex_child;  // coarse F2C parent -> 8 fine children (-1: skipped)
+    std::vector<int64_t> gx_parent, gx_child;  // coarse ghost-F2C parent -> 8 ghost children (-1: none)
+    std::vector<int64_t> co_parent, co_child;  // this level's F2C -> 8 children on the finer level
+    int64_t n = 0;
+  };
+  const Case* src = nullptr;
+  int Q = 19, Lmax = 0;
+  std::vector<Lev> lv;
+  // finest-level treated cells (ascending key order, lat_f members only)
+  std::vector<int64_t> tr_cell, tr_inc_src, tr_pull_src;  // tr_*_src [cell][Q]
+  std::vector<std::array<double, 26>> tr_dis;
+  // level-0 wet boundary cells and reservoir chains
+  std::vector<int64_t> wet_cell, wet_in;  // wet_in [cell][2]: x - n (one face) or x - sum n
+  std::vector<int> wet_act;
+  std::vector<std::vector<int64_t>> nee_cells, nee_nb;  // per face: cells, [cell][Q] neighbour or -1 (not a direction)
+  // markers
+  std::vector<int64_t> mk_idx;  // [n][27] index or -1
+  std::vector<int64_t> band_cell, band_ptr, band_mk;
+  std::vector<double> band_w;
+  std::vector<std::array<double, 3>> force_log;
+
+  int64_t at(int L, uint64_t k) const {
+    auto it = lv[L].idx.find(k);
+    return it == lv[L].idx.end() ? -1 : it->second;
+  }
+
+  // fresh: the initial state of Case::init (LBM_Manager.cpp:482-592) is set here and
+  // the case's maps are not read (they need not be allocated)
+  int build(const Case& c, bool fresh) {
+    src = &c;
+    Q = c.Q;
+    Lmax = c.Lmax;
+    lv.assign(Lmax + 1, {});
+    for (int L = 0; L <= Lmax; ++L) {
+      const Level& m = c.lv[L];
+      Lev& d = lv[L];
+      for (auto& kv : m.kind) d.keys.push_back(kv.first);
+      std::sort(d.keys.begin(), d.keys.end());
+      d.n = (int64_t)d.keys.size();
+      d.idx.reserve(d.n * 2);
+      for (int64_t i = 0; i < d.n; ++i) d.idx[d.keys[i]] = i;
+      d.kind.resize(d.n);
+      d.f.resize(Q * d.n);
+      d.fcol.resize(Q * d.n);
+      d.rho.assign(d.n, NAN);
+      d.u.assign(3 * d.n, NAN);
+      d.v_old.assign(3 * d.n, NAN);
+      d.has_macro.assign(d.n, 0);
+      double fe[27];
+      feq_c(Q, c.rho0, c.u0, fe);
+      for (int64_t i = 0; i < d.n && fresh; ++i) {
+        uint16_t kd = m.kind.at(d.keys[i]);
+        d.kind[i] = kd;
+        bool sol = kd & K_SOLID;
+        for (int q = 0; q < Q; ++q) d.f[q * d.n + i] = d.fcol[q * d.n + i] = sol ? 0.0 : fe[q];
+        if (kd & (K_FLUID | K_F2C | K_C2F | K_BCRES)) {
+          d.has_macro[i] = 1;
+          d.rho[i] = c.rho0;
+          for (int a = 0; a < 3; ++a) {
+            d.u[a * d.n + i] = c.u0[a];
+            d.v_old[a * d.n + i] = 0.0;
+          }
+        } else if (sol) {
+          d.has_macro[i] = 1;
+          d.rho[i] = 1.0;
+          for (int a = 0; a < 3; ++a) d.u[a * d.n + i] = d.v_old[a * d.n + i] = 0.0;
+        }
+      }
+      for (int64_t i = 0; i < d.n && !fresh; ++i) {
+        uint64_t k = d.keys[i];
+        d.kind[i] = m.kind.at(k);
+        for (int q = 0; q < Q; ++q) {
+          d.f[q * d.n + i] = m.f[q].at(k);
+          d.fcol[q * d.n + i] = m.fcol[q].at(k);
+        }
+        auto r = m.rho.find(k);
+        if (r != m.rho.end()) {
+          d.has_macro[i] = 1;
+          d.rho[i] = r->second;
+          auto& uu = m.u.at(k);
+          auto& vo = m.v_old.at(k);
+          for (int a = 0; a < 3; ++a) {
+            d.u[a * d.n + i] = uu[a];
+            d.v_old[a * d.n + i] = vo[a];
+          }
+        }
+      }
+      auto to_idx = [&](const std::vector<uint64_t>& ks, std::vector<int64_t>& out) {
+        out.clear();
+        for (uint64_t k : ks) out.push_back(d.idx.at(k));
+      };
+      to_idx(m.lat_f, d.lat_f);
+      to_idx(m.f2c, d.f2c);
+      to_idx(m.c2f, d.c2f);
+      to_idx(m.bcres, d.bcres);
+      d.strm_cells = d.lat_f;
+      d.strm_cells.insert(d.strm_cells.end(), d.c2f.begin(), d.c2f.end());
+      d.strm_src.assign(d.strm_cells.size() * Q, -1);
+      for (size_t j = 0; j < d.strm_cells.size(); ++j) {
+        uint64_t k = d.keys[d.strm_cells[j]];
+        for (int q = 1; q < Q; ++q) d.strm_src[j * Q + q] = at(L, c.nbr(k, L, EX[q], EY[q], EZ[q]));
+      }
+    }
+    // transfers
+    for (int L = 1; L <= Lmax; ++L) {
+      const Level& mc = c.lv[L - 1];
+      const Level& mf = c.lv[L];
+      Lev& dc = lv[L - 1];
+      Lev& df = lv[L];
+      uint64_t ch[8];
+      for (uint64_t k : mc.f2c) {
+        c.children(k, L, ch);
+        df.ex_parent.push_back(dc.idx.at(k));
+        for (int j = 0; j < 8; ++j) {
+          int64_t i = at(L, ch[j]);
+          if (i < 0) return 1;  // the map mode's .at() would throw
+          df.ex_child.push_back(c.dis.count(ch[j]) ? -1 : i);
+        }
+      }
+      for (uint64_t k : mc.ghost_f2c) {
+        c.children(k, L, ch);
+        df.gx_parent.push_back(dc.idx.at(k));
+        for (int j = 0; j < 8; ++j) {
+          bool ok = !c.dis.count(ch[j]) && mf.ghost_c2f_set.count(ch[j]);
+          df.gx_child.push_back(ok ? df.idx.at(ch[j]) : -1);
+        }
+      }
+      for (uint64_t k : mc.f2c) {
+        c.children(k, L, ch);
+        dc.co_parent.push_back(dc.idx.at(k));
+        for (int j = 0; j < 8; ++j) {
+          int64_t i = at(L, ch[j]);
+          if (i < 0) return 1;
+          dc.co_child.push_back(i);
+        }
+      }
+    }
+    // treated cells
+    const Level& top = c.lv[Lmax];
+    for (auto& dd : c.dis) {
+      uint64_t k = dd.first;
+      if (!top.lat_f_set.count(k)) continue;
+      tr_cell.push_back(lv[Lmax].idx.at(k));
+      tr_dis.push_back(dd.second);
+      for (int q = 0; q < Q; ++q) {
+        tr_inc_src.push_back(q ? at(Lmax, c.nbr(k, Lmax, EX[inv(q)], EY[inv(q)], EZ[inv(q)])) : -1);
+        tr_pull_src.push_back(q ? at(Lmax, c.nbr(k, Lmax, EX[q], EY[q], EZ[q])) : -1);
+      }
+    }
+    // level-0 wet boundaries (the cells and neighbours Case::wet_bc reads)
+    {
+      const Level& m0 = c.lv[0];
+      for (uint64_t k : m0.lat_f) {
+        int fm = Case::faces_of(m0.kind.at(k));
+        int act = 0, nact = 0;
+        for (int ff = 0; ff < 6; ++ff)
+          if ((fm & (1 << ff)) && (c.face_kind[ff] == BC_ZOUHE || c.face_kind[ff] == BC_OUTFLOW || c.face_kind[ff] == BC_NEE)) {
+            act |= 1 << ff;
+            ++nact;
+          }
+        if (!nact) continue;
+        wet_cell.push_back(lv[0].idx.at(k));
+        wet_act.push_back(act);
+        int d[3] = {0, 0, 0};
+        for (int ff = 0; ff < 6; ++ff)
+          if (act & (1 << ff)) {
+            int nn[3];
+            Case::normal(ff, nn);
+            for (int a = 0; a < 3; ++a) d[a] -= nn[a];
+          }
+        int64_t kin = at(0, c.nbr(k, 0, d[0], d[1], d[2]));
+        if (kin < 0) return 2;
+        wet_in.push_back(kin);
+      }
+      nee_cells.assign(6, {});
+      nee_nb.assign(6, {});
+      Case& cc = const_cast<Case&>(c);
+      for (int face = 0; face < 6; ++face) {
+        if (c.face_kind[face] != BC_RESERVOIR) continue;
+        for (uint64_t k : cc.nee_cells(face)) {
+          nee_cells[face].push_back(lv[0].idx.at(k));
+          int fm = Case::faces_of(m0.kind.at(k));
+          for (int q = 0; q < Q; ++q) {
+            bool ok = true;
+            int e[3] = {EX[q], EY[q], EZ[q]};
+            for (int ff = 0; ff < 6; ++ff)
+              if (fm & (1 << ff)) {
+                int nn[3];
+                Case::normal(ff, nn);
+                if (e[0] * nn[0] + e[1] * nn[1] + e[2] * nn[2] > 0) ok = false;
+              }
+            int64_t nb = ok ? at(0, c.nbr(k, 0, EX[q], EY[q], EZ[q])) : -2;
+            if (ok && nb < 0) return 3;  // .at() would throw
+            nee_nb[face].push_back(nb);
+          }
+        }
+      }
+    }
+    // markers: cell indices and the band's (marker, weight) lists in ascending (cell key, marker) order
+    if (c.n_mk) {
+      mk_idx.resize(27 * c.n_mk);
+      std::vector<std::tuple<uint64_t, int64_t, double>> pairs;
+      for (int64_t m = 0; m < c.n_mk; ++m)
+        for (int j = 0; j < 27; ++j) {
+          double w = c.mk_w[m * 27 + j];
+          mk_idx[m * 27 + j] = w == 0.0 ? -1 : lv[Lmax].idx.at(c.mk_keys[m * 27 + j]);
+          if (w != 0.0) pairs.emplace_back(c.mk_keys[m * 27 + j], m, w);
+        }
+      std::sort(pairs.begin(), pairs.end(), [](auto& a, auto& b) {
+        return std::get<0>(a) != std::get<0>(b) ? std::get<0>(a) < std::get<0>(b) : std::get<1>(a) < std::get<1>(b);
+      });
+      for (size_t i = 0; i < pairs.size(); ++i) {
+        if (i == 0 || std::get<0>(pairs[i]) != std::get<0>(pairs[i - 1])) {
+          band_cell.push_back(lv[Lmax].idx.at(std::get<0>(pairs[i])));
+          band_ptr.push_back((int64_t)i);
+        }
+        band_mk.push_back(std::get<1>(pairs[i]));
+        band_w.push_back(std::get<2>(pairs[i]));
+      }
+      band_ptr.push_back((int64_t)pairs.size());
+    }
+    return 0;
+  }
+
+  void load(const Lev& d, int64_t i, double* out, bool col) const {
+    const double* a = col ? d.fcol.data() : d.f.data();
+    for (int q = 0; q < Q; ++q) out[q] = a[q * d.n + i];
+  }
+
+  void calc_macros(int L) {
+    Lev& d = lv[L];
+    const int64_t n = d.n;
+    auto one = [&](int64_t i, bool vold) {
+      double f[27], u[3];
+      if (vold)
+        for (int a = 0; a < 3; ++a) d.v_old[a * n + i] = d.u[a * n + i];
+      load(d, i, f, false);
+      macros(Q, f, d.rho[i], u);
+      for (int a = 0; a < 3; ++a) d.u[a * n + i] = u[a];
+    };
+#pragma omp parallel for schedule(static)
+    for (size_t j = 0; j < d.lat_f.size(); ++j) one(d.lat_f[j], true);
+#pragma omp parallel for schedule(static)
+    for (size_t j = 0; j < d.f2c.size(); ++j) one(d.f2c[j], false);
+  }
+
+  void collide(int L) {
+    Lev& d = lv[L];
+    const int64_t n = d.n;
+    const double tL = src->tau[L];
+    const int coll = src->collision;
+    const double lesC = src->les_C;
+    auto one = [&](int64_t i) {
+      double f[27], fe[27], out[27];
+      load(d, i, f, false);
+      double rho = d.rho[i];
+      double u[3] = {d.u[i], d.u[n + i], d.u[2 * n + i]};
+      double t = tL;
+      if (lesC > 0) {
+        feq_c(Q, rho, u, fe);
+        t = smag_tau(t, lesC, stress_norm(Q, f, fe));
+      }
+      if (coll == 0) {
+        feq_c(Q, rho, u, fe);
+        for (int q = 0; q < Q; ++q) out[q] = f[q] - (f[q] - fe[q]) / t;
+      } else {
+        cumulant_collide(f, rho, u, t, out);
+      }
+      for (int q = 0; q < Q; ++q) d.fcol[q * n + i] = out[q];
+    };
+#pragma omp parallel for schedule(static)
+    for (size_t j = 0; j < d.lat_f.size(); ++j) one(d.lat_f[j]);
+#pragma omp parallel for schedule(static)
+    for (size_t j = 0; j < d.f2c.size(); ++j) one(d.f2c[j]);
+#pragma omp parallel for schedule(static)
+    for (size_t j = 0; j < d.c2f.size(); ++j)
+      for (int q = 0; q < Q; ++q) d.fcol[q * n + d.c2f[j]] = d.f[q * n + d.c2f[j]];
+  }
+
+  void stream(int L) {
+    Lev& d = lv[L];
+    const int64_t n = d.n;
+#pragma omp parallel for schedule(static)
+    for (size_t j = 0; j < d.strm_cells.size(); ++j) {
+      int64_t i = d.strm_cells[j];
+      d.f[i] = d.fcol[i];
+      for (int q = 1; q < Q; ++q) {
+        int64_t s = d.strm_src[j * Q + q];
+        if (s >= 0) d.f[q * n + i] = d.fcol[q * n + s];
+      }
+    }
+  }
+
+  void explode(int fine) {
+    const int coarse = fine - 1;
+    const int opt = src->options;
+    const double* tau = src->tau.data();
+    double s = (opt & 1) ? tau[fine] / (2.0 * tau[coarse]) : (2.0 * tau[fine]) / tau[coarse];
+    Lev& dc = lv[coarse];
+    Lev& df = lv[fine];
+    const double rho0 = src->rho0;
+    auto run = [&](const std::vector<int64_t>& par, const std::vector<int64_t>& chl) {
+#pragma omp parallel for schedule(static)
+      for (size_t p = 0; p < par.size(); ++p) {
+        double cf[27], fe[27], u[3], rho;
+        load(dc, par[p], cf, true);
+        macros(Q, cf, rho, u);
+        feq_hl(Q, rho, u, rho0, fe);
+        for (int j = 0; j < 8; ++j) {
+          int64_t c = chl[8 * p + j];
+          if (c < 0) continue;
+          for (int q = 0; q < Q; ++q) df.fcol[q * df.n + c] = fe[q] + s * (cf[q] - fe[q]);
+        }
+      }
+    };
+    run(df.ex_parent, df.ex_child);
+    run(df.gx_parent, df.gx_child);
+  }
+
+  void coalesce(int coarse) {
+    const int fine = coarse + 1;
+    const double* tau = src->tau.data();
+    double s = (src->options & 1) ? (2.0 * tau[coarse]) / tau[fine] : tau[coarse] / (2.0 * tau[fine]);
+    Lev& dc = lv[coarse];
+    Lev& df = lv[fine];
+    const double rho0 = src->rho0;
+#pragma omp parallel for schedule(static)
+    for (size_t p = 0; p < dc.co_parent.size(); ++p) {
+      double avg[27], fe[27], u[3], rho;
+      for (int q = 0; q < Q; ++q) {
+        avg[q] = 0;
+        for (int j = 0; j < 8; ++j) avg[q] += df.f[q * df.n + dc.co_child[8 * p + j]];
+        avg[q] /= 8.0;
+      }
+      macros(Q, avg, rho, u);
+      feq_hl(Q, rho, u, rho0, fe);
+      for (int q = 0; q < Q; ++q) dc.f[q * dc.n + dc.co_parent[p]] = fe[q] + s * (avg[q] - fe[q]);
+    }
+  }
+
+  void bc_after_stream() {
+    Lev& d = lv[0];
+    const int64_t n = d.n;
+    bool any_res = false;
+    for (int face = 0; face < 6; ++face) any_res |= src->face_kind[face] == BC_RESERVOIR;
+    std::vector<double> snap;
+    if (any_res && !src->nee_seq) snap = d.f;
+    for (int face = 0; face < 6; ++face) {
+      if (src->face_kind[face] != BC_RESERVOIR) continue;
+      const double* ub = src->face_u[face];
+      const auto& cells = nee_cells[face];
+      const auto& nbs = nee_nb[face];
+      for (size_t j = 0; j < cells.size(); ++j) {
+        int64_t i = cells[j];
+        for (int q = 0; q < Q; ++q) {
+          int64_t nb = nbs[j * Q + q];
+          if (nb == -2) continue;
+          double un3[3] = {d.u[nb], d.u[n + nb], d.u[2 * n + nb]};
+          double fw[27], fn[27];
+          feq_c(Q, d.rho[i], ub, fw);
+          feq_c(Q, d.rho[nb], un3, fn);
+          double fnb = src->nee_seq ? d.f[q * n + nb] : snap[q * n + nb];
+          d.f[q * n + i] = fw[q] + fnb - fn[q];
+        }
+      }
+    }
+    // wet-node faces: every update reads post-stream f of cells no rule modifies
+    std::vector<double> upd(wet_cell.size() * Q);
+#pragma omp parallel for schedule(static)
+    for (size_t j = 0; j < wet_cell.size(); ++j) {
+      int64_t i = wet_cell[j];
+      int act = wet_act[j];
+      double f[27];
+      load(d, i, f, false);
+      int nact = __builtin_popcount(act);
+      if (nact == 1) {
+        int face = __builtin_ctz(act);
+        int nn[3];
+        Case::normal(face, nn);
+        auto en = [&](int q) { return EX[q] * nn[0] + EY[q] * nn[1] + EZ[q] * nn[2]; };
+        int64_t kin = wet_in[j];
+        if (src->face_kind[face] == BC_NEE) {
+          double fn[27], un[3], rn, fw[27], fq[27];
+          load(d, kin, fn, false);
+          macros(Q, fn, rn, un);
+          feq_c(Q, rn, src->face_u[face], fw);
+          feq_c(Q, rn, un, fq);
+          for (int q = 0; q < Q; ++q) f[q] = fw[q] + fn[q] - fq[q];
+        } else if (src->face_kind[face] == BC_OUTFLOW) {
+          for (int q = 0; q < Q; ++q)
+            if (en(q) == 1) f[q] = d.f[q * n + kin];
+        } else {
+          const double* uw = src->face_u[face];
+          double S0 = 0, Sin = 0;
+          for (int q = 0; q < Q; ++q) {
+            if (en(q) == 0) S0 += f[q];
+            else if (en(q) == -1) Sin += f[q];
+          }
+          double un = uw[0] * nn[0] + uw[1] * nn[1] + uw[2] * nn[2];
+          double rho = (S0 + 2 * Sin) / (1 - un);
+          int ax = face / 2;
+          int t1 = (ax + 1) % 3, t2 = (ax + 2) % 3;
+          if (t1 > t2) std::swap(t1, t2);
+          double Nt[3] = {0, 0, 0};
+          for (int t : {t1, t2}) {
+            double sacc = 0;
+            for (int q = 0; q < Q; ++q)
+              if (en(q) == 0) {
+                int e[3] = {EX[q], EY[q], EZ[q]};
+                sacc += f[q] * e[t];
+              }
+            Nt[t] = 0.5 * sacc - rho * uw[t] / 3.0;
+          }
+          double g[27];
+          for (int q = 0; q < Q; ++q) g[q] = f[q];
+          for (int q = 0; q < Q; ++q) {
+            if (en(q) != 1) continue;
+            int e[3] = {EX[q], EY[q], EZ[q]};
+            double w = weight(Q, q);
+            double eu = e[0] * uw[0] + e[1] * uw[1] + e[2] * uw[2];
+            double tr = e[t1] * Nt[t1] + e[t2] * Nt[t2];
+            g[q] = f[inv(q)] + 6 * w * rho * eu - 36 * w * tr;
+          }
+          for (int q = 0; q < Q; ++q) f[q] = g[q];
+        }
+      } else {
+        const double* uw = nullptr;
+        for (int ff = 0; ff < 6; ++ff)
+          if ((act & (1 << ff)) && (src->face_kind[ff] == BC_ZOUHE || src->face_kind[ff] == BC_NEE)) uw = src->face_u[ff];
+        double fn[27], un[3], rn, fw[27], fq[27];
+        load(d, wet_in[j], fn, false);
+        macros(Q, fn, rn, un);
+        feq_c(Q, rn, uw ? uw : un, fw);
+        feq_c(Q, rn, un, fq);
+        for (int q = 0; q < Q; ++q) f[q] = fw[q] + fn[q] - fq[q];
+      }
+      for (int q = 0; q < Q; ++q) upd[j * Q + q] = f[q];
+    }
+    for (size_t j = 0; j < wet_cell.size(); ++j)
+      for (int q = 0; q < Q; ++q) d.f[q * n + wet_cell[j]] = upd[j * Q + q];
+  }
+
+  void ib_treat(double* F) {
+    Lev& d = lv[Lmax];
+    const int64_t n = d.n;
+    const int ib = src->ib;
+    const size_t nt = tr_cell.size();
+    std::vector<double> mterm(nt * Q, 0.0);
+    std::vector<uint8_t> link(nt * Q, 0);
+#pragma omp parallel for schedule(static)
+    for (size_t j = 0; j < nt; ++j) {
+      int64_t i = tr_cell[j];
+      const auto& dist_of = tr_dis[j];
+      double rb = d.rho[i];
+      d.f[i] = d.fcol[i];
+      for (int q = 1; q < Q; ++q) {
+        double dist = dist_of[inv(q) - 1];
+        if (dist != -1.) {
+          double finc = d.fcol[inv(q) * n + i];
+          int64_t sidx = tr_inc_src[j * Q + q];
+          double fs = sidx >= 0 ? d.fcol[q * n + sidx] : finc;
+          double fnew;
+          if (ib == 1) {
+            if (dist < 0.5) {
+              fnew = (1.0 - 2.0 * dist) * finc + 2.0 * dist * fs;
+            } else {
+              double fe = ib_weight(inv(q)) * rb;
+              double i2d = 1.0 / (2.0 * dist);
+              fnew = i2d * fs + (1.0 - i2d) * fe;
+            }
+          } else {
+            fnew = (1.0 - dist) * finc + dist * fs;
+          }
+          d.f[q * n + i] = fnew;
+          mterm[j * Q + q] = finc + fnew;
+          link[j * Q + q] = 1;
+        } else {
+          int64_t sidx = tr_pull_src[j * Q + q];
+          d.f[q * n + i] = sidx < 0 ? d.fcol[inv(q) * n + i] : d.fcol[q * n + sidx];
+        }
+      }
+      double ra = 0;
+      for (int q = 0; q < Q; ++q) ra += d.f[q * n + i];
+      double& f0 = d.f[i];
+      if (ra < 0.3) f0 += 1.0 * (0.3 - ra);
+      else if (ra < 0.5) f0 += 0.5 * (0.5 - ra);
+      else if (ra > 3.0) f0 += 1.0 * (3.0 - ra);
+      else if (ra > 2.0) f0 += 0.5 * (2.0 - ra);
+    }
+    double fx = 0, fy = 0, fz = 0;
+    for (size_t j = 0; j < nt; ++j)
+      for (int q = 1; q < Q; ++q)
+        if (link[j * Q + q]) {
+          double m = mterm[j * Q + q];
+          fx += -m * EX[q];
+          fy += -m * EY[q];
+          fz += -m * EZ[q];
+        }
+    force_log.push_back({fx, fy, fz});
+    if (F) { F[0] = fx; F[1] = fy; F[2] = fz; }
+    const int sweeps = src->mdf_sweeps;
+#pragma omp parallel for schedule(static)
+    for (size_t j = 0; j < nt; ++j) {
+      int64_t i = tr_cell[j];
+      const auto& dist_of = tr_dis[j];
+      for (int it = 0; it < sweeps; ++it) {
+        double rho = 0, u[3] = {0, 0, 0};
+        for (int q = 0; q < Q; ++q) {
+          double fq = d.f[q * n + i];
+          rho += fq;
+          u[0] += fq * EX[q];
+          u[1] += fq * EY[q];
+          u[2] += fq * EZ[q];
+        }
+        if (rho <= 0.) continue;
+        u[0] /= rho; u[1] /= rho; u[2] /= rho;
+        double um = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        if (um < 1e-10) continue;
+        double dM[3] = {-0.5 * rho * u[0], -0.5 * rho * u[1], -0.5 * rho * u[2]};
+        double adj = 0;
+        for (int q = 1; q < Q; ++q) {
+          if (dist_of[inv(q) - 1] == -1.) continue;
+          double df = ib_weight(q) * 3.0 * (EX[q] * dM[0] + EY[q] * dM[1] + EZ[q] * dM[2]);
+          d.f[q * n + i] += df;
+          adj -= df;
+        }
+        d.f[i] += adj;
+      }
+    }
+  }
+
+  void marker_correct() {
+    const Case& c = *src;
+    if (!c.n_mk) return;
+    Lev& d = lv[Lmax];
+    const int64_t n = d.n;
+    std::vector<std::array<double, 3>> F(c.n_mk);
+#pragma omp parallel for schedule(static)
+    for (int64_t m = 0; m < c.n_mk; ++m) {
+      double rk = 0, uk[3] = {0, 0, 0};
+      for (int j = 0; j < 27; ++j) {
+        const double w = c.mk_w[m * 27 + j];
+        if (w == 0.0) continue;
+        double f[27], rho, u[3];
+        load(d, mk_idx[m * 27 + j], f, true);
+        macros(Q, f, rho, u);
+        rk += w * rho;
+        for (int a = 0; a < 3; ++a) uk[a] += w * u[a];
+      }
+      for (int a = 0; a < 3; ++a) F[m][a] = c.mk_dv[m] * rk * (c.mk_ub[a] - uk[a]);
+    }
+    const size_t nb = band_cell.size();
+    std::vector<double> newf(nb * Q);
+#pragma omp parallel for schedule(static)
+    for (size_t b = 0; b < nb; ++b) {
+      double dm[3] = {0, 0, 0};
+      for (int64_t p = band_ptr[b]; p < band_ptr[b + 1]; ++p)
+        for (int a = 0; a < 3; ++a) dm[a] += band_w[p] * F[band_mk[p]][a];
+      double f[27], rho, u[3], u2[3], fe1[27], fe2[27];
+      load(d, band_cell[b], f, true);
+      macros(Q, f, rho, u);
+      for (int a = 0; a < 3; ++a) u2[a] = u[a] + dm[a] / rho;
+      feq_c(Q, rho, u2, fe2);
+      feq_c(Q, rho, u, fe1);
+      for (int q = 0; q < Q; ++q) newf[b * Q + q] = f[q] + (fe2[q] - fe1[q]);
+    }
+    for (size_t b = 0; b < nb; ++b)
+      for (int q = 0; q < Q; ++q) d.fcol[q * n + band_cell[b]] = newf[b * Q + q];
+    double fx = 0, fy = 0, fz = 0;
+    for (int64_t m = 0; m < c.n_mk; ++m) {
+      fx += F[m][0];
+      fy += F[m][1];
+      fz += F[m][2];
+    }
+    force_log.push_back({fx, fy, fz});
+  }
+
+  void kernel(int L) {
+    const int opt = src->options;
+    const bool ib = src->ib != 0;
+    calc_macros(L);
+    collide(L);
+    if (L == Lmax) marker_correct();
+    if (L < Lmax) kernel(L + 1);
+    if (L > 0) explode(L);
+    stream(L);
+    if (L == 0) bc_after_stream();
+    else if (L == Lmax && ib) ib_treat(nullptr);
+    if (L < Lmax) coalesce(L);
+    if (L == 0) return;
+    calc_macros(L);
+    collide(L);
+    if (L == Lmax) marker_correct();
+    if (L < Lmax) kernel(L + 1);
+    if (opt & 2) explode(L);
+    stream(L);
+    if (L == 0) bc_after_stream();
+    else if (L == Lmax && ib) ib_treat(nullptr);
+    if (L < Lmax) coalesce(L);
+  }
+  void step() {
+    kernel(0);
+    for (int L = 0; L <= Lmax; ++L) calc_macros(L);
+  }
+};
 }  // namespace
 
 extern "C" {
@@ -1026,6 +1653,70 @@ int64_t mto_force_log(void* h, double* out, int64_t max) {
   return n;
 }
 
+
+// ---- dense mode (same bits as the map mode, sorted arrays, threads)
+// fresh = 1: from the case's level sets alone (mto_init need not have run; the
+// case must outlive the dense copy); fresh = 0: from the map mode's current state
+void* mto_dense_create(void* h, int fresh) {
+  Case* c = (Case*)h;
+  if (fresh) c->init_tau();
+  Dense* d = new Dense;
+  if (d->build(*c, fresh != 0)) {
+    delete d;
+    return nullptr;
+  }
+  return d;
+}
+void mto_dense_destroy(void* d) { delete (Dense*)d; }
+int mto_dense_step(void* h, int n) {
+  Dense* d = (Dense*)h;
+  for (int i = 0; i < n; ++i) d->step();
+  return 0;
+}
+int mto_dense_get(void* h, int L, int what, const uint64_t* keys, int64_t n, double* out) {
+  Dense* d = (Dense*)h;
+  Dense::Lev& l = d->lv[L];
+  for (int64_t j = 0; j < n; ++j) {
+    int64_t i = d->at(L, keys[j]);
+    if (i < 0) return 1;
+    if (what <= 1) {
+      const auto& a = what == 0 ? l.f : l.fcol;
+      for (int q = 0; q < d->Q; ++q) out[q * n + j] = a[q * l.n + i];
+    } else if (what == 2) {
+      out[j] = l.has_macro[i] ? l.rho[i] : NAN;
+    } else {
+      const auto& a = what == 3 ? l.u : l.v_old;
+      for (int c = 0; c < 3; ++c) out[c * n + j] = l.has_macro[i] ? a[c * l.n + i] : NAN;
+    }
+  }
+  return 0;
+}
+int mto_dense_set(void* h, int L, int what, const uint64_t* keys, int64_t n, const double* in) {
+  Dense* d = (Dense*)h;
+  Dense::Lev& l = d->lv[L];
+  for (int64_t j = 0; j < n; ++j) {
+    int64_t i = d->at(L, keys[j]);
+    if (i < 0) return 1;
+    if (what <= 1) {
+      auto& a = what == 0 ? l.f : l.fcol;
+      for (int q = 0; q < d->Q; ++q) a[q * l.n + i] = in[q * n + j];
+    } else if (what == 2) {
+      l.rho[i] = in[j];
+      l.has_macro[i] = 1;
+    } else {
+      auto& a = what == 3 ? l.u : l.v_old;
+      for (int c = 0; c < 3; ++c) a[c * l.n + i] = in[c * n + j];
+    }
+  }
+  return 0;
+}
+int64_t mto_dense_force_log(void* h, double* out, int64_t max) {
+  Dense* d = (Dense*)h;
+  int64_t n = (int64_t)d->force_log.size();
+  for (int64_t i = 0; i < n && i < max; ++i)
+    for (int a = 0; a < 3; ++a) out[3 * i + a] = d->force_log[i][a];
+  return n;
+}
 // ---- primitives for closed-form tests
 void mto_feq(int Q, double rho, const double* u, double* out) { feq_c(Q, rho, u, out); }
 void mto_feq_heluo(int Q, double rho, const double* u, double rho0, double* out) { feq_hl(Q, rho, u, rho0, out); }

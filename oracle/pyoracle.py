@@ -40,6 +40,14 @@ def lib():
         L.mto_set_options.argtypes = [C.c_void_p, C.c_int]
         L.mto_set_tau.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.mto_set_mdf_sweeps.argtypes = [C.c_void_p, C.c_int]
+        L.mto_dense_create.restype = C.c_void_p
+        L.mto_dense_create.argtypes = [C.c_void_p, C.c_int]
+        L.mto_dense_destroy.argtypes = [C.c_void_p]
+        L.mto_dense_step.argtypes = [C.c_void_p, C.c_int]
+        L.mto_dense_get.argtypes = [C.c_void_p, C.c_int, C.c_int, _u64p, C.c_int64, _dp]
+        L.mto_dense_set.argtypes = [C.c_void_p, C.c_int, C.c_int, _u64p, C.c_int64, _dp]
+        L.mto_dense_force_log.restype = C.c_int64
+        L.mto_dense_force_log.argtypes = [C.c_void_p, _dp, C.c_int64]
         L.mto_add_level.argtypes = [C.c_void_p, C.c_int, _u64p, _u16p, C.c_int64]
         L.mto_set_dis.argtypes = [C.c_void_p, _u64p, _dp, C.c_int64]
         L.mto_set_markers.argtypes = [C.c_void_p, C.c_int64, _u64p, _dp, _dp, _dp]
@@ -85,7 +93,7 @@ def _k(a):
 class OracleCase:
     """A case held by the oracle; mirrors LBM_Manager's state and loop."""
 
-    def __init__(self, case, nee_sequential=True):
+    def __init__(self, case, nee_sequential=True, _init=True):
         L = lib()
         self.case = case
         self.Q = case["Q"]
@@ -113,7 +121,8 @@ class OracleCase:
             ub = np.ascontiguousarray(mk["ub"], dtype=np.float64)
             L.mto_set_markers(self.h, dv.size, keys.ctypes.data_as(_u64p), w.ctypes.data_as(_dp),
                               dv.ctypes.data_as(_dp), ub.ctypes.data_as(_dp))
-        L.mto_init(self.h)
+        if _init:
+            L.mto_init(self.h)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -195,6 +204,71 @@ class OracleCase:
         out = np.zeros(3 * max(n, 1))
         lib().mto_force_log(self.h, out.ctypes.data_as(_dp), n)
         return out[: 3 * n].reshape(n, 3)
+
+
+class DenseOracleCase(OracleCase):
+    """The oracle's dense mode: the same algorithm and the same bits as OracleCase,
+    on sorted arrays with index tables and threads, fast enough for full-size runs.
+    Whole steps, field access and the force log only (no single seams).
+    from_state=OracleCase starts from that case's current state; otherwise the
+    initial state is built from the level sets without allocating the maps."""
+
+    def __init__(self, case, nee_sequential=True, from_state=None):
+        if from_state is not None:
+            self._mapcase = from_state
+            self.case, self.Q = from_state.case, from_state.Q
+            self.d = lib().mto_dense_create(from_state.h, 0)
+        else:
+            self._mapcase = OracleCase(case, nee_sequential, _init=False)
+            self.case, self.Q = case, case["Q"]
+            self.d = lib().mto_dense_create(self._mapcase.h, 1)
+        if not self.d:
+            raise RuntimeError("dense oracle: a transfer child or boundary neighbour is missing")
+        self.h = None  # the seams of the map mode are not offered
+
+    def __del__(self):
+        if getattr(self, "d", None) and _lib is not None:
+            _lib.mto_dense_destroy(self.d)
+            self.d = None
+
+    def step(self, n=1):
+        lib().mto_dense_step(self.d, n)
+
+    def get(self, level, what, keys=None):
+        if keys is None:
+            keys = self.case["levels"][level]["keys"]
+        keys, pk = _k(keys)
+        n = keys.size
+        rows = {0: self.Q, 1: self.Q, 2: 1, 3: 3, 4: 3}[what]
+        out = np.empty(rows * n)
+        if lib().mto_dense_get(self.d, level, what, pk, n, out.ctypes.data_as(_dp)):
+            raise KeyError("key not in level")
+        return out.reshape(rows, n) if rows > 1 else out
+
+    def set(self, level, what, values, keys=None):
+        if keys is None:
+            keys = self.case["levels"][level]["keys"]
+        keys, pk = _k(keys)
+        v, pv = _d(values)
+        if lib().mto_dense_set(self.d, level, what, pk, keys.size, pv):
+            raise KeyError("key not in level")
+
+    def force_log(self):
+        n = lib().mto_dense_force_log(self.d, None, 0)
+        out = np.zeros(3 * max(n, 1))
+        lib().mto_dense_force_log(self.d, out.ctypes.data_as(_dp), n)
+        return out[: 3 * n].reshape(n, 3)
+
+    def _no_seam(self, *a):
+        raise NotImplementedError("the dense oracle runs whole steps only")
+
+    macros = collide = stream = explode = coalesce = bc_after_stream = ib_treat = marker_correct = _no_seam
+
+    def set_tau(self, taus):
+        self._mapcase.set_tau(taus)
+
+    def set_mdf_sweeps(self, n):
+        self._mapcase.set_mdf_sweeps(n)
 
 
 # ---- primitives ----------------------------------------------------------------

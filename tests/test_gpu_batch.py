@@ -67,6 +67,30 @@ def test_packed_spheres_equal_alone():
     gb.close()
 
 
+def test_poisoned_member_flagged_others_unchanged():
+    """A member whose populations are non-finite (NaN initial velocity) is the only
+    one the per-member device scan flags (the reference's NaN checks,
+    LBM_Simulator.cpp:33-41), and the members beside it stay bit for bit what they
+    compute alone."""
+    ms = [C.cavity(12), C.cavity(16, refine_box=((5, 5, 5), (11, 11, 11))), C.cavity(12, lid=0.06)]
+    bad = dict(ms[1], u0=(float("nan"), 0.0, 0.0))
+    packed = C.pack([ms[0], bad, ms[2]])
+    gb = amr.AmrCase(packed, precision=amr.FP32_SHIFTED)
+    assert np.array_equal(gb.scan_members() > 0, [False, True, False])
+    gb.step(20)
+    flags = gb.scan_members()
+    assert flags[0] == 0 and flags[2] == 0 and flags[1] > 0
+    for i in (0, 2):
+        ga = amr.AmrCase(ms[i], precision=amr.FP32_SHIFTED)
+        ga.step(20)
+        assert ga.scan_members()[0] == 0
+        for L in range(ms[i]["Lmax"] + 1):
+            for a, b in zip(member_fields(gb, packed, i, ms[i], L), alone_fields(ga, ms[i], L)):
+                assert np.array_equal(a, b), f"member {i} level {L}"
+        ga.close()
+    gb.close()
+
+
 def test_member_box_validation():
     """Overlapping member boxes and misaligned origins are rejected."""
     import ctypes

@@ -343,9 +343,14 @@ def test_set_field_then_step():
     compare(g, o, case)
 
 
-def test_interface_cell_on_both_sides_rejected():
+@pytest.mark.parametrize("bad,frag", [(C.KIND_F2C | C.KIND_C2F, "both sides"),
+                                      (C.KIND_FLUID | C.KIND_F2C, "both a fluid cell"),
+                                      (C.KIND_FLUID | C.KIND_C2F, "both a fluid cell")])
+def test_interface_cell_on_both_sides_rejected(bad, frag):
     """A level whose cell is both coarse-side (F2C) and fine-side (C2F) of an
-    interface is rejected at upload (the per-level streams rely on the split)."""
+    interface, or a fluid cell that is also an interface cell, is rejected at
+    upload (the schedules rely on the split: such a cell would be written by two
+    levels in one launch)."""
     import ctypes
     lib = amr.load()
     d = amr.CaseDesc()
@@ -356,11 +361,11 @@ def test_interface_cell_on_both_sides_rejected():
     assert lib.mt_case_create(ctypes.byref(d), ctypes.byref(h)) == 0
     try:
         keys = np.array([0, 8], dtype=np.uint64)
-        kinds = np.array([C.KIND_FLUID, C.KIND_F2C | C.KIND_C2F], dtype=np.uint16)
+        kinds = np.array([C.KIND_FLUID, bad], dtype=np.uint16)
         rc = lib.mt_case_upload_level(h, 1, keys.ctypes.data_as(amr._u64p),
                                       kinds.ctypes.data_as(amr._u16p), 2)
         assert rc == -1
-        assert "both sides" in lib.mt_last_error().decode()
+        assert frag in lib.mt_last_error().decode()
     finally:
         lib.mt_case_destroy(h)
 

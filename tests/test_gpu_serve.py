@@ -110,16 +110,66 @@ def test_native_server_matches_alone():
     srv.submit(3, reqs[3].case, reqs[3].max_steps, 0.0)
     for _ in range(50):
         nr, nw = srv.step()
-        for rid, steps, cd in srv.poll():
+        for rid, steps, cd, diverged in srv.poll():
+            assert not diverged
             got[rid] = (steps, cd)
         if nr == 0 and nw == 0:
             break
-    for rid, steps, cd in srv.poll():
+    for rid, steps, cd, diverged in srv.poll():
         got[rid] = (steps, cd)
     srv.close()
     assert sorted(got) == [0, 1, 2, 3]
     for r in reqs:
         alone = serve.run_alone(r, precision=amr.FP32_SHIFTED, chunk=100)
+        assert got[r.id][0] == alone.steps
+        assert np.isclose(got[r.id][1], alone.cd[-1], rtol=1e-9, atol=1e-12), r.id
+
+
+def poisoned(req):
+    """The request with a non-finite initial velocity: every population of it is NaN."""
+    req.case = dict(req.case, u0=(float("nan"), 0.0, 0.0))
+    return req
+
+
+def test_diverged_request_is_flagged_and_evicted():
+    """A request whose state goes non-finite is found by the per-member device scan
+    after its first chunk, finishes as failed and leaves the batch at the next
+    chunk's repack; the requests beside it compute what they compute alone."""
+    rng = np.random.default_rng(10)
+    reqs = [small_request(0, 0.0, rng), poisoned(small_request(1, 0.0, rng)), small_request(2, 1.5, rng)]
+    srv = serve.Server(precision=amr.FP32_SHIFTED, chunk=100, chunk_seconds=1.0, admit_every=4)
+    done = srv.run(reqs)
+    by_id = {r.id: r for r in done}
+    assert sorted(by_id) == [0, 1, 2]
+    bad = by_id[1]
+    assert bad.failed and bad.steps == 100 and bad.done == 1.0
+    # the eviction repack at 1.0 is due although the admission interval is 4 chunks
+    assert srv.repack_log[0] == (0.0, [0, 1])
+    assert srv.repack_log[1] == (1.0, [0])
+    good = [by_id[0], by_id[2]]
+    assert not any(r.failed for r in good)
+    check_against_alone([reqs[0], reqs[2]], good, amr.FP32_SHIFTED)
+
+
+def test_native_server_flags_and_evicts_a_diverged_request():
+    rng = np.random.default_rng(11)
+    reqs = [small_request(0, 0.0, rng), poisoned(small_request(1, 0.0, rng)), small_request(2, 0.0, rng)]
+    srv = serve.NativeServer(precision=amr.FP32_SHIFTED, chunk=100, admit_every=4)
+    for r in reqs:
+        srv.submit(r.id, r.case, r.max_steps, 0.0)
+    got = {}
+    for _ in range(50):
+        nr, nw = srv.step()
+        for rid, steps, cd, diverged in srv.poll():
+            got[rid] = (steps, cd, diverged)
+        if nr == 0 and nw == 0:
+            break
+    srv.close()
+    assert sorted(got) == [0, 1, 2]
+    assert got[1][2] and got[1][0] == 100
+    for r in (reqs[0], reqs[2]):
+        alone = serve.run_alone(r, precision=amr.FP32_SHIFTED, chunk=100)
+        assert not got[r.id][2] and not alone.failed
         assert got[r.id][0] == alone.steps
         assert np.isclose(got[r.id][1], alone.cd[-1], rtol=1e-9, atol=1e-12), r.id
 

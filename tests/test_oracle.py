@@ -660,3 +660,41 @@ def test_ib_density_clamp_and_force_log():
     treated = ((lv["kinds"] & C.KIND_DIS) != 0) & ((lv["kinds"] & C.KIND_FLUID) != 0)
     rho = o.f(2)[:, treated].sum(0)
     assert rho.min() >= 0.3 - 1e-12
+
+
+# ---------------------------------------------------------------- dense mode
+DENSE_CASES = {
+    "cavity_zouhe": lambda: C.cavity(12),
+    "cavity_nee_two_level": lambda: C.cavity(12, wall=C.BC_NEE, tau=0.55, refine_box=((4, 4, 4), (8, 8, 8))),
+    "sphere_cumulant": lambda: C.sphere_amr((32, 24, 24), radius=3.0, center=(12, 12, 12), shells=(5.0, 1.5)),
+    "sphere_lbgk_dw": lambda: C.sphere_amr((32, 24, 24), radius=3.0, center=(12, 12, 12), shells=(5.0, 1.5), Q=19,
+                                           collision=C.COLLIDE_LBGK, ib=C.IB_DW),
+    "sphere_les_both_options": lambda: C.sphere_amr((32, 24, 24), radius=3.0, center=(12, 12, 12),
+                                                    shells=(5.5, 2.0), les_C=0.1, options=3),
+    "reference_sphere": lambda: C.reference_sphere((32, 24, 24), radius=3.0, center=(12, 12, 12),
+                                                   shells=(5.0, 1.5)),
+    "markers": lambda: C.sphere_markers((32, 24, 24), radius=3.0, center=(12, 12, 12), shells=(5.5, 2.0),
+                                        n_markers=1800),
+}
+
+
+@pytest.mark.parametrize("name", list(DENSE_CASES))
+def test_dense_mode_is_the_map_mode(name):
+    """The dense oracle (sorted arrays, index tables, threads) computes the map
+    mode's bits: every population, macro and v_old of every level and every
+    logged force, after three coarse steps, also when started from a state set
+    into the map mode."""
+    case = DENSE_CASES[name]()
+    a, b = O.OracleCase(case), O.DenseOracleCase(case)
+    a.step(3)
+    b.step(3)
+    for L in range(case["Lmax"] + 1):
+        for what in range(5):
+            x, y = a.get(L, what), b.get(L, what)
+            assert np.array_equal(x, y, equal_nan=True), (L, what)
+    assert np.array_equal(a.force_log(), b.force_log())
+    c = O.DenseOracleCase(case, from_state=a)
+    a.step(1)
+    c.step(1)
+    for L in range(case["Lmax"] + 1):
+        assert np.array_equal(a.f(L), c.f(L))
