@@ -75,11 +75,25 @@ def total_mass(g, Lmax):
     return sum(g.reduce(L)[0] / 8.0 ** L for L in range(Lmax + 1))
 
 
+def warm_graphs(g, steps):
+    """Captures, before the timed region, every HIP graph a call of mt_step(steps)
+    replays: graphs are keyed by the levels' buffer parities at their start (level
+    0 alternates per coarse step), their length (4, or steps mod 4 for the last)
+    and whether they end the call, so 8 + steps mod 4 coarse steps at each
+    level-0 parity build them all."""
+    r = steps % 4
+    g.step(8 + r)
+    g.step(1)
+    g.step(8 + r)
+    g.sync()
+
+
 def run_gpu(case, steps, warmup, precision, timing_steps, keep_fields=False):
     from mortontrie_amr_amd import amr, cases as C
     Lmax = case["Lmax"]
     g = amr.AmrCase(case, precision=precision, graphs=True)
     g.step(warmup)
+    warm_graphs(g, steps)
     g.sync()
     m0 = total_mass(g, Lmax)
     g.ib_force(reset=True)
@@ -179,13 +193,9 @@ def run_batch(args):
         same.append(ok)
         a.close()
     nan = sum(g.reduce(L)[2] for L in range(packed["Lmax"] + 1))
-    # members whose flow diverged (LBGK at tau near 1/2 and the larger lid speeds)
-    bad = []
-    for i, m in enumerate(members):
-        lv = m["levels"][0]
-        keys = lv["keys"][(lv["kinds"] & C.KIND_FLUID) != 0]
-        if not np.isfinite(g.rho(0, C.member_keys(packed, i, 0, keys))).all():
-            bad.append(i)
+    # members whose flow diverged (LBGK at tau near 1/2 and the larger lid
+    # speeds), from the per-member device scan
+    bad = [int(i) for i in np.nonzero(g.scan_members())[0]]
     g.close()
     cells = sum(lv["keys"].size for lv in packed["levels"])
     out = {
@@ -201,7 +211,7 @@ def run_batch(args):
                                    members[i]["dims0"][0]] for i in bad[:12]],
         "algorithmic_GBs": round(upd_step * 152 * args.steps / secs / 1e9, 1),
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def run_serve(args):
@@ -217,9 +227,7 @@ def run_serve(args):
     lat = np.array([r.done - r.arrival for r in done])
     # capacity: the same requests all waiting at the start (the Poisson run's
     # rate is bounded by the 20/s arrivals)
-    sat = serve.poisson_requests(args.requests, rate=20.0, seed=1)
-    for r in sat:
-        r.arrival = 0.0
+    sat = [serve.Request(r.id, r.case, 0.0, r.max_steps, r.tol) for r in reqs]
     srv_sat = serve.Server(precision=amr.FP32_SHIFTED, chunk=100, admit_every=4)
     t0 = time.perf_counter()
     done_sat = srv_sat.run(sat)
@@ -243,6 +251,12 @@ def run_serve(args):
     errs, merrs, bad = [], [], []
     for r in done:
         p = plain[r.id]
+        if r.failed or p.failed:
+            bad.append(r.case["request"] | {"tau0": round(r.case["tau0"], 5), "served_failed": r.failed,
+                                            "plain_failed": p.failed})
+            errs.append(float("nan"))
+            merrs.append(float("nan"))
+            continue
         k = min(len(r.cd), len(p.cd))
         a, b = np.array(r.cd[:k]), np.array(p.cd[:k])
         if not (np.isfinite(a).all() and np.isfinite(b).all()):
@@ -273,9 +287,28 @@ def run_serve(args):
         "drag_rel_error_mean_max": float(np.nanmax(merrs)) if np.isfinite(merrs).any() else None,
         "drag_rel_error_mean_median": float(np.nanmedian(merrs)) if np.isfinite(merrs).any() else None,
         "non_finite_requests": bad,
+        "diverged_requests_evicted": sorted(r.id for r in done if r.failed),
+        "native_server_diverged": sorted(int(d[0]) for d in nat_done if d[3]),
         "repacks": srv.repacks, "steps_per_request": [r.steps for r in sorted(done, key=lambda r: r.id)],
     }
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def host_cpu():
+    """The host CPU's model name and the cores this process may use."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count()
+    return model, n
 
 
 def cpu_baseline(case, steps):
@@ -296,6 +329,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="the sphere line only (no cavity / markers / batch / serving sub-objects)")
     ap.add_argument("--counters", action="store_true",
                     help="short eager run plus a calibration copy, for rocprofv3 --pmc passes")
     ap.add_argument("--workload", choices=("sphere", "markers", "cavity", "batch", "serve"), default="sphere",
@@ -312,36 +347,98 @@ def main():
     if args.workload == "batch":
         if args.steps == 4000:
             args.steps = 2000
-        run_batch(args)
+        print(json.dumps(run_batch(args)), flush=True)
         return
     if args.workload == "serve":
-        run_serve(args)
+        print(json.dumps(run_serve(args)), flush=True)
         return
-    from mortontrie_amr_amd import cases as C, amr
-    if args.workload == "cavity":
-        case = C.cavity(64)
-        workload = CAVITY_WORKLOAD
-        if args.steps == 4000:
-            args.steps = 1000
-    elif args.workload == "markers":
-        # with the reference's f_neq rescaling at the interfaces (options=2) this
-        # flow diverges after ~1700 steps; the convective rescaling keeps it finite
-        case = C.sphere_markers(n_markers=6000, les_C=0.1, options=3)
-        workload = MARKER_WORKLOAD
-        if args.steps == 4000:
-            args.steps = 5000
-    else:
-        case = C.sphere_amr(options=2)
-        workload = WORKLOAD
+    case, workload, steps = workload_case(args.workload, args.steps)
+    args.steps = steps
     if args.counters:
         counter_run(case, 4, 5, 256)
         return
+    out = measure(case, workload, args, variants=not args.no_variants, cpu=not args.no_cpu)
+    if args.workload == "sphere" and not args.no_extras:
+        out["workloads"] = extras(args)
+    print(json.dumps(out), flush=True)
+
+
+def workload_case(name, steps):
+    """(case, workload name, steps) of a workload; steps=4000 (the default)
+    becomes the workload's stated step count."""
+    from mortontrie_amr_amd import cases as C
+    if name == "cavity":
+        return C.cavity(64), CAVITY_WORKLOAD, 1000 if steps == 4000 else steps
+    if name == "markers":
+        # with the reference's f_neq rescaling at the interfaces (options=2) this
+        # flow diverges after ~1700 steps; the convective rescaling keeps it finite
+        return (C.sphere_markers(n_markers=6000, les_C=0.1, options=3), MARKER_WORKLOAD,
+                5000 if steps == 4000 else steps)
+    return C.sphere_amr(options=2), WORKLOAD, steps
+
+
+def reference_schedule(args):
+    """The sphere with the reference's interface semantics (explosion in sub-step A
+    only, LBM_Simulator.cpp:47-51): separate explosion kernels and per-level
+    streams, no merged launches.  The flow is unstable at tau0 = 0.5192 under
+    these semantics (finite for ~100 coarse steps), so 20 steps are timed."""
+    from mortontrie_amr_amd import amr, cases as C
+    case = C.sphere_amr(options=0)
+    upd = sum(n << L for L, n in enumerate(level_updates(case)))
+    g = amr.AmrCase(case, precision=amr.FP64, graphs=True)
+    g.step(4)
+    warm_graphs(g, 20)
+    t0 = time.perf_counter()
+    g.step(20)
+    g.sync()
+    secs = time.perf_counter() - t0
+    nan = sum(g.reduce(L)[2] for L in range(case["Lmax"] + 1))
+    g.close()
+    return {"MLUPS": round(upd * 20 / secs / 1e6, 2), "ms_per_step": round(secs / 20 * 1e3, 5), "steps": 20,
+            "nan_count": int(nan),
+            "schedule": "reference interface semantics: explosion kernels in sub-step A, per-level streams"}
+
+
+def extras(args):
+    """The rest of the metric, each workload at its stated configuration: the
+    64^3 cavity, the 6000-marker sphere with precision variants, the
+    reference-semantics schedule, the 256-cavity packed batch and the Poisson
+    serving stream."""
+    import argparse as _ap
+    out = {}
+    for name in ("cavity", "markers"):
+        case, workload, steps = workload_case(name, 4000)
+        a = _ap.Namespace(**vars(args))
+        a.steps = steps
+        m = measure(case, workload, a, variants=True, cpu=False)
+        out[name] = {k: m[k] for k in ("value", "unit", "steps", "ms_per_step", "dtype", "drag_coefficient",
+                                       "mass_drift", "nan_count") if k in m}
+        out[name]["workload"] = workload
+        out[name]["roofline"] = {k: m["roofline"][k] for k in ("achieved", "frac", "kernel", "avg_launch_ms")}
+        out[name]["variants"] = m.get("variants")
+        print(json.dumps({"progress": name, "MLUPS": m["value"]}), file=sys.stderr, flush=True)
+    out["reference_schedule"] = reference_schedule(args)
+    a = _ap.Namespace(**vars(args))
+    a.steps, a.warmup = 2000, 20
+    b = run_batch(a)
+    out["batch"] = {k: b[k] for k in ("value", "unit", "steps", "ms_per_step", "dtype", "alone_equals_packed",
+                                      "diverged_members", "algorithmic_GBs", "autotune_ms_per_step")}
+    out["batch"]["workload"] = b["config"]["workload"]
+    print(json.dumps({"progress": "batch", "MLUPS": b["value"]}), file=sys.stderr, flush=True)
+    v = run_serve(args)
+    out["serve"] = {k: v[k] for k in v if k not in ("metric", "n_gpus", "higher_is_better", "data",
+                                                    "steps_per_request")}
+    return out
+
+
+def measure(case, workload, args, variants=True, cpu=True):
+    from mortontrie_amr_amd import amr
     per_level = level_updates(case)
     upd_step = sum(n << L for L, n in enumerate(per_level))
     Q = case["Q"]
     Lf = case["Lmax"]
 
-    r = run_gpu(case, args.steps, args.warmup, amr.FP64, args.timing_steps, keep_fields=not args.no_variants)
+    r = run_gpu(case, args.steps, args.warmup, amr.FP64, args.timing_steps, keep_fields=variants)
     mlups = upd_step * args.steps / r["secs"] / 1e6
     avg_ms = r["ms"][Lf] / max(r["calls"][Lf], 1)
     # one launch per finest sub-step carries the coarser levels' updates that
@@ -398,7 +495,7 @@ def main():
     if "markers" in case:
         out["config"].update(markers=int(case["markers"]["dv"].size), les_C=case["les_C"],
                              solid_boundary="direct forcing by Lagrangian markers")
-    if not args.no_variants:
+    if variants:
         var = {}
         for name, prec in (("fp32_shifted", amr.FP32_SHIFTED), ("fp16_storage", amr.FP16_STORAGE)):
             v = run_gpu(case, args.steps, args.warmup, prec, args.timing_steps, keep_fields=True)
@@ -416,15 +513,17 @@ def main():
                 "nan_count": v["nan"],
             }
         out["variants"] = var
-    if not args.no_cpu:
+    if cpu:
         n = args.cpu_steps
         cs = cpu_baseline(case, n)
+        model, ncpu = host_cpu()
         out["cpu_baseline"] = {
             "value": round(upd_step * n / cs / 1e6, 4), "unit": "MLUPS", "cores": 1, "kind": "port",
             "sample": f"{n} coarse steps of the same workload, map-faithful oracle (one unordered_map per "
                       f"population), single thread, {cs:.1f} s",
+            "cpu_model": model, "nproc": ncpu,
         }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":
