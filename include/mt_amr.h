@@ -58,6 +58,9 @@ extern "C" {
 #define MT_IB_NONE 0
 #define MT_IB_FH 1 /* IB_Method::FH                */
 #define MT_IB_DW 2 /* IB_Method::DISTANCE_WEIGHTED */
+#define MT_IB_UNIFORM 3 /* UniformBoundary::treat (ComplexBoundary_Manager.cpp:15-40):
+                           mt_ib_treat only; mt_step refuses it (the reference
+                           never schedules it)                                  */
 
 #define MT_FP64 0          /* populations stored and computed in FP64               */
 #define MT_FP32_SHIFTED 1  /* stored as FP32 f_q - w_q, FP32 arithmetic             */
@@ -222,6 +225,32 @@ int mt_sync(mt_case* c);
 /* mass and momentum over lat_f and F2C of one level, and a NaN count       */
 int mt_reduce(mt_case* c, int level, double* mass, double* momentum3, int64_t* nan_count);
 /* accumulated IB force since the last reset (LBM_Manager.cpp:846-852, 964-979) */
+/* ---- run diagnostics as device reductions (LBM_Manager::infoPrint,
+ *      LBM_Manager.cpp:914-962; diagL2MaxF / diagL2RhoMin, :854-912).
+ *      mt_set_diagnostics(c, 1) makes every update of mt_step leave its
+ *      density and velocity, so the previous sub-step's velocity (the v_old that
+ *      calculateMacros keeps, :1003-1050) is at hand; it costs each update 4
+ *      reals per cell of extra writes.  mt_diagnostics then reduces, over the
+ *      state at the end of the last mt_step (FP64 sums in a fixed order):      */
+typedef struct mt_diag {
+  double velocity_error;  /* sqrt(sum |v - v_old|^2) / sqrt(sum |v|^2 + 1e-16), lat_f of every level */
+  double max_v[8];        /* per level: max |v| over lat_f and lat_overlap_F2C                   */
+  int64_t n_big[8];       /* per level: cells with |v| > 1                                       */
+  /* finest level, lat_f: diagL2MaxF                                                               */
+  double max_f, max_fcol;
+  uint64_t max_f_key, max_fcol_key;
+  int32_t max_f_q, max_fcol_q;
+  /* diagL2RhoMin: minimum sum f and sum fcol, counts of sum f <= 0, sum fcol <= 0, sum f < 0.5,
+   * the key of the minimum sum f and whether it is a solid-boundary (dis) cell                  */
+  double min_rho_f, min_rho_fcol;
+  int64_t n_neg_f, n_neg_fcol, n_low_f;
+  uint64_t worst_key;
+  int32_t worst_is_dis;
+  int32_t valid_v_old;    /* 1 when every update since the previous mt_step left its macros  */
+} mt_diag;
+int mt_set_diagnostics(mt_case* c, int on);
+int mt_diagnostics(mt_case* c, mt_diag* out);
+
 /* ---- per-member failure scan: the NaN checks of LBMkernel (LBM_Simulator.cpp:33-41,
  *      52-60, 78-86, 102-111, 128-137) reduced on the device per member of a packed
  *      case: bad_cells[i] = fluid and F2C cells of member i, all levels, holding a

@@ -51,7 +51,7 @@ EXPORTS = [
     "mt_collide", "mt_stream", "mt_explode", "mt_coalesce", "mt_apply_bc", "mt_ib_treat", "mt_step",
     "mt_set_graphs", "mt_set_schedule", "mt_set_timing", "mt_get_timing", "mt_sync", "mt_reduce", "mt_ib_force",
     "mt_case_set_members", "mt_ib_force_members", "mt_autotune", "mt_case_copy_member",
-    "mt_case_upload_markers", "mt_scan_members", "mt_server_create", "mt_server_submit", "mt_server_step",
+    "mt_case_upload_markers", "mt_scan_members", "mt_set_diagnostics", "mt_diagnostics", "mt_server_create", "mt_server_submit", "mt_server_step",
     "mt_server_poll",
     "mt_server_destroy",
 ]
@@ -61,6 +61,16 @@ _lib = None
 
 # MT_FIND_NOALLOC_MAX_KEYS of include/mt_amr.h: the largest call of mt_find_noalloc
 FIND_NOALLOC_MAX_KEYS = 1 << 18
+
+
+class Diag(C.Structure):
+    """mt_diag of include/mt_amr.h."""
+    _fields_ = [("velocity_error", C.c_double), ("max_v", C.c_double * 8), ("n_big", C.c_int64 * 8),
+                ("max_f", C.c_double), ("max_fcol", C.c_double), ("max_f_key", C.c_uint64),
+                ("max_fcol_key", C.c_uint64), ("max_f_q", C.c_int32), ("max_fcol_q", C.c_int32),
+                ("min_rho_f", C.c_double), ("min_rho_fcol", C.c_double), ("n_neg_f", C.c_int64),
+                ("n_neg_fcol", C.c_int64), ("n_low_f", C.c_int64), ("worst_key", C.c_uint64),
+                ("worst_is_dis", C.c_int32), ("valid_v_old", C.c_int32)]
 
 
 class MtError(RuntimeError):
@@ -110,6 +120,8 @@ def load():
     L.mt_server_poll.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), _dp,
                                  C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]
     L.mt_scan_members.argtypes = [C.c_void_p, C.c_int32, _i64p]
+    L.mt_set_diagnostics.argtypes = [C.c_void_p, C.c_int]
+    L.mt_diagnostics.argtypes = [C.c_void_p, C.POINTER(Diag)]
     L.mt_server_destroy.argtypes = [C.c_void_p]
     _lib = L
     return L
@@ -327,6 +339,23 @@ class AmrCase:
         n = C.c_int64()
         _check(load().mt_ib_force_members(self.h, s.ctypes.data_as(_dp), nm, C.byref(n), 1 if reset else 0))
         return s, n.value
+
+    def set_diagnostics(self, on=True):
+        """Every update of step() leaves its macros, so diagnostics() has v_old."""
+        _check(load().mt_set_diagnostics(self.h, 1 if on else 0))
+
+    def diagnostics(self):
+        """infoPrint's velocity error and per-level max |v| / n(|v| > 1), and the
+        finest level's diagL2MaxF / diagL2RhoMin, as device reductions."""
+        d = Diag()
+        _check(load().mt_diagnostics(self.h, C.byref(d)))
+        n = self.case["Lmax"] + 1
+        return {"velocity_error": d.velocity_error, "max_v": np.array(d.max_v[:n]),
+                "n_big": np.array(d.n_big[:n], dtype=np.int64), "max_f": d.max_f, "max_fcol": d.max_fcol,
+                "min_rho_f": d.min_rho_f, "min_rho_fcol": d.min_rho_fcol, "n_neg_f": d.n_neg_f,
+                "n_neg_fcol": d.n_neg_fcol, "n_low_f": d.n_low_f, "worst_is_dis": d.worst_is_dis,
+                "max_f_key": d.max_f_key, "max_fcol_key": d.max_fcol_key, "worst_key": d.worst_key,
+                "max_f_q": d.max_f_q, "max_fcol_q": d.max_fcol_q, "valid_v_old": d.valid_v_old}
 
     def scan_members(self):
         """Per member: fluid/F2C cells (all levels) holding a non-finite population (mt_scan_members)."""

@@ -53,6 +53,7 @@ COLLIDE_CUMULANT = 1
 IB_NONE = 0
 IB_FH = 1
 IB_DW = 2
+IB_UNIFORM = 3     # UniformBoundary::treat: seam path (mt_ib_treat) only
 
 # D3Q19 / D3Q27 velocity sets in the reference order (header/Constants.h:492-538)
 EX27 = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0,

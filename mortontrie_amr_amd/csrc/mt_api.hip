@@ -1,5 +1,6 @@
 // mt_api.hip -- the C-ABI (include/mt_amr.h) and the coarse-step schedule.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -325,7 +326,7 @@ static void drop_graphs(Case& c) {
 }
 
 static int parity_key(const Case& c, bool last) {
-  int k = last ? 1 : 0;
+  int k = (last ? 1 : 0) | (c.diag ? (1 << 20) : 0);
   for (int L = 0; L <= c.Lmax; ++L) k |= (c.lv[L].pf << (1 + 2 * L)) | (c.lv[L].pr << (2 + 2 * L));
   return k;
 }
@@ -462,6 +463,7 @@ int mt_case_create(const mt_case_desc* d, mt_case** out) {
   }
   if (d->precision < MT_FP64 || d->precision > MT_FP16_STORAGE) { set_error("bad precision"); return MT_ERR_ARG; }
   if (d->tau0 <= 0.5) { set_error("tau0 must exceed 1/2"); return MT_ERR_ARG; }
+  if (d->ib < MT_IB_NONE || d->ib > MT_IB_UNIFORM) { set_error("bad solid-boundary method"); return MT_ERR_ARG; }
   for (int f = 0; f < 6; ++f)
     if (d->face_kind[f] < MT_BC_NONE || d->face_kind[f] > MT_BC_NEE) { set_error("bad face kind"); return MT_ERR_ARG; }
   for (int a = 0; a < 3; ++a) {
@@ -855,6 +857,10 @@ int mt_ib_treat(mt_case* c, double* force3) {
 int mt_step(mt_case* c, int n) {
   CHECK_FINAL(c);
   if (n < 0) { set_error("negative step count"); return MT_ERR_ARG; }
+  if (c->d.ib == MT_IB_UNIFORM) {
+    set_error("the uniform solid boundary runs through the seam entry points (mt_ib_treat), not mt_step");
+    return MT_ERR_ARG;
+  }
   if (n == 0) return MT_OK;
   if (!c->ahead) TRY(prologue(*c));
   if (c->xin && !c->xprimed) {
@@ -864,6 +870,7 @@ int mt_step(mt_case* c, int n) {
   }
   use_level_streams(*c, true);
   int r = MT_OK;
+  if (c->diag) c->diag_steps += n;
   for (int i = 0; i < n && r == MT_OK;) {
     const int k = std::min(kStepsPerGraph, n - i);
     r = coarse_steps(*c, k, i + k == n);
@@ -982,6 +989,60 @@ int mt_reduce(mt_case* c, int level, double* mass, double* momentum3, int64_t* n
     momentum3[2] = out[3];
   }
   if (nan_count) *nan_count = (int64_t)out[4];
+  return MT_OK;
+}
+
+int mt_set_diagnostics(mt_case* c, int on) {
+  CHECK_FINAL(c);
+  if (c->diag != (on != 0)) {
+    c->diag = on != 0;
+    c->diag_steps = 0;
+    drop_graphs(*c);
+  }
+  return MT_OK;
+}
+
+int mt_diagnostics(mt_case* c, mt_diag* out) {
+  CHECK_FINAL(c);
+  if (!out) { set_error("null output"); return MT_ERR_ARG; }
+  if (c->Lmax >= 8) { set_error("mt_diag holds 8 levels"); return MT_ERR_ARG; }
+  TRY(settle(*c));
+  std::memset(out, 0, sizeof *out);
+  double gu = 0, gr = 0;
+  for (int L = 0; L <= c->Lmax; ++L) {
+    double d[16];
+    TRY(launch_diag(*c, L, d));
+    gu += d[0];
+    gr += d[1];
+    out->max_v[L] = d[2];
+    out->n_big[L] = (int64_t)d[3];
+    if (L == c->Lmax) {
+      DevLevel& l = c->lv[L];
+      auto key_at = [&](double slot, uint64_t* k) -> int {
+        if (slot < 0) { *k = 0; return MT_OK; }
+        MT_HIP(hipMemcpyAsync(k, l.key + (int64_t)slot, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+        return MT_OK;
+      };
+      out->max_f = d[4];
+      out->max_f_q = (int32_t)d[6];
+      out->max_fcol = d[7];
+      out->max_fcol_q = (int32_t)d[9];
+      out->min_rho_f = d[10];
+      out->min_rho_fcol = d[12];
+      out->n_neg_f = (int64_t)d[13];
+      out->n_neg_fcol = (int64_t)d[14];
+      out->n_low_f = (int64_t)d[15];
+      TRY(key_at(d[5], &out->max_f_key));
+      TRY(key_at(d[8], &out->max_fcol_key));
+      TRY(key_at(d[11], &out->worst_key));
+      uint16_t kd = 0;
+      if (d[11] >= 0) MT_HIP(hipMemcpyAsync(&kd, l.kind + (int64_t)d[11], 2, hipMemcpyDeviceToHost, c->stream));
+      MT_HIP(hipStreamSynchronize(c->stream));
+      out->worst_is_dis = (kd & MT_KIND_DIS) ? 1 : 0;
+    }
+  }
+  out->velocity_error = std::sqrt(gu) / std::sqrt(gr + 1e-16);
+  out->valid_v_old = (c->diag && c->diag_steps > 0) ? 1 : 0;
   return MT_OK;
 }
 

@@ -134,6 +134,8 @@ struct Case {
   bool ahead = false;  // fcol[pf] already holds the next collision (mt_step state)
   bool graphs = true;
   bool timing = false;
+  bool diag = false;  // every update leaves its macros (mt_set_diagnostics)
+  int64_t diag_steps = 0;
   // launch configuration (results do not depend on it; mt_autotune picks it)
   int xmap = 0;               // fused kernel block -> brick map (FusedArgs::xmap)
   bool level_streams = true;  // levels on concurrent streams in mt_step
@@ -257,6 +259,8 @@ int launch_reduce(Case& c, int L, double* h_out);
 // per member: fluid and F2C cells of any level holding a non-finite population
 // in either fcol buffer
 int launch_member_scan(Case& c, int64_t* h_counts);
+// infoPrint / diagL2* partial values of one level (k_diag's 16 values, combined)
+int launch_diag(Case& c, int L, double* h_out);
 
 }  // namespace mt
 

@@ -979,7 +979,41 @@ __global__ __launch_bounds__(256) void k_ib(SeamArgs a, const int32_t* dis_row, 
   __shared__ double red[12];
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double fx = 0, fy = 0, fz = 0;
-  if (s < a.v.ns) {
+  if (s < a.v.ns && method == MT_IB_UNIFORM) {
+    // UniformBoundary::treat (ComplexBoundary_Manager.cpp:15-40): every dis cell,
+    // directions in order; a wall link q writes f_inv(q) =
+    // (d (fcol_inv(q) + fcol_q) + (1 - d) fcol_q(x - e_q)) / (1 + d), any other
+    // direction pulls f_q = fcol_q(x + e_q); later directions overwrite earlier
+    // writes as the loop does; f_0 and the force are untouched
+    const uint16_t k = a.v.kind[s];
+    if ((k & MT_KIND_DIS) && dis_row[s] >= 0) {
+      const S* fin = (const S*)a.v.fcol_in;
+      const int64_t ns = a.v.ns;
+      const int32_t* bn = a.v.bnbr;
+      const uint32_t pm = a.v.pmask[s];
+      const double* drow = dis + (int64_t)dis_row[s] * (Q - 1);
+      R f[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) f[q] = load_pop<Q, S, R>((const S*)a.v.f, ns, q, (int)s);
+#pragma unroll
+      for (int q = 1; q < Q; ++q) {
+        const int qi = qinv(q);
+        const R d = (R)drow[qi - 1];
+        if (drow[qi - 1] != -1.0) {
+          if ((pm >> qi) & 1) {
+            const int t = slot_off(bn, (int)s, kEX[qi], kEY[qi], kEZ[qi]);
+            const R fo = load_pop<Q, S, R>(fin, ns, q, t);
+            f[qi] = (d * (load_pop<Q, S, R>(fin, ns, qi, (int)s) + load_pop<Q, S, R>(fin, ns, q, (int)s)) +
+                     ((R)1 - d) * fo) / ((R)1 + d);
+          }
+        } else if ((pm >> q) & 1) {
+          f[q] = load_pop<Q, S, R>(fin, ns, q, slot_off(bn, (int)s, kEX[q], kEY[q], kEZ[q]));
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)a.v.f, ns, q, (int)s, f[q]);
+    }
+  } else if (s < a.v.ns) {
     const uint16_t k = a.v.kind[s];
     if ((k & MT_KIND_DIS) && (k & MT_KIND_FLUID) && dis_row[s] >= 0) {
       const S* fin = (const S*)a.v.fcol_in;
@@ -1253,6 +1287,77 @@ __global__ void k_fill_u64(unsigned long long* p, int64_t n) {
   if (i < n) p[i] = 0;
 }
 
+// infoPrint's velocity diagnostics (LBM_Manager.cpp:914-962) and, on the finest
+// level, diagL2MaxF / diagL2RhoMin (:854-912), per block of 256 slots (grid
+// stride).  Partials, 16 per block:
+//   0 sum |v - v_old|^2 over lat_f, 1 sum |v|^2 over lat_f, 2 max |v| over
+//   lat_f and F2C, 3 cells with |v| > 1; finest level only: 4 max |f| (5 its
+//   slot, 6 its q), 7 max |fcol| (8, 9), 10 min sum f (11 its slot), 12 min
+//   sum fcol, 13 cells with sum f <= 0, 14 with sum fcol <= 0, 15 with sum f < 0.5.
+// Ties keep the lowest slot and q, so the result is independent of the order.
+constexpr int kDiagVals = 16;
+template <int Q, typename S>
+__global__ __launch_bounds__(256) void k_diag(LvArgs v, int finest, const void* u_prev, double* part) {
+  typedef typename RealOf<S>::type R;
+  __shared__ double lds[kDiagVals][256];
+  double d[kDiagVals] = {0, 0, 0, 0, 0, -1, 0, 0, -1, 0, 1e300, -1, 1e300, 0, 0, 0};
+  const int64_t ns = v.ns;
+  for (int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x; s < ns; s += (int64_t)gridDim.x * 256) {
+    const uint16_t k = v.kind[s];
+    if (!(k & (MT_KIND_FLUID | MT_KIND_F2C))) continue;
+    const double ux = (double)ld_real<R>(v.u_in, s), uy = (double)ld_real<R>(v.u_in, ns + s),
+                 uz = (double)ld_real<R>(v.u_in, 2 * ns + s);
+    const double vmag = sqrt(ux * ux + uy * uy + uz * uz);
+    if (vmag > d[2]) d[2] = vmag;
+    if (vmag > 1.0) d[3] += 1;
+    if (!(k & MT_KIND_FLUID)) continue;
+    const double ox = (double)ld_real<R>(u_prev, s), oy = (double)ld_real<R>(u_prev, ns + s),
+                 oz = (double)ld_real<R>(u_prev, 2 * ns + s);
+    d[0] += (ux - ox) * (ux - ox) + (uy - oy) * (uy - oy) + (uz - oz) * (uz - oz);
+    d[1] += ux * ux + uy * uy + uz * uz;
+    if (!finest) continue;
+    double rf = 0, rfc = 0;
+    for (int q = 0; q < Q; ++q) {
+      const double fv = (double)load_pop<Q, S, R>((const S*)v.f, ns, q, (int)s);
+      const double fc = (double)load_pop<Q, S, R>((const S*)v.fcol_in, ns, q, (int)s);
+      rf += fv;
+      rfc += fc;
+      if (fabs(fv) > d[4]) { d[4] = fabs(fv); d[5] = (double)s; d[6] = q; }
+      if (fabs(fc) > d[7]) { d[7] = fabs(fc); d[8] = (double)s; d[9] = q; }
+    }
+    if (rf < d[10]) { d[10] = rf; d[11] = (double)s; }
+    if (rfc < d[12]) d[12] = rfc;
+    if (rf <= 0.0) d[13] += 1;
+    if (rfc <= 0.0) d[14] += 1;
+    if (rf < 0.5) d[15] += 1;
+  }
+  for (int j = 0; j < kDiagVals; ++j) lds[j][threadIdx.x] = d[j];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // thread order: sums in a fixed order, extrema with the first (lowest
+    // slot) holder, each thread's slots ascending
+    for (int t = 1; t < 256; ++t) {
+      d[0] += lds[0][t];
+      d[1] += lds[1][t];
+      d[2] = fmax(d[2], lds[2][t]);
+      d[3] += lds[3][t];
+      auto better_max = [&](int vi, int si) {
+        const double x = lds[vi][t], xs = lds[si][t];
+        return x > d[vi] || (x == d[vi] && xs >= 0 && (d[si] < 0 || xs < d[si]));
+      };
+      if (better_max(4, 5)) { d[4] = lds[4][t]; d[5] = lds[5][t]; d[6] = lds[6][t]; }
+      if (better_max(7, 8)) { d[7] = lds[7][t]; d[8] = lds[8][t]; d[9] = lds[9][t]; }
+      const double x = lds[10][t], xs = lds[11][t];
+      if (x < d[10] || (x == d[10] && xs >= 0 && (d[11] < 0 || xs < d[11]))) { d[10] = x; d[11] = xs; }
+      d[12] = fmin(d[12], lds[12][t]);
+      d[13] += lds[13][t];
+      d[14] += lds[14][t];
+      d[15] += lds[15][t];
+    }
+    for (int j = 0; j < kDiagVals; ++j) part[(int64_t)blockIdx.x * kDiagVals + j] = d[j];
+  }
+}
+
 // ================================================================== launchers
 static inline int64_t nblk(int64_t n, int b = 256) { return n <= 0 ? 1 : (n + b - 1) / b; }
 
@@ -1359,7 +1464,8 @@ int launch_ib(Case& c, bool acc) {
     return MT_ERR_STATE;
   }
   MT_DISPATCH(c, (k_ib<Q_, S_><<<nb, 256, 0, c.stream>>>(a, l.dis_row, l.dis, c.d.ib, c.force_part)));
-  k_force_reduce<<<1, 256, 0, c.stream>>>(c.force_part, nb, c.force_acc);
+  // UniformBoundary reports no force (it never calls accumIBForce)
+  if (c.d.ib != MT_IB_UNIFORM) k_force_reduce<<<1, 256, 0, c.stream>>>(c.force_part, nb, c.force_acc);
   MT_HIP(hipGetLastError());
   return MT_OK;
 }
@@ -1451,7 +1557,7 @@ int launch_coalesce(Case& c, int L, bool collide_after, bool write_macros) {
   a.bmember_c = lc.bmember;
   a.Lc = L;
   a.collide = collide_after ? 1 : 0;
-  a.write_macros = write_macros ? 1 : 0;
+  a.write_macros = (write_macros || c.diag) ? 1 : 0;
   a.fcol_out = lc.fcol[1 - lc.pf];
   a.rho_out = lc.rho[1 - lc.pr];
   a.u_out = lc.u[1 - lc.pr];
@@ -1502,7 +1608,9 @@ static int fused_args(Case& c, int L, bool write_f, bool write_macros, bool c2f_
     set_error("force slot buffer too small");
     return MT_ERR_STATE;
   }
-  a.write_macros = write_macros ? 1 : 0;
+  // with diagnostics on, every update leaves its macros, so the previous
+  // update's stay in the other buffer (v_old of calculateMacros)
+  a.write_macros = (write_macros || c.diag) ? 1 : 0;
   a.c2f_fcol = c2f_fcol ? 1 : 0;
   a.c2f_f = (L > 0 && !c.lv[L - 1].coal_in_fused) ? 1 : 0;
   a.cpar = nullptr;
@@ -1552,7 +1660,7 @@ static int fused_args(Case& c, int L, bool write_f, bool write_macros, bool c2f_
     a.cfcol_out = lc.fcol[1 - lc.pf];
     a.crho_out = lc.rho[1 - lc.pr];
     a.cu_out = lc.u[1 - lc.pr];
-    a.c_write_macros = coal->write_macros;
+    a.c_write_macros = (coal->write_macros || c.diag) ? 1 : 0;
   }
   // grid: a multiple of 8 blocks so the XCD remap covers every brick index
   nb = ((l.n_active + 7) / 8) * 8;
@@ -1935,6 +2043,35 @@ int launch_member_scan(Case& c, int64_t* h_counts) {
   MT_HIP(hipMemcpyAsync(h.data(), c.member_bad, sizeof(unsigned long long) * nm, hipMemcpyDeviceToHost, c.stream));
   MT_HIP(hipStreamSynchronize(c.stream));
   for (int64_t i = 0; i < nm; ++i) h_counts[i] = (int64_t)h[i];
+  return MT_OK;
+}
+
+int launch_diag(Case& c, int L, double* h_out) {
+  DevLevel& l = c.lv[L];
+  LvArgs v = lvargs(c, L, l.pf, l.pf, l.pr, l.pr);
+  const int64_t nb = std::min<int64_t>(nblk(l.n_slots), c.red_cap / kDiagVals);
+  MT_DISPATCH(c, (k_diag<Q_, S_><<<nb, 256, 0, c.stream>>>(v, L == c.Lmax ? 1 : 0, l.u[1 - l.pr], c.red_tmp)));
+  MT_HIP(hipGetLastError());
+  std::vector<double> part(nb * kDiagVals);
+  MT_HIP(hipMemcpyAsync(part.data(), c.red_tmp, sizeof(double) * nb * kDiagVals, hipMemcpyDeviceToHost, c.stream));
+  MT_HIP(hipStreamSynchronize(c.stream));
+  double* d = h_out;
+  for (int j = 0; j < kDiagVals; ++j) d[j] = part[j];
+  for (int64_t b = 1; b < nb; ++b) {
+    const double* p = part.data() + b * kDiagVals;
+    d[0] += p[0];
+    d[1] += p[1];
+    d[2] = std::max(d[2], p[2]);
+    d[3] += p[3];
+    // blocks hold ascending slot ranges per grid-stride pass; ties keep the lowest slot
+    if (p[4] > d[4] || (p[4] == d[4] && p[5] >= 0 && (d[5] < 0 || p[5] < d[5]))) { d[4] = p[4]; d[5] = p[5]; d[6] = p[6]; }
+    if (p[7] > d[7] || (p[7] == d[7] && p[8] >= 0 && (d[8] < 0 || p[8] < d[8]))) { d[7] = p[7]; d[8] = p[8]; d[9] = p[9]; }
+    if (p[10] < d[10] || (p[10] == d[10] && p[11] >= 0 && (d[11] < 0 || p[11] < d[11]))) { d[10] = p[10]; d[11] = p[11]; }
+    d[12] = std::min(d[12], p[12]);
+    d[13] += p[13];
+    d[14] += p[14];
+    d[15] += p[15];
+  }
   return MT_OK;
 }
 

@@ -19,6 +19,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <stdexcept>
 #include <tuple>
 #include <unordered_map>
 #include <unordered_set>
@@ -784,8 +785,35 @@ struct Case {
     wet_bc();
   }
 
+  // UniformBoundary::treat (ComplexBoundary_Manager.cpp:15-40): every dis cell,
+  // directions in order; a missing neighbour throws std::out_of_range as .at() does
+  void uniform_treat() {
+    Level& l = lv[Lmax];
+    for (auto& d : dis) {
+      uint64_t k = d.first;
+      for (int q = 1; q < Q; ++q) {
+        double dist = d.second[inv(q) - 1];
+        if (dist != -1) {
+          uint64_t ki = nbr(k, Lmax, EX[inv(q)], EY[inv(q)], EZ[inv(q)]);
+          double fc = l.fcol[q].at(k);
+          double fci = l.fcol[inv(q)].at(k);
+          double fo = l.fcol[q].at(ki);
+          l.f[inv(q)].at(k) = (dist * (fci + fc) + (1. - dist) * fo) / (1. + dist);
+        } else {
+          uint64_t kn = nbr(k, Lmax, EX[q], EY[q], EZ[q]);
+          l.f[q].at(k) = l.fcol[q].at(kn);
+        }
+      }
+    }
+  }
+
   // SinglePointInterpolation::treat (ComplexBoundary_Manager.cpp:43-225)
   void ib_treat(double* F) {
+    if (ib == 3) {
+      uniform_treat();
+      if (F) F[0] = F[1] = F[2] = 0.0;
+      return;
+    }
     int L = Lmax;
     Level& l = lv[L];
     double fx = 0, fy = 0, fz = 0;
@@ -947,6 +975,7 @@ struct Dense {
   // the case's maps are not read (they need not be allocated)
   int build(const Case& c, bool fresh) {
     src = &c;
+    if (c.ib == 3) return 4;  // the uniform boundary runs in the map mode only
     Q = c.Q;
     Lmax = c.Lmax;
     lv.assign(Lmax + 1, {});
@@ -1643,7 +1672,14 @@ int mto_stream(void* h, int L) { ((Case*)h)->stream(L); return 0; }
 int mto_explode(void* h, int L) { ((Case*)h)->explode(L); return 0; }
 int mto_coalesce(void* h, int L) { ((Case*)h)->coalesce(L); return 0; }
 int mto_bc_after_stream(void* h) { ((Case*)h)->bc_after_stream(); return 0; }
-int mto_ib_treat(void* h, double* F) { ((Case*)h)->ib_treat(F); return 0; }
+int mto_ib_treat(void* h, double* F) {
+  try {
+    ((Case*)h)->ib_treat(F);
+  } catch (const std::out_of_range&) {
+    return 1;  // a neighbour the reference's .at() would not find
+  }
+  return 0;
+}
 int mto_marker_correct(void* h) { ((Case*)h)->marker_correct(); return 0; }
 int64_t mto_force_log(void* h, double* out, int64_t max) {
   Case* c = (Case*)h;
@@ -1717,6 +1753,67 @@ int64_t mto_dense_force_log(void* h, double* out, int64_t max) {
     for (int a = 0; a < 3; ++a) out[3 * i + a] = d->force_log[i][a];
   return n;
 }
+// infoPrint's velocity diagnostics (LBM_Manager.cpp:914-962) and diagL2MaxF /
+// diagL2RhoMin (:854-912) of the current state, cells in ascending key order
+// (ties keep the lowest key).  vals: 0 velocity error, 1..8 max |v| per level,
+// 9..16 cells with |v| > 1 per level, 17 max |f|, 18 max |fcol|, 19 min sum f,
+// 20 min sum fcol, 21 n sum f <= 0, 22 n sum fcol <= 0, 23 n sum f < 0.5,
+// 24 the worst cell is a dis cell; keys: max |f|, max |fcol|, worst; qs: their q.
+int mto_diagnostics(void* h, double* vals, uint64_t* keys, int* qs) {
+  Case* c = (Case*)h;
+  for (int i = 0; i < 25; ++i) vals[i] = 0;
+  double gu = 0, gr = 0;
+  for (int L = 0; L <= c->Lmax && L < 8; ++L) {
+    Level& l = c->lv[L];
+    double mx = 0, nb = 0;
+    auto vel = [&](uint64_t k) {
+      auto v = l.u.at(k);
+      double m = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      if (m > mx) mx = m;
+      if (m > 1.0) nb += 1;
+    };
+    for (uint64_t k : l.lat_f) {
+      vel(k);
+      auto v = l.u.at(k);
+      auto o = l.v_old.at(k);
+      gu += (v[0] - o[0]) * (v[0] - o[0]) + (v[1] - o[1]) * (v[1] - o[1]) + (v[2] - o[2]) * (v[2] - o[2]);
+      gr += v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    }
+    for (uint64_t k : l.f2c) vel(k);
+    vals[1 + L] = mx;
+    vals[9 + L] = nb;
+  }
+  vals[0] = std::sqrt(gu) / std::sqrt(gr + 1e-16);
+  Level& l = c->lv[c->Lmax];
+  double maxf = 0, maxfc = 0, minf = 1e300, minfc = 1e300, nnf = 0, nnfc = 0, nlow = 0;
+  keys[0] = keys[1] = keys[2] = 0;
+  qs[0] = qs[1] = 0;
+  for (uint64_t k : l.lat_f) {
+    double rf = 0, rfc = 0;
+    for (int q = 0; q < c->Q; ++q) {
+      double fv = l.f[q].at(k), fc = l.fcol[q].at(k);
+      rf += fv;
+      rfc += fc;
+      if (std::fabs(fv) > maxf) { maxf = std::fabs(fv); keys[0] = k; qs[0] = q; }
+      if (std::fabs(fc) > maxfc) { maxfc = std::fabs(fc); keys[1] = k; qs[1] = q; }
+    }
+    if (rf < minf) { minf = rf; keys[2] = k; }
+    if (rfc < minfc) minfc = rfc;
+    if (rf <= 0.0) nnf += 1;
+    if (rfc <= 0.0) nnfc += 1;
+    if (rf < 0.5) nlow += 1;
+  }
+  vals[17] = maxf;
+  vals[18] = maxfc;
+  vals[19] = minf;
+  vals[20] = minfc;
+  vals[21] = nnf;
+  vals[22] = nnfc;
+  vals[23] = nlow;
+  vals[24] = c->dis.count(keys[2]) ? 1 : 0;
+  return 0;
+}
+
 // ---- primitives for closed-form tests
 void mto_feq(int Q, double rho, const double* u, double* out) { feq_c(Q, rho, u, out); }
 void mto_feq_heluo(int Q, double rho, const double* u, double rho0, double* out) { feq_hl(Q, rho, u, rho0, out); }

@@ -40,6 +40,7 @@ def lib():
         L.mto_set_options.argtypes = [C.c_void_p, C.c_int]
         L.mto_set_tau.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.mto_set_mdf_sweeps.argtypes = [C.c_void_p, C.c_int]
+        L.mto_diagnostics.argtypes = [C.c_void_p, _dp, _u64p, _ip]
         L.mto_dense_create.restype = C.c_void_p
         L.mto_dense_create.argtypes = [C.c_void_p, C.c_int]
         L.mto_dense_destroy.argtypes = [C.c_void_p]
@@ -132,6 +133,22 @@ class OracleCase:
     def step(self, n=1):
         lib().mto_step(self.h, n)
 
+    def diagnostics(self):
+        """infoPrint's velocity error and per-level max |v|, diagL2MaxF and
+        diagL2RhoMin of the current state (the map mode keeps v_old as the
+        reference does)."""
+        vals = np.zeros(25)
+        keys = np.zeros(3, np.uint64)
+        qs = np.zeros(2, np.int32)
+        lib().mto_diagnostics(self.h, vals.ctypes.data_as(_dp), keys.ctypes.data_as(_u64p),
+                              qs.ctypes.data_as(_ip))
+        n = self.case["Lmax"] + 1
+        return {"velocity_error": vals[0], "max_v": vals[1:1 + n], "n_big": vals[9:9 + n].astype(np.int64),
+                "max_f": vals[17], "max_fcol": vals[18], "min_rho_f": vals[19], "min_rho_fcol": vals[20],
+                "n_neg_f": int(vals[21]), "n_neg_fcol": int(vals[22]), "n_low_f": int(vals[23]),
+                "worst_is_dis": int(vals[24]), "max_f_key": int(keys[0]), "max_fcol_key": int(keys[1]),
+                "worst_key": int(keys[2]), "max_f_q": int(qs[0]), "max_fcol_q": int(qs[1])}
+
     def set_mdf_sweeps(self, n):
         """Number of multi-direct forcing sweeps treat() runs (the reference's 8)."""
         lib().mto_set_mdf_sweeps(self.h, int(n))
@@ -192,7 +209,8 @@ class OracleCase:
 
     def ib_treat(self):
         F = np.zeros(3)
-        lib().mto_ib_treat(self.h, F.ctypes.data_as(_dp))
+        if lib().mto_ib_treat(self.h, F.ctypes.data_as(_dp)):
+            raise KeyError("treat: a neighbour of a solid-boundary cell is missing")
         return F
 
     def marker_correct(self):
@@ -266,6 +284,22 @@ class DenseOracleCase(OracleCase):
 
     def set_tau(self, taus):
         self._mapcase.set_tau(taus)
+
+    def diagnostics(self):
+        """infoPrint's velocity error and per-level max |v|, diagL2MaxF and
+        diagL2RhoMin of the current state (the map mode keeps v_old as the
+        reference does)."""
+        vals = np.zeros(25)
+        keys = np.zeros(3, np.uint64)
+        qs = np.zeros(2, np.int32)
+        lib().mto_diagnostics(self.h, vals.ctypes.data_as(_dp), keys.ctypes.data_as(_u64p),
+                              qs.ctypes.data_as(_ip))
+        n = self.case["Lmax"] + 1
+        return {"velocity_error": vals[0], "max_v": vals[1:1 + n], "n_big": vals[9:9 + n].astype(np.int64),
+                "max_f": vals[17], "max_fcol": vals[18], "min_rho_f": vals[19], "min_rho_fcol": vals[20],
+                "n_neg_f": int(vals[21]), "n_neg_fcol": int(vals[22]), "n_low_f": int(vals[23]),
+                "worst_is_dis": int(vals[24]), "max_f_key": int(keys[0]), "max_fcol_key": int(keys[1]),
+                "worst_key": int(keys[2]), "max_f_q": int(qs[0]), "max_fcol_q": int(qs[1])}
 
     def set_mdf_sweeps(self, n):
         self._mapcase.set_mdf_sweeps(n)

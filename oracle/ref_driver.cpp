@@ -115,7 +115,7 @@ int ref_set_dx(const double* dx) {
   return 0;
 }
 
-// ib: 1 FH, 2 distance-weighted.  May be called once per process: the
+// ib: 1 FH, 2 distance-weighted, 3 UniformBoundary.  May be called once per process: the
 // collision model is a function-local singleton that captures the first
 // manager's maps and relaxation times (Collision_Manager.hpp:33-34, 57-61).
 int ref_construct(int ib) {
@@ -128,6 +128,10 @@ int ref_construct(int ib) {
   if (ib == 2) {
     delete g_lbm->solidBCModel;
     g_lbm->solidBCModel = new SinglePointInterpolation(g_lbm->user, IB_Method::DISTANCE_WEIGHTED);
+  } else if (ib == 3) {
+    // the alternative the reference keeps commented out (LBM_Manager.cpp:46)
+    delete g_lbm->solidBCModel;
+    g_lbm->solidBCModel = new UniformBoundary(g_lbm->user);
   }
   return 0;
 }
@@ -239,6 +243,18 @@ int ref_bc_velocity(double* out) {
     out[3 * i + 1] = v.y;
     out[3 * i + 2] = v.z;
   }
+  return 0;
+}
+
+// the reference's printed diagnostics of the current state: 0 infoPrint(0)
+// (LBM_Manager.cpp:914-981), 1 diagL2MaxF, 2 diagL2RhoMin (:854-912)
+int ref_info(int which) {
+  if (!g_lbm) return 1;
+  if (which == 0) g_lbm->infoPrint(0);
+  else if (which == 1) g_lbm->diagL2MaxF("ref");
+  else if (which == 2) g_lbm->diagL2RhoMin("ref");
+  else return 2;
+  std::cout.flush();
   return 0;
 }
 

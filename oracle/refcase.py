@@ -21,6 +21,8 @@ Operations (rows of three ints):
                 5 the six AfterStream face strategies, 6 treat()
     (2, 0, 0)   record the state of every level and the running IB force sums
     (3, 0, 0)   reset the running IB force sums
+    (4, w, 0)   the reference's printed diagnostics, parsed: w = 0 infoPrint,
+                1 diagL2MaxF, 2 diagL2RhoMin (printed with 6 significant digits)
 """
 from __future__ import annotations
 
@@ -46,7 +48,7 @@ SET_FACES = [[0], [1], [2], [3], [4], [5], [1, 3], [0, 3], [0, 2], [1, 2], [3, 5
 MASK_TO_SET = {sum(1 << f for f in fs): i for i, fs in enumerate(SET_FACES)}
 C_DX = 0.1  # background spacing of the reference's sphere configuration (header/Constants.h:193)
 
-STEP, SEAM, SNAP, RESET_FORCE = 0, 1, 2, 3
+STEP, SEAM, SNAP, RESET_FORCE, INFO = 0, 1, 2, 3, 4
 MACROS, COLLIDE, STREAM, EXPLODE, COALESCE, BC, TREAT = range(7)
 
 
@@ -90,7 +92,32 @@ def run(case, state, ops, ib=1, timeout=600):
         for s in range(int(out["n_snap"])):
             levels = [{k: out[f"s{s}_{k}{L}"] for k in ("f", "fcol", "rho", "u")} for L in range(3)]
             snaps.append({"levels": levels, "force": out[f"s{s}_force"]})
-    return {"tau": tau, "bc_u": bc_u}, snaps
+        info = [str(out[f"info{i}"]) for i in range(int(out["n_info"]))]
+    return {"tau": tau, "bc_u": bc_u, "info": [parse_info(t) for t in info]}, snaps
+
+
+def parse_info(text):
+    """The numbers of one printed diagnostic (keys printed as 64-bit binary strings)."""
+    import re
+    num = r"([-+0-9.eE]+|nan|inf)"
+    out = {"text": text}
+    m = re.search(r"Velocity error = " + num, text)
+    if m:
+        out["velocity_error"] = float(m.group(1))
+        lv = re.findall(r"L(\d)=" + num + r"/(\d+)", text)
+        out["max_v"] = [float(v) for _, v, _ in lv]
+        out["n_big"] = [int(n) for _, _, n in lv]
+    m = re.search(r"max\|f\|=" + num + r" \(q(\d+),([01]+)\)\s+max\|fcol\|=" + num + r" \(q(\d+),([01]+)\)", text)
+    if m:
+        out.update(max_f=float(m.group(1)), max_f_q=int(m.group(2)), max_f_key=int(m.group(3), 2),
+                   max_fcol=float(m.group(4)), max_fcol_q=int(m.group(5)), max_fcol_key=int(m.group(6), 2))
+    m = re.search(r"min_rho_f=" + num + r" min_rho_fcol=" + num + r" n_neg_f=(\d+) n_neg_fcol=(\d+) "
+                  r"n_low_f\(<0.5\)=(\d+) \| worst: code=([01]+) rho_f=" + num, text)
+    if m:
+        out.update(min_rho_f=float(m.group(1)), min_rho_fcol=float(m.group(2)), n_neg_f=int(m.group(3)),
+                   n_neg_fcol=int(m.group(4)), n_low_f=int(m.group(5)), worst_key=int(m.group(6), 2),
+                   worst_is_dis=int("[IB]" in text))
+    return out
 
 
 # ---- child side ----------------------------------------------------------------
@@ -108,6 +135,7 @@ def _child(src, dst):
     lib.ref_call.argtypes = [C.c_int, C.c_int]
     lib.ref_force.argtypes = [dp, C.c_int]
     lib.ref_bc_velocity.argtypes = [dp]
+    lib.ref_info.argtypes = [C.c_int]
     assert lib.ref_q() == 19 and lib.ref_lmax() == 2
     d = np.load(src)
 
@@ -157,6 +185,7 @@ def _child(src, dst):
                 lib.ref_set(L, what, kp(keys[L]), n, fp(v))
     out = {"tau": tau, "bc_u": bcu.reshape(6, 3)}
     ns = 0
+    ni = 0
     for code, a, b in d["ops"]:
         if code == STEP:
             assert lib.ref_step(int(a)) == 0
@@ -176,6 +205,22 @@ def _child(src, dst):
         elif code == RESET_FORCE:
             F = np.zeros(4)
             lib.ref_force(fp(F), 1)
+        elif code == INFO:
+            # the reference prints: catch its stdout in a file for this call
+            sys.stdout.flush()
+            cap = os.path.join(os.path.dirname(dst), f"info{ni}.txt")
+            fd = os.open(cap, os.O_WRONLY | os.O_CREAT | os.O_TRUNC)
+            saved = os.dup(1)
+            os.dup2(fd, 1)
+            try:
+                assert lib.ref_info(int(a)) == 0
+            finally:
+                os.dup2(saved, 1)
+                os.close(saved)
+                os.close(fd)
+            out[f"info{ni}"] = np.array(open(cap).read())
+            ni += 1
+    out["n_info"] = np.int64(ni)
     out["n_snap"] = np.int64(ns)
     np.savez(dst, **out)
 

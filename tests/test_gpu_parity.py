@@ -302,6 +302,64 @@ def test_seams_match_reference_order():
         compare(g, o, case)
 
 
+@pytest.mark.parametrize("kw", [dict(options=2), dict(Q=19, collision=C.COLLIDE_LBGK)])
+def test_diagnostics_match_oracle(kw):
+    """The device reductions of infoPrint (velocity error with the previous
+    sub-step's velocity as v_old, per-level max |v|) and diagL2MaxF / diagL2RhoMin
+    equal the oracle's (whose are pinned to the reference's prints)."""
+    case = small_sphere(**kw)
+    g, o = amr.AmrCase(case), O.OracleCase(case)
+    g.set_diagnostics(True)
+    for n in (1, 3):
+        g.step(n)
+        o.step(n)
+        a, b = g.diagnostics(), o.diagnostics()
+        assert a["valid_v_old"] == 1
+        assert abs(a["velocity_error"] - b["velocity_error"]) <= 1e-10 * b["velocity_error"]
+        np.testing.assert_allclose(a["max_v"], b["max_v"], rtol=1e-12)
+        assert list(a["n_big"]) == list(b["n_big"])
+        for k in ("max_f", "max_fcol", "min_rho_f", "min_rho_fcol"):
+            assert abs(a[k] - b[k]) <= 1e-12 * abs(b[k]), k
+        for k in ("n_neg_f", "n_neg_fcol", "n_low_f"):
+            assert a[k] == b[k], k
+        # the arguments of the extrema: mirror-symmetric cells tie up to rounding,
+        # so the device's cell must hold the oracle's extremum, not be the same cell
+        Lf = case["Lmax"]
+        for key, q, field, m in (("max_f_key", "max_f_q", o.f, "max_f"), ("max_fcol_key", "max_fcol_q", o.fcol, "max_fcol")):
+            v = field(Lf, np.array([a[key]], np.uint64))[a[q], 0]
+            assert abs(abs(v) - b[m]) <= 1e-12 * b[m], key
+        wk = np.array([a["worst_key"]], np.uint64)
+        assert abs(o.f(case["Lmax"], wk).sum() - b["min_rho_f"]) < 1e-12
+    # the same run with diagnostics off computes the same bits
+    h = amr.AmrCase(case)
+    h.step(4)
+    for L in range(case["Lmax"] + 1):
+        assert np.array_equal(h.f(L), g.f(L))
+
+
+def test_uniform_boundary_seam():
+    """UniformBoundary::treat through mt_ib_treat equals the oracle's (the oracle's
+    is pinned to the reference's compiled code); mt_step refuses the method."""
+    case = small_sphere(Q=19, collision=C.COLLIDE_LBGK, ib=C.IB_UNIFORM)
+    g, o = amr.AmrCase(case), O.OracleCase(case)
+    rng = np.random.default_rng(21)
+    for L in range(case["Lmax"] + 1):
+        for what in (0, 1):
+            v = o.get(L, what)
+            live = (case["levels"][L]["kinds"] & C.KIND_SOLID) == 0
+            v[:, live] *= 1 + 0.03 * rng.standard_normal((v.shape[0], int(live.sum())))
+            o.set(L, what, v)
+            g.set(L, amr.FIELD_F if what == 0 else amr.FIELD_FCOL, v)
+    g.ib_treat()
+    o.ib_treat()
+    L = case["Lmax"]
+    assert rel(g.f(L), o.f(L)) < TOL
+    dis = (case["levels"][L]["kinds"] & C.KIND_DIS) != 0
+    assert not np.allclose(g.f(L)[:, dis], g.fcol(L)[:, dis])
+    with pytest.raises(amr.MtError, match="seam"):
+        g.step(1)
+
+
 def test_seams_then_fused_continue():
     """Seam calls after mt_step and mt_step after seams keep one consistent state."""
     case = C.cavity(12, refine_box=((4, 4, 4), (8, 8, 8)))

@@ -167,3 +167,49 @@ def test_ghost_explosion_reaches_ghost_children():
     assert changed[ghost].all() and ghost.sum() > 0
     assert not changed[~ghost & ((kinds & C.KIND_C2F) == 0)].any()
     assert_same_state(snaps[0], o, case, "explode 2")
+
+
+def test_uniform_boundary_matches_reference():
+    """UniformBoundary::treat (ComplexBoundary_Manager.cpp:15-40), the alternative
+    solid boundary the reference leaves unscheduled: one call from a perturbed state,
+    and three coarse steps with it as the solid-boundary model, bit for bit."""
+    case = ref_case(C.IB_UNIFORM)
+    st = perturbed_state(case, seed=13, amp=0.05)
+    par, snaps = R.run(case, st, [(R.SEAM, R.TREAT, 2), (R.SNAP, 0, 0), (R.STEP, 3, 0), (R.SNAP, 0, 0)],
+                       ib=3)
+    o = oracle_from(case, st, par["tau"])
+    o.ib_treat()
+    assert_same_state(snaps[0], o, case, "treat")
+    o.step(3)
+    assert_same_state(snaps[1], o, case, "steps")
+    assert snaps[1]["force"][3] == 0  # no accumIBForce
+
+
+def test_diagnostics_match_reference():
+    """infoPrint's velocity error and per-level max |v| (LBM_Manager.cpp:914-962),
+    diagL2MaxF and diagL2RhoMin (:854-912) of the oracle against the values the
+    reference prints (6 significant digits) after two coarse steps."""
+    case = ref_case()
+    st = perturbed_state(case, seed=17, amp=0.05)
+    par, _ = R.run(case, st, [(R.STEP, 2, 0), (R.INFO, 0, 0), (R.INFO, 1, 0), (R.INFO, 2, 0)])
+    o = oracle_from(case, st, par["tau"])
+    o.step(2)
+    d = o.diagnostics()
+    info, maxf, rho = par["info"]
+    assert abs(d["velocity_error"] - info["velocity_error"]) <= 1e-5 * abs(info["velocity_error"])
+    np.testing.assert_allclose(d["max_v"], info["max_v"], rtol=1e-5)
+    assert list(d["n_big"]) == info["n_big"]
+    for k in ("max_f", "max_fcol"):
+        assert abs(d[k] - maxf[k]) <= 1e-5 * abs(maxf[k]), k
+    assert d["max_f_q"] == maxf["max_f_q"] and d["max_fcol_q"] == maxf["max_fcol_q"]
+    assert d["max_f_key"] == maxf["max_f_key"] and d["max_fcol_key"] == maxf["max_fcol_key"]
+    for k in ("min_rho_f", "min_rho_fcol"):
+        assert abs(d[k] - rho[k]) <= 1e-5 * abs(rho[k]), k
+    for k in ("n_neg_f", "n_neg_fcol", "n_low_f"):
+        assert d[k] == rho[k], k
+    # the density clamp leaves several cells at the same minimum: the reference
+    # reports the first in hash order, the oracle the lowest key of the tie
+    wk = np.array([rho["worst_key"]], np.uint64)
+    assert abs(o.f(2, wk).sum() - d["min_rho_f"]) < 1e-12
+    assert rho["worst_is_dis"] == int(wk[0] in set(case["dis_keys"].tolist()))
+    assert d["velocity_error"] > 0
