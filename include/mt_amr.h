@@ -251,6 +251,38 @@ typedef struct mt_diag {
 int mt_set_diagnostics(mt_case* c, int on);
 int mt_diagnostics(mt_case* c, mt_diag* out);
 
+/* ---- the per-level cell sets built on the GPU from a geometry: replaces
+ *      Grid_Manager::initial + Lat_Manager::voxelize (Grid_Manager.cpp:40-134,
+ *      Lat_Voxelize.cpp:17-1285) for a body given as an analytic sphere or a
+ *      closed triangle mesh (an STL surface, Shape.cpp:341-548).  Level L+1
+ *      covers the level-L cells inside the body or within shells[L] (level-0
+ *      cell units) of its surface; the finest level's cells inside are solid;
+ *      link fractions come from the first crossing of the surface along each
+ *      link from a cell centre outside (Moller-Trumbore on the triangles,
+ *      Shape.cpp:809-843).  The sets (ascending keys) and dis are read back with
+ *      the copy calls and feed mt_case_upload_level / mt_case_upload_dis.      */
+#define MT_SHAPE_SPHERE 0
+#define MT_SHAPE_MESH 1
+typedef struct mt_levelset_desc {
+  int32_t Q, Lmax;
+  int32_t dims0[3];       /* level-0 cells per axis                                       */
+  int32_t face_kind[6];   /* MT_BC_*: periodic pairs wrap the level-0 refinement band;
+                             the level-0 face cells get the face bits                      */
+  int32_t reservoir;      /* 1: a frozen lat_bc shell around level 0 (Lat_Voxelize.cpp:242-288) */
+  int32_t shape;          /* MT_SHAPE_*                                                   */
+  int32_t solid;          /* 1: the body is solid on the finest level, with link fractions */
+  double shells[8];
+  double center[3], radius;   /* MT_SHAPE_SPHERE                                          */
+  const double* tri;          /* MT_SHAPE_MESH: [n_tri][3 vertices][xyz], level-0 units   */
+  int64_t n_tri;
+} mt_levelset_desc;
+typedef struct mt_levelset mt_levelset;
+int mt_levelset_build(const mt_levelset_desc* d, mt_levelset** out);
+int mt_levelset_counts(mt_levelset* ls, int64_t* n_cells_per_level, int64_t* n_dis);
+int mt_levelset_copy(mt_levelset* ls, int level, uint64_t* keys, uint16_t* kinds);
+int mt_levelset_copy_dis(mt_levelset* ls, uint64_t* keys, double* fractions);
+int mt_levelset_destroy(mt_levelset* ls);
+
 /* ---- per-member failure scan: the NaN checks of LBMkernel (LBM_Simulator.cpp:33-41,
  *      52-60, 78-86, 102-111, 128-137) reduced on the device per member of a packed
  *      case: bad_cells[i] = fluid and F2C cells of member i, all levels, holding a

@@ -51,7 +51,8 @@ EXPORTS = [
     "mt_collide", "mt_stream", "mt_explode", "mt_coalesce", "mt_apply_bc", "mt_ib_treat", "mt_step",
     "mt_set_graphs", "mt_set_schedule", "mt_set_timing", "mt_get_timing", "mt_sync", "mt_reduce", "mt_ib_force",
     "mt_case_set_members", "mt_ib_force_members", "mt_autotune", "mt_case_copy_member",
-    "mt_case_upload_markers", "mt_scan_members", "mt_set_diagnostics", "mt_diagnostics", "mt_server_create", "mt_server_submit", "mt_server_step",
+    "mt_case_upload_markers", "mt_scan_members", "mt_set_diagnostics", "mt_diagnostics", "mt_levelset_build",
+    "mt_levelset_counts", "mt_levelset_copy", "mt_levelset_copy_dis", "mt_levelset_destroy", "mt_server_create", "mt_server_submit", "mt_server_step",
     "mt_server_poll",
     "mt_server_destroy",
 ]
@@ -71,6 +72,17 @@ class Diag(C.Structure):
                 ("min_rho_f", C.c_double), ("min_rho_fcol", C.c_double), ("n_neg_f", C.c_int64),
                 ("n_neg_fcol", C.c_int64), ("n_low_f", C.c_int64), ("worst_key", C.c_uint64),
                 ("worst_is_dis", C.c_int32), ("valid_v_old", C.c_int32)]
+
+
+class LevelSetDesc(C.Structure):
+    """mt_levelset_desc of include/mt_amr.h."""
+    _fields_ = [("Q", C.c_int32), ("Lmax", C.c_int32), ("dims0", C.c_int32 * 3), ("face_kind", C.c_int32 * 6),
+                ("reservoir", C.c_int32), ("shape", C.c_int32), ("solid", C.c_int32), ("shells", C.c_double * 8),
+                ("center", C.c_double * 3), ("radius", C.c_double), ("tri", C.POINTER(C.c_double)),
+                ("n_tri", C.c_int64)]
+
+
+SHAPE_SPHERE, SHAPE_MESH = 0, 1
 
 
 class MtError(RuntimeError):
@@ -121,6 +133,11 @@ def load():
                                  C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]
     L.mt_scan_members.argtypes = [C.c_void_p, C.c_int32, _i64p]
     L.mt_set_diagnostics.argtypes = [C.c_void_p, C.c_int]
+    L.mt_levelset_build.argtypes = [C.POINTER(LevelSetDesc), C.POINTER(C.c_void_p)]
+    L.mt_levelset_counts.argtypes = [C.c_void_p, _i64p, _i64p]
+    L.mt_levelset_copy.argtypes = [C.c_void_p, C.c_int, _u64p, _u16p]
+    L.mt_levelset_copy_dis.argtypes = [C.c_void_p, _u64p, _dp]
+    L.mt_levelset_destroy.argtypes = [C.c_void_p]
     L.mt_diagnostics.argtypes = [C.c_void_p, C.POINTER(Diag)]
     L.mt_server_destroy.argtypes = [C.c_void_p]
     _lib = L
@@ -388,3 +405,47 @@ class AmrCase:
 
 def taus(case):
     return _cases.taus(case)
+
+
+def build_levels(Q, Lmax, dims0, faces, shells, center=None, radius=None, triangles=None, solid=True,
+                 reservoir=False):
+    """The per-level cell sets and link fractions built on the GPU (mt_levelset_*)
+    for an analytic sphere (center, radius) or a closed triangle mesh
+    (triangles: [n][3][3], level-0 units).  Returns (levels, dis_keys, dis_vals)
+    in the form of cases.build_nested."""
+    L = load()
+    d = LevelSetDesc()
+    d.Q, d.Lmax = Q, Lmax
+    d.dims0[:] = [int(v) for v in dims0]
+    d.face_kind[:] = [int(f[0]) for f in faces]
+    d.reservoir, d.solid = int(bool(reservoir)), int(bool(solid))
+    for i, v in enumerate(shells):
+        d.shells[i] = float(v)
+    tri = None
+    if triangles is None:
+        d.shape = SHAPE_SPHERE
+        d.center[:] = [float(v) for v in center]
+        d.radius = float(radius)
+    else:
+        d.shape = SHAPE_MESH
+        tri = np.ascontiguousarray(triangles, dtype=np.float64).reshape(-1, 9)
+        d.tri = tri.ctypes.data_as(_dp)
+        d.n_tri = tri.shape[0]
+    h = C.c_void_p()
+    _check(L.mt_levelset_build(C.byref(d), C.byref(h)))
+    try:
+        n = np.zeros(Lmax + 1, dtype=np.int64)
+        nd = C.c_int64()
+        _check(L.mt_levelset_counts(h, n.ctypes.data_as(_i64p), C.byref(nd)))
+        levels = []
+        for lv in range(Lmax + 1):
+            keys = np.zeros(n[lv], dtype=np.uint64)
+            kinds = np.zeros(n[lv], dtype=np.uint16)
+            _check(L.mt_levelset_copy(h, lv, keys.ctypes.data_as(_u64p), kinds.ctypes.data_as(_u16p)))
+            levels.append({"keys": keys, "kinds": kinds})
+        dk = np.zeros(nd.value, dtype=np.uint64)
+        dv = np.zeros((nd.value, Q - 1))
+        _check(L.mt_levelset_copy_dis(h, dk.ctypes.data_as(_u64p), dv.ctypes.data_as(_dp)))
+    finally:
+        L.mt_levelset_destroy(h)
+    return levels, dk, dv

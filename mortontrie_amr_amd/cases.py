@@ -376,11 +376,14 @@ def sphere_ib_dist(center, radius):
 
 def sphere_amr(coarse=(64, 32, 32), radius=8.0, center=(16.0, 16.0, 16.0), shells=(6.0, 3.0),
                Q=27, collision=COLLIDE_CUMULANT, U=0.04, Re=100.0, ib=IB_FH, les_C=0.0,
-               periodic_yz=True, name=None, keep_solid=True, with_sphere=True, options=0, periodic_x=False):
+               periodic_yz=True, name=None, keep_solid=True, with_sphere=True, options=0, periodic_x=False,
+               builder="host"):
     """Refined sphere in a channel: Zou-He inflow u_x=U at x=0, outflow at x-max,
     periodic y and z; level L+1 covers coarse cells whose centre lies within
     radius+shells[L] of the sphere centre; cells inside the sphere are solid on the
-    finest level and the cut links carry analytic fractions.  Re = U*2R/nu0."""
+    finest level and the cut links carry analytic fractions.  Re = U*2R/nu0.
+    builder="gpu" builds the level sets on the device (mt_levelset_build) with
+    the same rules and arithmetic as the host builder."""
     Lmax = len(shells)
     nu0 = abs(U) * 2.0 * radius / Re
     tau0 = 3.0 * nu0 + 0.5
@@ -400,8 +403,13 @@ def sphere_amr(coarse=(64, 32, 32), radius=8.0, center=(16.0, 16.0, 16.0), shell
     def solid(cx, cy, cz):
         return r_of(cx, cy, cz) < radius
 
-    levels, dk, dv = build_nested(coarse, Lmax, refine, solid=solid if with_sphere else None, faces=faces, Q=Q,
-                                  ib_dist=sphere_ib_dist(C, radius) if with_sphere else None)
+    if builder == "gpu":
+        from . import amr
+        levels, dk, dv = amr.build_levels(Q, Lmax, coarse, faces, shells, center=C, radius=radius,
+                                          solid=with_sphere)
+    else:
+        levels, dk, dv = build_nested(coarse, Lmax, refine, solid=solid if with_sphere else None, faces=faces,
+                                      Q=Q, ib_dist=sphere_ib_dist(C, radius) if with_sphere else None)
     if not keep_solid:
         top = levels[Lmax]
         keep = (top["kinds"] & KIND_SOLID) == 0
@@ -459,6 +467,220 @@ def reference_sphere(coarse=(24, 16, 16), radius=3.0, center=(8.0, 8.0, 8.0), sh
                                   ib_dist=sphere_ib_dist(C, radius), reservoir=True)
     return _base_case(Q, Lmax, tau0, collision, coarse, faces, levels, dk, dv, u0=(U, 0.0, 0.0), ib=ib,
                       cd={"U": U, "d": 2.0 * radius * (1 << Lmax)}, name=name or "reference_sphere")
+
+
+# ---- bodies given as triangle meshes -----------------------------------------------
+def icosphere(center, radius, subdiv=2):
+    """A closed triangle mesh of the sphere: the icosahedron, each face split into
+    4 subdiv times, vertices pushed onto the sphere; [n][3][3], outward winding."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    V = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    V = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in V]
+    F = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(i, j):
+            k = (min(i, j), max(i, j))
+            if k not in cache:
+                m = V[i] + V[j]
+                V.append(m / np.linalg.norm(m))
+                cache[k] = len(V) - 1
+            return cache[k]
+
+        G = []
+        for a, b, c in F:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            G += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        F = G
+    P = np.array(V) * float(radius) + np.asarray(center, dtype=np.float64)
+    return P[np.array(F)]
+
+
+def read_stl(path, scale=1.0, offset=(0.0, 0.0, 0.0)):
+    """Triangles [n][3][3] of an ASCII STL file (the format Shape::geofile_stl reads,
+    Shape.cpp:341-548): 'vertex x y z' lines in threes, then scale and offset."""
+    verts = []
+    with open(path) as fh:
+        for line in fh:
+            w = line.split()
+            if w and w[0] == "vertex":
+                verts.append([float(w[1]), float(w[2]), float(w[3])])
+    T = np.array(verts, dtype=np.float64).reshape(-1, 3, 3)
+    return T * float(scale) + np.asarray(offset, dtype=np.float64)
+
+
+def _tri_dist2(T, px, py, pz):
+    """Squared distance from the points to the triangle T (closest point by regions,
+    Ericson, Real-Time Collision Detection 5.1.5), vectorised over the points."""
+    A, B, Cv = T
+    ab, ac = B - A, Cv - A
+    apx, apy, apz = px - A[0], py - A[1], pz - A[2]
+    d1 = ab[0] * apx + ab[1] * apy + ab[2] * apz
+    d2 = ac[0] * apx + ac[1] * apy + ac[2] * apz
+    bpx, bpy, bpz = px - B[0], py - B[1], pz - B[2]
+    d3 = ab[0] * bpx + ab[1] * bpy + ab[2] * bpz
+    d4 = ac[0] * bpx + ac[1] * bpy + ac[2] * bpz
+    cpx, cpy, cpz = px - Cv[0], py - Cv[1], pz - Cv[2]
+    d5 = ab[0] * cpx + ab[1] * cpy + ab[2] * cpz
+    d6 = ac[0] * cpx + ac[1] * cpy + ac[2] * cpz
+    vc = d1 * d4 - d3 * d2
+    vb = d5 * d2 - d1 * d6
+    va = d3 * d6 - d5 * d4
+    with np.errstate(divide="ignore", invalid="ignore"):
+        v_ab = d1 / (d1 - d3)
+        w_ac = d2 / (d2 - d6)
+        w_bc = (d4 - d3) / ((d4 - d3) + (d5 - d6))
+        den = 1.0 / (va + vb + vc)
+    v_in, w_in = vb * den, vc * den
+
+    def d2of(qx, qy, qz):
+        return (px - qx) * (px - qx) + (py - qy) * (py - qy) + (pz - qz) * (pz - qz)
+
+    out = d2of(A[0] + ab[0] * v_in + ac[0] * w_in, A[1] + ab[1] * v_in + ac[1] * w_in, A[2] + ab[2] * v_in + ac[2] * w_in)
+    r = (va <= 0) & ((d4 - d3) >= 0) & ((d5 - d6) >= 0)
+    out = np.where(r, d2of(B[0] + w_bc * (Cv[0] - B[0]), B[1] + w_bc * (Cv[1] - B[1]), B[2] + w_bc * (Cv[2] - B[2])), out)
+    r = (vb <= 0) & (d2 >= 0) & (d6 <= 0)
+    out = np.where(r, d2of(A[0] + w_ac * ac[0], A[1] + w_ac * ac[1], A[2] + w_ac * ac[2]), out)
+    r = (d6 >= 0) & (d5 <= d6)
+    out = np.where(r, d2of(Cv[0], Cv[1], Cv[2]), out)
+    r = (vc <= 0) & (d1 >= 0) & (d3 <= 0)
+    out = np.where(r, d2of(A[0] + v_ab * ab[0], A[1] + v_ab * ab[1], A[2] + v_ab * ab[2]), out)
+    r = (d3 >= 0) & (d4 <= d3)
+    out = np.where(r, d2of(B[0], B[1], B[2]), out)
+    r = (d1 <= 0) & (d2 <= 0)
+    out = np.where(r, d2of(A[0], A[1], A[2]), out)
+    return out
+
+
+class MeshBody:
+    """Host (numpy) geometry of a closed triangle mesh, the rules the device
+    builder applies: inside by the generalised winding number (Van Oosterom-
+    Strackee solid angles), distance to the surface, and the first crossing along
+    a link (Moller-Trumbore, Shape.cpp:809-843)."""
+
+    def __init__(self, triangles):
+        self.T = np.asarray(triangles, dtype=np.float64).reshape(-1, 3, 3)
+        self.lo = self.T.reshape(-1, 3).min(axis=0)
+        self.hi = self.T.reshape(-1, 3).max(axis=0)
+
+    def _near(self, cx, cy, cz, margin):
+        """Broadcast point arrays and the mask of points within `margin` of the
+        mesh's bounding box (the only ones a triangle test can change)."""
+        px, py, pz = np.broadcast_arrays(cx, cy, cz)
+        m = np.ones(px.shape, bool)
+        for a, p in enumerate((px, py, pz)):
+            m &= (p >= self.lo[a] - margin) & (p <= self.hi[a] + margin)
+        return px, py, pz, m
+
+    def inside(self, cx, cy, cz):
+        px, py, pz, m = self._near(cx, cy, cz, 0.0)
+        out = np.zeros(px.shape, bool)
+        if m.any():
+            out[m] = self._inside(px[m], py[m], pz[m])
+        return out
+
+    def _inside(self, px, py, pz):
+        w = np.zeros(px.shape)
+        for T in self.T:
+            a = [T[0][k] - p for k, p in enumerate((px, py, pz))]
+            b = [T[1][k] - p for k, p in enumerate((px, py, pz))]
+            c = [T[2][k] - p for k, p in enumerate((px, py, pz))]
+            la = np.sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2])
+            lb = np.sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2])
+            lc = np.sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2])
+            det = a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) + a[2] * (b[0] * c[1] - b[1] * c[0])
+            den = la * lb * lc + (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) * lc + \
+                (a[0] * c[0] + a[1] * c[1] + a[2] * c[2]) * lb + (b[0] * c[0] + b[1] * c[1] + b[2] * c[2]) * la
+            w = w + 2.0 * np.arctan2(det, den)
+        return np.abs(w) > 2.0 * np.pi
+
+    def dist2(self, cx, cy, cz, margin=np.inf):
+        """Squared distance to the surface; points farther than `margin` from the
+        bounding box read inf."""
+        px, py, pz, m = self._near(cx, cy, cz, margin)
+        out = np.full(px.shape, np.inf)
+        if m.any():
+            sub = np.full(int(m.sum()), np.inf)
+            qx, qy, qz = px[m], py[m], pz[m]
+            for T in self.T:
+                sub = np.minimum(sub, _tri_dist2(T, qx, qy, qz))
+            out[m] = sub
+        return out
+
+    def refine(self, shell):
+        def f(L, cx, cy, cz):
+            return self.inside(cx, cy, cz) | (self.dist2(cx, cy, cz, shell[L] * 1.001) <= shell[L] * shell[L])
+        return f
+
+    def link(self, cx, cy, cz, ex, ey, ez, h):
+        px, py, pz, m = self._near(cx, cy, cz, 2.0 * h)
+        best = np.full(px.shape, 2.0)
+        if m.any():
+            best[m] = self._link(px[m], py[m], pz[m], ex, ey, ez, h)
+        return best
+
+    def _link(self, px, py, pz, ex, ey, ez, h):
+        d = (h * ex, h * ey, h * ez)
+        best = np.full(px.shape, 2.0)
+        for T in self.T:
+            e1 = T[1] - T[0]
+            e2 = T[2] - T[0]
+            sx, sy, sz = px - T[0][0], py - T[0][1], pz - T[0][2]
+            hx = d[1] * e2[2] - d[2] * e2[1]
+            hy = d[2] * e2[0] - d[0] * e2[2]
+            hz = d[0] * e2[1] - d[1] * e2[0]
+            a = e1[0] * hx + e1[1] * hy + e1[2] * hz
+            if abs(a) < 1e-300:
+                continue
+            f = 1.0 / a
+            u = f * (sx * hx + sy * hy + sz * hz)
+            qx = sy * e1[2] - sz * e1[1]
+            qy = sz * e1[0] - sx * e1[2]
+            qz = sx * e1[1] - sy * e1[0]
+            v = f * (d[0] * qx + d[1] * qy + d[2] * qz)
+            t = f * (e2[0] * qx + e2[1] * qy + e2[2] * qz)
+            ok = (u >= 0) & (u <= 1) & (v >= 0) & (u + v <= 1) & (t >= 0)
+            best = np.where(ok & (t < best), t, best)
+        return best
+
+    def ib_dist(self):
+        def dist(cx, cy, cz, ex, ey, ez, h):
+            out = self.link(cx, cy, cz, float(ex), float(ey), float(ez), h)
+            inside = self.inside(cx, cy, cz)
+            return np.where((out <= 1.0) & ~inside, out, np.nan)
+        return dist
+
+
+def mesh_amr(triangles, coarse=(64, 32, 32), shells=(6.0, 3.0), Q=27, collision=COLLIDE_CUMULANT, U=0.04, Re=100.0,
+             d_ref=None, ib=IB_FH, les_C=0.0, options=2, name=None, builder="gpu"):
+    """A body given as a closed triangle mesh (an STL surface) in the channel of
+    sphere_amr: Zou-He inflow at x=0, outflow at x-max, periodic y/z.  Level L+1
+    covers the cells inside the body or within shells[L] of its surface; the finest
+    level's inside cells are solid and the cut links carry Moller-Trumbore
+    fractions.  Re = U d_ref / nu0 with d_ref the body's extent along y by default.
+    builder="gpu" builds the level sets on the device (mt_levelset_build), "host"
+    with numpy (the same rules; slow, for checking)."""
+    T = np.asarray(triangles, dtype=np.float64).reshape(-1, 3, 3)
+    Lmax = len(shells)
+    if d_ref is None:
+        d_ref = float(T[..., 1].max() - T[..., 1].min())
+    nu0 = abs(U) * d_ref / Re
+    tau0 = 3.0 * nu0 + 0.5
+    ur = reported(U)
+    faces = [(BC_ZOUHE, ur, 0.0, 0.0), (BC_OUTFLOW, 0.0, 0.0, 0.0)] + [(BC_PERIODIC, 0.0, 0.0, 0.0)] * 4
+    if builder == "gpu":
+        from . import amr
+        levels, dk, dv = amr.build_levels(Q, Lmax, coarse, faces, shells, triangles=T, solid=True)
+    else:
+        body = MeshBody(T)
+        levels, dk, dv = build_nested(coarse, Lmax, body.refine(shells), solid=body.inside, faces=faces, Q=Q,
+                                      ib_dist=body.ib_dist())
+    return _base_case(Q, Lmax, tau0, collision, coarse, faces, levels, dk, dv, u0=(ur, 0.0, 0.0), ib=ib,
+                      les_C=les_C, cd={"U": ur, "d": d_ref * (1 << Lmax)}, name=name or "mesh_amr", options=options)
 
 
 # ---- packed batching ---------------------------------------------------------

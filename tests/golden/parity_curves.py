@@ -12,9 +12,11 @@ sum rho and sum |u|^2 per level.
     python tests/golden/parity_curves.py device cavity OUT.npz   # the HIP path
     python tests/golden/parity_curves.py compare A.npz B.npz     # rel-L2 per checkpoint
 
-The oracle run of the sphere takes about half an hour on 8 cores; its samples at
-100, 1000 and 4000 steps are kept as tests/golden/sphere_4000.npz, which
-tests/test_gpu_long.py compares the device with.
+The oracle run of the sphere takes about 40 minutes on 8 cores; every fourth
+sampled cell at 100, 1000 and 4000 steps, with a tolerance of 1e-12 on the relative
+L2 of rho and u, is kept as tests/golden/sphere_4000.npz (`python
+tests/golden/parity_curves.py fixture SPHERE_ORACLE.npz`), which
+tests/test_gpu_workloads.py compares the device with.
 """
 from __future__ import annotations
 
@@ -104,8 +106,23 @@ def compare(a_path, b_path):
         print(f"{int(s):5d} " + "  ".join(row))
 
 
+def fixture(oracle_path):
+    src = np.load(oracle_path)
+    out = {"checkpoints": np.array([100, 1000, 4000])}
+    for L in range(3):
+        out[f"keys_L{L}"] = src[f"keys_L{L}"][::4]
+    for s in (100, 1000, 4000):
+        for L in range(3):
+            out[f"rho_L{L}_s{s}"] = src[f"rho_L{L}_s{s}"][::4]
+            out[f"u_L{L}_s{s}"] = src[f"u_L{L}_s{s}"][:, ::4]
+        out[f"tol_s{s}"] = np.array([1e-12, 1e-12])
+    np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "sphere_4000.npz"), **out)
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "compare":
         compare(sys.argv[2], sys.argv[3])
+    elif sys.argv[1] == "fixture":
+        fixture(sys.argv[2])
     else:
         run(sys.argv[1], sys.argv[2], sys.argv[3])

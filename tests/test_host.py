@@ -181,3 +181,35 @@ def test_pack_layout_and_key_translation():
 def test_pack_rejects_mixed_models():
     with pytest.raises(ValueError):
         C.pack([C.cavity(8), C.cavity(8, Q=27)])
+
+
+def test_mesh_body_geometry(tmp_path):
+    """The host mesh rules the device builder mirrors: winding-number inside,
+    distance to the surface and Moller-Trumbore link crossings on an icosphere
+    against the sphere it approximates; the ASCII STL reader on the same mesh."""
+    Cc = np.array([5.2, 4.9, 5.1])
+    T = C.icosphere(Cc, 2.0, 3)
+    body = C.MeshBody(T)
+    rng = np.random.default_rng(2)
+    p = Cc + rng.uniform(-4, 4, (2000, 3))
+    r = np.linalg.norm(p - Cc, axis=1)
+    keep = np.abs(r - 2.0) > 0.05  # the mesh lies within 2 * (1 - cos 9.7 deg) < 0.03 of the sphere
+    ins = body.inside(p[:, 0], p[:, 1], p[:, 2])
+    assert np.array_equal(ins[keep], (r < 2.0)[keep])
+    d = np.sqrt(body.dist2(p[:, 0], p[:, 1], p[:, 2]))
+    assert np.abs(d - np.abs(r - 2.0)).max() < 0.03
+    # a link from outside straight at the centre crosses at r - 2 (within the facet sag)
+    q = Cc + np.array([3.0, 0.0, 0.0])
+    t = body.link(q[0], q[1], q[2], -1.0, 0.0, 0.0, 2.0)
+    assert abs(float(t) * 2.0 - 1.0) < 0.03
+    assert float(body.link(q[0], q[1], q[2], 1.0, 0.0, 0.0, 2.0)) > 1.0  # pointing away: no crossing
+    f = tmp_path / "body.stl"
+    with open(f, "w") as fh:
+        fh.write("solid s\n")
+        for tri in T:
+            fh.write(" facet normal 0 0 0\n  outer loop\n")
+            for v in tri:
+                fh.write(f"   vertex {float(v[0])!r} {float(v[1])!r} {float(v[2])!r}\n")
+            fh.write("  endloop\n endfacet\n")
+        fh.write("endsolid s\n")
+    assert np.array_equal(C.read_stl(str(f)), T)
