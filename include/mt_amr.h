@@ -193,6 +193,14 @@ int mt_ib_treat(mt_case* c, double* force3);
  *      fused per-level kernels, captured as HIP graphs when graphs are on.    */
 int mt_step(mt_case* c, int n_coarse);
 int mt_set_graphs(mt_case* c, int on);
+/* Per-level relaxation times of one member as given, in place of
+ * tau_L = 2^L (tau0 - 1/2) + 1/2 from the descriptor.  The reference computes
+ * each level's tau from the viscosity and that level's conversion factors
+ * (LBM_Manager::setUnit, LBM_Manager.cpp:814-817), which can differ from the
+ * doubling rule in the last bit; the transfer factors 2 tau_f / tau_c and
+ * tau_c / (2 tau_f) (LBM_Manager.cpp:1065, 1137) follow from the values given.
+ * n_levels must be Lmax + 1, each tau > 1/2.                                   */
+int mt_set_tau(mt_case* c, int32_t member, const double* tau, int32_t n_levels);
 /* How mt_step issues the levels' fused updates (no counterpart in the
  * reference; every schedule computes the same bits):
  *   MT_SCHED_MERGED   one launch per finest sub-step on one stream, carrying
@@ -282,6 +290,21 @@ int mt_levelset_counts(mt_levelset* ls, int64_t* n_cells_per_level, int64_t* n_d
 int mt_levelset_copy(mt_levelset* ls, int level, uint64_t* keys, uint16_t* kinds);
 int mt_levelset_copy_dis(mt_levelset* ls, uint64_t* keys, double* fractions);
 int mt_levelset_destroy(mt_levelset* ls);
+
+/* ---- field snapshots and output files (the reference's IO_Manager::writeFlow ->
+ *      IO_TECPLOTH::write, tecplot.cpp:16-177, and the AMR node dump of
+ *      Obj_Manager::initial, Obj_Manager.cpp:483-489).  mt_snapshot compacts the
+ *      cells of one level whose kind has a bit of kind_mask (0: lat_f |
+ *      lat_overlap_F2C) on the device, in ascending key order, with density and
+ *      velocity (u as [3][n]); with keys, rho and u all NULL it returns the count
+ *      in *n, otherwise *n holds the buffers' capacity on entry.
+ *      mt_write_tecplot writes the FEBRICK zone of every level's lat_f and F2C
+ *      cells: node coordinates in units of dx0 per level-0 cell, U,V,W = velocity
+ *      x convert_u.  mt_write_nodes writes the finest level's keys, one 64-bit
+ *      binary string per line.                                                  */
+int mt_snapshot(mt_case* c, int level, uint32_t kind_mask, int64_t* n, uint64_t* keys, double* rho, double* u);
+int mt_write_tecplot(mt_case* c, const char* path, double dx0, double convert_u);
+int mt_write_nodes(mt_case* c, const char* path);
 
 /* ---- per-member failure scan: the NaN checks of LBMkernel (LBM_Simulator.cpp:33-41,
  *      52-60, 78-86, 102-111, 128-137) reduced on the device per member of a packed

@@ -49,10 +49,11 @@ EXPORTS = [
     "mt_last_error", "mt_case_create", "mt_case_destroy", "mt_case_upload_level", "mt_case_upload_dis",
     "mt_case_finalize", "mt_case_level_info", "mt_find", "mt_find_noalloc", "mt_get_field", "mt_set_field", "mt_macros",
     "mt_collide", "mt_stream", "mt_explode", "mt_coalesce", "mt_apply_bc", "mt_ib_treat", "mt_step",
-    "mt_set_graphs", "mt_set_schedule", "mt_set_timing", "mt_get_timing", "mt_sync", "mt_reduce", "mt_ib_force",
+    "mt_set_graphs", "mt_set_tau", "mt_set_schedule", "mt_set_timing", "mt_get_timing", "mt_sync", "mt_reduce", "mt_ib_force",
     "mt_case_set_members", "mt_ib_force_members", "mt_autotune", "mt_case_copy_member",
     "mt_case_upload_markers", "mt_scan_members", "mt_set_diagnostics", "mt_diagnostics", "mt_levelset_build",
-    "mt_levelset_counts", "mt_levelset_copy", "mt_levelset_copy_dis", "mt_levelset_destroy", "mt_server_create", "mt_server_submit", "mt_server_step",
+    "mt_levelset_counts", "mt_levelset_copy", "mt_levelset_copy_dis", "mt_levelset_destroy", "mt_snapshot",
+    "mt_write_tecplot", "mt_write_nodes", "mt_server_create", "mt_server_submit", "mt_server_step",
     "mt_server_poll",
     "mt_server_destroy",
 ]
@@ -138,7 +139,11 @@ def load():
     L.mt_levelset_copy.argtypes = [C.c_void_p, C.c_int, _u64p, _u16p]
     L.mt_levelset_copy_dis.argtypes = [C.c_void_p, _u64p, _dp]
     L.mt_levelset_destroy.argtypes = [C.c_void_p]
+    L.mt_snapshot.argtypes = [C.c_void_p, C.c_int, C.c_uint32, _i64p, _u64p, _dp, _dp]
+    L.mt_write_tecplot.argtypes = [C.c_void_p, C.c_char_p, C.c_double, C.c_double]
+    L.mt_write_nodes.argtypes = [C.c_void_p, C.c_char_p]
     L.mt_diagnostics.argtypes = [C.c_void_p, C.POINTER(Diag)]
+    L.mt_set_tau.argtypes = [C.c_void_p, C.c_int32, _dp, C.c_int32]
     L.mt_server_destroy.argtypes = [C.c_void_p]
     _lib = L
     return L
@@ -311,6 +316,11 @@ class AmrCase:
 
     SCHEDULES = {"merged": 0, "streams": 1, "serial": 2}
 
+    def set_tau(self, tau, member=0):
+        """Per-level relaxation times of one member as given (mt_set_tau)."""
+        t = np.ascontiguousarray(tau, np.float64)
+        _check(load().mt_set_tau(self.h, member, t.ctypes.data_as(_dp), t.size))
+
     def set_schedule(self, name):
         """How mt_step issues the levels' updates: "merged" (one launch per finest
         sub-step, the default), "streams" (a stream per level) or "serial"."""
@@ -373,6 +383,26 @@ class AmrCase:
                 "n_neg_fcol": d.n_neg_fcol, "n_low_f": d.n_low_f, "worst_is_dis": d.worst_is_dis,
                 "max_f_key": d.max_f_key, "max_fcol_key": d.max_fcol_key, "worst_key": d.worst_key,
                 "max_f_q": d.max_f_q, "max_fcol_q": d.max_fcol_q, "valid_v_old": d.valid_v_old}
+
+    def snapshot(self, level, kind_mask=0):
+        """(keys, rho, u[3][n]) of the level's cells of the given kinds (default
+        lat_f and F2C), compacted on the device in ascending key order."""
+        n = C.c_int64()
+        _check(load().mt_snapshot(self.h, level, kind_mask, C.byref(n), None, None, None))
+        keys = np.zeros(n.value, np.uint64)
+        rho = np.zeros(n.value)
+        u = np.zeros((3, n.value))
+        _check(load().mt_snapshot(self.h, level, kind_mask, C.byref(n), keys.ctypes.data_as(_u64p),
+                                  rho.ctypes.data_as(_dp), u.ctypes.data_as(_dp)))
+        return keys, rho, u
+
+    def write_tecplot(self, path, dx0=1.0, convert_u=1.0):
+        """The reference's Tecplot FEBRICK flow file (IO_TECPLOTH::write)."""
+        _check(load().mt_write_tecplot(self.h, str(path).encode(), float(dx0), float(convert_u)))
+
+    def write_nodes(self, path):
+        """The finest level's keys as 64-bit binary strings (the AMR node dump)."""
+        _check(load().mt_write_nodes(self.h, str(path).encode()))
 
     def scan_members(self):
         """Per member: fluid/F2C cells (all levels) holding a non-finite population (mt_scan_members)."""

@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmt_amr.so")
-SOURCES = ["mt_tables.hip", "mt_kernels.hip", "mt_api.hip", "mt_serve.hip", "mt_voxel.hip"]
+SOURCES = ["mt_tables.hip", "mt_kernels.hip", "mt_api.hip", "mt_serve.hip", "mt_voxel.hip", "mt_output.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MT_OFFLOAD_ARCH", "gfx950")
 

@@ -992,6 +992,19 @@ int mt_reduce(mt_case* c, int level, double* mass, double* momentum3, int64_t* n
   return MT_OK;
 }
 
+int mt_set_tau(mt_case* c, int32_t member, const double* tau, int32_t n_levels) {
+  CHECK_FINAL(c);
+  if (member < 0 || member >= (int32_t)c->h_mem.size()) { set_error("member out of range"); return MT_ERR_ARG; }
+  if (!tau || n_levels != c->Lmax + 1) { set_error("tau needs one value per level"); return MT_ERR_ARG; }
+  for (int L = 0; L <= c->Lmax; ++L)
+    if (!(tau[L] > 0.5) || !std::isfinite(tau[L])) { set_error("tau must exceed 1/2"); return MT_ERR_ARG; }
+  MemberParams& P = c->h_mem[member];
+  set_member_tau(*c, P, tau);
+  MT_HIP(hipMemcpyAsync(c->d_mem + member, &P, sizeof P, hipMemcpyHostToDevice, c->stream));
+  MT_HIP(hipStreamSynchronize(c->stream));
+  return MT_OK;
+}
+
 int mt_set_diagnostics(mt_case* c, int on) {
   CHECK_FINAL(c);
   if (c->diag != (on != 0)) {

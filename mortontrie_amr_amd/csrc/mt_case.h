@@ -259,6 +259,7 @@ int launch_reduce(Case& c, int L, double* h_out);
 // per member: fluid and F2C cells of any level holding a non-finite population
 // in either fcol buffer
 int launch_member_scan(Case& c, int64_t* h_counts);
+void set_member_tau(const Case& c, MemberParams& P, const double* tau);
 // infoPrint / diagL2* partial values of one level (k_diag's 16 values, combined)
 int launch_diag(Case& c, int L, double* h_out);
 

@@ -577,19 +577,26 @@ int copy_member(Case& dst, int32_t dm, Case& src, int32_t sm) {
   return r;
 }
 
+// tau_L of a member and the transfer factors that follow from it.
+void set_member_tau(const Case& c, MemberParams& P, const double* tau) {
+  const bool conv = (c.d.options & MT_OPT_CONVECTIVE_XFER) != 0;
+  for (int L = 0; L <= c.Lmax; ++L) P.tau[L] = tau[L];
+  for (int L = 1; L <= c.Lmax; ++L) P.xscale[L] = conv ? P.tau[L] / (2.0 * P.tau[L - 1]) : (2.0 * P.tau[L]) / P.tau[L - 1];
+  for (int L = 0; L < c.Lmax; ++L) P.cscale[L] = conv ? (2.0 * P.tau[L]) / P.tau[L + 1] : P.tau[L] / (2.0 * P.tau[L + 1]);
+}
+
 // Per-member parameters on the device: tau_L = 2^L (tau0 - 1/2) + 1/2
 // (LBM_Manager.cpp:814-818), the transfer factors 2 tau_f / tau_c and
 // tau_c / (2 tau_f) (LBM_Manager.cpp:1065, 1137; swapped for the convective
 // option), the wet-node face configuration and the member's box per level.
 int build_members(Case& c) {
-  const bool conv = (c.d.options & MT_OPT_CONVECTIVE_XFER) != 0;
   c.h_mem.clear();
   c.wet_any = c.res_any = false;
   for (const mt_member_desc& m : c.h_mdesc) {
     MemberParams P = {};
-    for (int L = 0; L <= c.Lmax; ++L) P.tau[L] = double(1 << L) * (m.tau0 - 0.5) + 0.5;
-    for (int L = 1; L <= c.Lmax; ++L) P.xscale[L] = conv ? P.tau[L] / (2.0 * P.tau[L - 1]) : (2.0 * P.tau[L]) / P.tau[L - 1];
-    for (int L = 0; L < c.Lmax; ++L) P.cscale[L] = conv ? (2.0 * P.tau[L]) / P.tau[L + 1] : P.tau[L] / (2.0 * P.tau[L + 1]);
+    double tau[kMaxLevels];
+    for (int L = 0; L <= c.Lmax; ++L) tau[L] = double(1 << L) * (m.tau0 - 0.5) + 0.5;
+    set_member_tau(c, P, tau);
     P.les_C = m.les_C;
     P.rho0 = m.rho0;
     for (int a = 0; a < 3; ++a) P.u0[a] = m.u0[a];

@@ -54,6 +54,7 @@
 #include "ComplexBoundary_Manager.hpp"
 #include "BC/D3Q19_BC_Manager.hpp"
 #include "BC/NonEquilibrumExtrapolation.hpp"
+#include "tecplot.h"
 #undef private
 #undef protected
 
@@ -255,6 +256,17 @@ int ref_info(int which) {
   else if (which == 2) g_lbm->diagL2RhoMin("ref");
   else return 2;
   std::cout.flush();
+  return 0;
+}
+
+// IO_TECPLOTH::write(0) (tecplot.cpp:16-177): output/sphereFlow_0.000000.dat
+// under the current directory; *cf_u receives the velocity factor it applies
+int ref_write_tecplot(double* cf_u) {
+  if (!g_lbm) return 1;
+  ensureOutputDir();
+  IO_TECPLOTH t;
+  t.write(0);
+  *cf_u = g_lbm->CF_U;
   return 0;
 }
 

@@ -23,6 +23,7 @@ Operations (rows of three ints):
     (3, 0, 0)   reset the running IB force sums
     (4, w, 0)   the reference's printed diagnostics, parsed: w = 0 infoPrint,
                 1 diagL2MaxF, 2 diagL2RhoMin (printed with 6 significant digits)
+    (5, 0, 0)   the reference's Tecplot flow file (IO_TECPLOTH::write), its text
 """
 from __future__ import annotations
 
@@ -48,7 +49,7 @@ SET_FACES = [[0], [1], [2], [3], [4], [5], [1, 3], [0, 3], [0, 2], [1, 2], [3, 5
 MASK_TO_SET = {sum(1 << f for f in fs): i for i, fs in enumerate(SET_FACES)}
 C_DX = 0.1  # background spacing of the reference's sphere configuration (header/Constants.h:193)
 
-STEP, SEAM, SNAP, RESET_FORCE, INFO = 0, 1, 2, 3, 4
+STEP, SEAM, SNAP, RESET_FORCE, INFO, TECPLOT = 0, 1, 2, 3, 4, 5
 MACROS, COLLIDE, STREAM, EXPLODE, COALESCE, BC, TREAT = range(7)
 
 
@@ -93,7 +94,8 @@ def run(case, state, ops, ib=1, timeout=600):
             levels = [{k: out[f"s{s}_{k}{L}"] for k in ("f", "fcol", "rho", "u")} for L in range(3)]
             snaps.append({"levels": levels, "force": out[f"s{s}_force"]})
         info = [str(out[f"info{i}"]) for i in range(int(out["n_info"]))]
-    return {"tau": tau, "bc_u": bc_u, "info": [parse_info(t) for t in info]}, snaps
+        plt = [(str(out[f"plt{i}"]), float(out[f"plt_cfu{i}"])) for i in range(int(out["n_plt"]))]
+    return {"tau": tau, "bc_u": bc_u, "info": [parse_info(t) for t in info], "tecplot": plt}, snaps
 
 
 def parse_info(text):
@@ -120,6 +122,34 @@ def parse_info(text):
     return out
 
 
+def perturbed_state(case, seed, amp=0.02):
+    """The oracle's initial state of `case` with every live cell's f and fcol (the
+    frozen reservoir and the solid excepted) scaled by 1 + amp * N(0, 1) per
+    population (numpy default_rng(seed)), and the densities and velocities made
+    consistent with f: a state far enough from equilibrium that every term of
+    every formula matters."""
+    from oracle import pyoracle as O
+    sys.path.insert(0, ROOT)
+    from mortontrie_amr_amd import cases as C
+    o = O.OracleCase(case)
+    rng = np.random.default_rng(seed)
+    Q = case["Q"]
+    ex, ey, ez = C.velocities(Q)
+    st = []
+    for L in range(case["Lmax"] + 1):
+        kinds = case["levels"][L]["kinds"].astype(np.int64)
+        live = (kinds & (C.KIND_SOLID | C.KIND_BCRES)) == 0
+        f, fc, rho, u = o.f(L), o.fcol(L), o.rho(L), o.u(L)
+        for a in (f, fc):
+            a[:, live] *= 1.0 + amp * rng.standard_normal((Q, int(live.sum())))
+        has = live & ~np.isnan(rho)
+        r = f[:, has].sum(0)
+        rho[has] = r
+        u[:, has] = np.stack([(f[:, has] * e[:, None]).sum(0) for e in (ex, ey, ez)]) / r
+        st.append({"f": f, "fcol": fc, "rho": rho, "u": u})
+    return st
+
+
 # ---- child side ----------------------------------------------------------------
 def _child(src, dst):
     lib = C.CDLL(REF_LIB, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
@@ -136,6 +166,7 @@ def _child(src, dst):
     lib.ref_force.argtypes = [dp, C.c_int]
     lib.ref_bc_velocity.argtypes = [dp]
     lib.ref_info.argtypes = [C.c_int]
+    lib.ref_write_tecplot.argtypes = [dp]
     assert lib.ref_q() == 19 and lib.ref_lmax() == 2
     d = np.load(src)
 
@@ -186,6 +217,7 @@ def _child(src, dst):
     out = {"tau": tau, "bc_u": bcu.reshape(6, 3)}
     ns = 0
     ni = 0
+    npl = 0
     for code, a, b in d["ops"]:
         if code == STEP:
             assert lib.ref_step(int(a)) == 0
@@ -220,7 +252,20 @@ def _child(src, dst):
                 os.close(fd)
             out[f"info{ni}"] = np.array(open(cap).read())
             ni += 1
+        elif code == TECPLOT:
+            cfu = np.zeros(1)
+            here = os.getcwd()
+            os.chdir(os.path.dirname(dst))
+            try:
+                assert lib.ref_write_tecplot(fp(cfu)) == 0
+                name = [f for f in os.listdir("output") if f.startswith("sphereFlow")][0]
+                out[f"plt{npl}"] = np.array(open(os.path.join("output", name)).read())
+            finally:
+                os.chdir(here)
+            out[f"plt_cfu{npl}"] = cfu[0]
+            npl += 1
     out["n_info"] = np.int64(ni)
+    out["n_plt"] = np.int64(npl)
     out["n_snap"] = np.int64(ns)
     np.savez(dst, **out)
 

@@ -58,26 +58,7 @@ def ref_case(ib=C.IB_FH, solid=True):
 
 
 def perturbed_state(case, seed, amp=0.02):
-    """The oracle's initial state with every live cell's f and fcol (the frozen
-    reservoir and the solid excepted) moved off equilibrium by amp*w_q noise, and
-    the densities and velocities made consistent with f."""
-    o = O.OracleCase(case)
-    rng = np.random.default_rng(seed)
-    w = O.feq(19, 1.0, np.zeros(3))
-    ex, ey, ez = C.velocities(19)
-    st = []
-    for L in range(case["Lmax"] + 1):
-        kinds = case["levels"][L]["kinds"].astype(np.int64)
-        live = (kinds & (C.KIND_SOLID | C.KIND_BCRES)) == 0
-        f, fc, rho, u = o.f(L), o.fcol(L), o.rho(L), o.u(L)
-        for a in (f, fc):
-            a[:, live] *= 1.0 + amp * rng.standard_normal((19, int(live.sum())))
-        has = live & ~np.isnan(rho)
-        r = f[:, has].sum(0)
-        rho[has] = r
-        u[:, has] = np.stack([(f[:, has] * e[:, None]).sum(0) for e in (ex, ey, ez)]) / r
-        st.append({"f": f, "fcol": fc, "rho": rho, "u": u})
-    return st
+    return R.perturbed_state(case, seed, amp)
 
 
 def oracle_from(case, state, tau):
