@@ -145,6 +145,30 @@ int mt_case_set_members(mt_case* c, const mt_member_desc* members, int32_t n);
  * the physical flow is positive).  Before mt_case_finalize; single-member cases. */
 int mt_case_upload_markers(mt_case* c, int64_t n, const uint64_t* keys27, const double* weights27,
                            const double* dV, const double* ub3);
+/* Moving markers: the FSI marker interface the reference sketches (Infor_IB and
+ * map_node_IB, the cells within reach of the moving solid points, kept up to
+ * date as the solid moves: header/Grid_Class.h:101-109, 250-253,
+ * header/Grid_Manager.h:50-52, Grid_Manager_dynamic.cpp:595-803), on the device.
+ * Positions are in finest-level cell units (cell i spans [i, i + 1)); a
+ * marker's kernel cells are the nodes within 3/2 per axis, weighted by the
+ * three-point kernel of Roma, Peskin & Berger (J. Comput. Phys. 153, 1999), and
+ * must all be streamed fluid cells of the finest level.
+ *   mt_markers_move: new positions xyz[n][3] and velocities ub[n][3] (reported
+ *     frame, as mt_get_field's; NULL keeps them) for all n markers;
+ *   mt_markers_rigid: moves every marker with a rigid motion over dt finest
+ *     time steps: x' = center + vel dt + Rot(omega dt)(x - center) (Rodrigues;
+ *     vel, omega physical, lattice units), marker velocity vel + omega x r'
+ *     (stored in the reported frame); needs positions from mt_markers_move;
+ *   mt_markers_get: the positions (NULL to skip) and velocities.
+ * The kernel cells, weights and band are rebuilt on the device (a sort of the
+ * (cell, marker) pairs when some marker's cells change, new weights in place
+ * otherwise).  The new positions act from the next collision on.  A move that
+ * would put a kernel cell outside the streamed fluid cells of the finest level
+ * returns MT_ERR_ARG naming the first such marker and changes nothing.  After
+ * mt_case_finalize.                                                            */
+int mt_markers_move(mt_case* c, int64_t n, const double* xyz, const double* ub);
+int mt_markers_rigid(mt_case* c, double dt, const double* center3, const double* vel3, const double* omega3);
+int mt_markers_get(mt_case* c, int64_t n, double* xyz, double* ub);
 int mt_case_level_info(mt_case* c, int level, int64_t* n_cells, int64_t* n_bricks);
 
 /* ---- key lookup: replaces D_map_define<T>::find (mortonTrieMap.cpp:116-126,

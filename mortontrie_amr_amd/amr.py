@@ -51,7 +51,7 @@ EXPORTS = [
     "mt_collide", "mt_stream", "mt_explode", "mt_coalesce", "mt_apply_bc", "mt_ib_treat", "mt_step",
     "mt_set_graphs", "mt_set_tau", "mt_set_schedule", "mt_set_timing", "mt_get_timing", "mt_sync", "mt_reduce", "mt_ib_force",
     "mt_case_set_members", "mt_ib_force_members", "mt_autotune", "mt_case_copy_member",
-    "mt_case_upload_markers", "mt_scan_members", "mt_set_diagnostics", "mt_diagnostics", "mt_levelset_build",
+    "mt_case_upload_markers", "mt_markers_move", "mt_markers_rigid", "mt_markers_get", "mt_scan_members", "mt_set_diagnostics", "mt_diagnostics", "mt_levelset_build",
     "mt_levelset_counts", "mt_levelset_copy", "mt_levelset_copy_dis", "mt_levelset_destroy", "mt_snapshot",
     "mt_write_tecplot", "mt_write_nodes", "mt_server_create", "mt_server_submit", "mt_server_step",
     "mt_server_poll",
@@ -144,6 +144,9 @@ def load():
     L.mt_write_nodes.argtypes = [C.c_void_p, C.c_char_p]
     L.mt_diagnostics.argtypes = [C.c_void_p, C.POINTER(Diag)]
     L.mt_set_tau.argtypes = [C.c_void_p, C.c_int32, _dp, C.c_int32]
+    L.mt_markers_move.argtypes = [C.c_void_p, C.c_int64, _dp, _dp]
+    L.mt_markers_rigid.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp]
+    L.mt_markers_get.argtypes = [C.c_void_p, C.c_int64, _dp, _dp]
     L.mt_server_destroy.argtypes = [C.c_void_p]
     _lib = L
     return L
@@ -403,6 +406,28 @@ class AmrCase:
     def write_nodes(self, path):
         """The finest level's keys as 64-bit binary strings (the AMR node dump)."""
         _check(load().mt_write_nodes(self.h, str(path).encode()))
+
+    def move_markers(self, xyz, ub=None):
+        """New marker positions [n][3] (finest cells) and velocities (reported frame;
+        None keeps them); kernel cells and band rebuilt on the device."""
+        x = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        v = None if ub is None else np.ascontiguousarray(ub, np.float64).reshape(-1, 3)
+        _check(load().mt_markers_move(self.h, x.shape[0], x.ctypes.data_as(_dp),
+                                      None if v is None else v.ctypes.data_as(_dp)))
+
+    def move_markers_rigid(self, dt, center, vel=(0.0, 0.0, 0.0), omega=(0.0, 0.0, 0.0)):
+        """Rigid motion of every marker over dt finest time steps (mt_markers_rigid);
+        vel and omega physical, lattice units."""
+        a = [np.ascontiguousarray(v, np.float64).reshape(3) for v in (center, vel, omega)]
+        _check(load().mt_markers_rigid(self.h, float(dt), *[v.ctypes.data_as(_dp) for v in a]))
+
+    def markers(self):
+        """(positions [n][3], velocities [n][3] in the reported frame)."""
+        n = len(self.case["markers"]["dv"])
+        x = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        _check(load().mt_markers_get(self.h, n, x.ctypes.data_as(_dp), v.ctypes.data_as(_dp)))
+        return x, v
 
     def scan_members(self):
         """Per member: fluid/F2C cells (all levels) holding a non-finite population (mt_scan_members)."""

@@ -141,6 +141,12 @@ static bool has_reservoir(const Case& c) { return c.res_any; }
 // buffer already holds the next collision and the reference's fcol is the other.
 static int settle(Case& c) {
   if (!c.ahead) return MT_OK;
+  if (c.mk_acc_ahead) {
+    // the discarded collision's marker force leaves the accumulated force (its
+    // collision is redone, and counted, by the next step)
+    TRY(launch_mk_unforce(c));
+    c.mk_acc_ahead = false;
+  }
   if (c.xin && c.xprimed) {
     // the last pair's explosion targets sit in fcol[1 - pf of the coarser level]
     // (the other buffer already holds the next pair's): move them to the
@@ -163,6 +169,7 @@ static int prologue(Case& c) {
   // the markers act on every finest-level collision
   TRY(launch_markers(c, c.lv[c.Lmax].pf, c.st[c.Lmax]));
   c.ahead = true;
+  c.mk_acc_ahead = c.n_mk > 0;
   return MT_OK;
 }
 
@@ -622,6 +629,44 @@ int mt_case_upload_markers(mt_case* c, int64_t n, const uint64_t* keys27, const 
   c->h_mk_w.assign(w27, w27 + 27 * n);
   c->h_mk_dv.assign(dv, dv + n);
   for (int a = 0; a < 3 && n; ++a) c->mk_ub[a] = ub3[a];
+  return MT_OK;
+}
+
+int mt_markers_move(mt_case* c, int64_t n, const double* xyz, const double* ub) {
+  CHECK_FINAL(c);
+  if (!c->n_mk) { set_error("the case has no markers"); return MT_ERR_STATE; }
+  if (n != c->n_mk || !xyz) { set_error("positions of all n markers needed"); return MT_ERR_ARG; }
+  TRY(settle(*c));
+  MT_HIP(hipMemcpyAsync(c->mv_x, xyz, 8 * 3 * n, hipMemcpyHostToDevice, c->stream));
+  if (ub)
+    MT_HIP(hipMemcpyAsync(c->mv_v, ub, 8 * 3 * n, hipMemcpyHostToDevice, c->stream));
+  else
+    MT_HIP(hipMemcpyAsync(c->mv_v, c->mk_v, 8 * 3 * n, hipMemcpyDeviceToDevice, c->stream));
+  return move_markers(*c, false, 0.0, nullptr, nullptr, nullptr);
+}
+
+int mt_markers_rigid(mt_case* c, double dt, const double* center3, const double* vel3, const double* omega3) {
+  CHECK_FINAL(c);
+  if (!c->n_mk) { set_error("the case has no markers"); return MT_ERR_STATE; }
+  if (!c->mk_x_set) { set_error("marker positions unknown: set them with mt_markers_move first"); return MT_ERR_STATE; }
+  if (!center3 || !vel3 || !omega3) { set_error("null motion argument"); return MT_ERR_ARG; }
+  for (int a = 0; a < 3; ++a)
+    if (!std::isfinite(center3[a]) || !std::isfinite(vel3[a]) || !std::isfinite(omega3[a]) || !std::isfinite(dt)) {
+      set_error("non-finite motion argument");
+      return MT_ERR_ARG;
+    }
+  TRY(settle(*c));
+  return move_markers(*c, true, dt, center3, vel3, omega3);
+}
+
+int mt_markers_get(mt_case* c, int64_t n, double* xyz, double* ub) {
+  CHECK_FINAL(c);
+  if (!c->n_mk) { set_error("the case has no markers"); return MT_ERR_STATE; }
+  if (n != c->n_mk) { set_error("n differs from the case's marker count"); return MT_ERR_ARG; }
+  if (xyz && !c->mk_x_set) { set_error("marker positions unknown: set them with mt_markers_move first"); return MT_ERR_STATE; }
+  if (xyz) MT_HIP(hipMemcpyAsync(xyz, c->mk_x, 8 * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  if (ub) MT_HIP(hipMemcpyAsync(ub, c->mk_v, 8 * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  MT_HIP(hipStreamSynchronize(c->stream));
   return MT_OK;
 }
 
@@ -1087,6 +1132,7 @@ int mt_ib_force(mt_case* c, double* sum3, int64_t* count, int reset) {
   }
   if (count) *count = (int64_t)(acc[3] + acc[7]);
   if (reset) {
+    c->mk_acc_ahead = false;
     MT_HIP(hipMemsetAsync(c->force_acc, 0, 4 * sizeof(double), c->stream));
     MT_HIP(hipMemsetAsync(c->force_acc + 7, 0, sizeof(double), c->stream));
     MT_HIP(hipMemsetAsync(c->force_slots, 0, sizeof(double) * slots.size(), c->stream));
@@ -1137,6 +1183,7 @@ int mt_ib_force_members(mt_case* c, double* sum3, int32_t n_members, int64_t* co
     for (int a = 0; a < 3; ++a) sum3[3 * bm[act[b]] + a] += slots[3 * b + a];
   if (count) *count = (int64_t)(acc[3] + acc[7]);
   if (reset) {
+    c->mk_acc_ahead = false;
     MT_HIP(hipMemsetAsync(c->force_acc, 0, 4 * sizeof(double), c->stream));
     MT_HIP(hipMemsetAsync(c->force_acc + 7, 0, sizeof(double), c->stream));
     MT_HIP(hipMemsetAsync(c->force_slots, 0, sizeof(double) * slots.size(), c->stream));

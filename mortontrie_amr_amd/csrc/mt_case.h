@@ -128,6 +128,22 @@ struct Case {
   int32_t* band_ptr = nullptr;      // [n_band + 1]
   int32_t* band_mk = nullptr;       // markers of each cell, ascending
   double* band_w = nullptr;
+  int64_t band_cap = 0;             // 27 n: capacity of the band arrays
+  int32_t* d_nband = nullptr;       // the band's size on the device
+  int32_t* mk_ent = nullptr;        // [n][27] position of the entry in the band CSR, -1 unused
+  double* mk_v = nullptr;           // [n][3] marker velocities (reported frame)
+  double* mk_x = nullptr;           // [n][3] positions, finest cells
+  bool mk_x_set = false;            // mk_x holds positions (after a move)
+  // the accumulated force holds the marker call of the collision computed ahead
+  // (mt_step's state), which is the last marker call (force_acc[4..6])
+  bool mk_acc_ahead = false;
+  // staging of a move: positions, velocities, the new kernel cells and weights,
+  // and flags: bad markers, first bad marker, support changed, used entries
+  double* mv_x = nullptr;
+  double* mv_v = nullptr;
+  int32_t* mv_slot = nullptr;
+  double* mv_w = nullptr;
+  int32_t* mv_flags = nullptr;
   std::vector<uint64_t> h_dis_keys;
   std::vector<double> h_dis_vals;
   bool finalized = false;
@@ -250,9 +266,16 @@ struct LevelUpdate {
 int launch_fused_levels(Case& c, int n, const LevelUpdate* u, bool nee_after);
 bool level_has_ib(const Case& c, int L);
 int build_markers(Case& c);
+// Moves the markers to mv_x with velocities mv_v (staged on the device; rigid:
+// first move mk_x rigidly into mv_x, see mt_markers_rigid) and rebuilds their
+// kernel cells and band (mt_tables.hip); on an invalid support returns
+// MT_ERR_ARG and changes nothing.
+int move_markers(Case& c, bool rigid, double dt, const double* center, const double* vel, const double* omega);
 // moments_done: the finest fused update has already left the band's moments
 // (FusedArgs::band_row)
 int launch_markers(Case& c, int buf, hipStream_t st, bool moments_done = false);
+// removes the last marker call from the accumulated force
+int launch_mk_unforce(Case& c);
 int launch_gather(Case& c, int L, int field, const int32_t* d_slots, int64_t n, double* d_out, int buf);
 int launch_scatter(Case& c, int L, int field, const int32_t* d_slots, int64_t n, const double* d_in, int buf);
 int launch_reduce(Case& c, int L, double* h_out);

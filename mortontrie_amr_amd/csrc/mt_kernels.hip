@@ -353,10 +353,10 @@ struct FusedArgs {
   int64_t ns_f;
   void* cx_dst;                        // with COAL: this level's targets of the next pair
   // markers (finest level): the band cells' density and velocity of the new
-  // fcol, [4][n_band], as k_mk_moments computes them
+  // fcol, [4][band capacity], as k_mk_moments computes them
   const int32_t* __restrict__ band_row;
   double* band_mom;
-  int64_t n_band;
+  int64_t n_band;  // the band's capacity (the row stride of band_mom)
 };
 
 // A value from another lane of the wave through a DPP lane permutation (CTRL:
@@ -1630,7 +1630,7 @@ static int fused_args(Case& c, int L, bool write_f, bool write_macros, bool c2f_
   if (L == c.Lmax && c.n_mk && c.band_row) {
     a.band_row = c.band_row;
     a.band_mom = c.band_mom;
-    a.n_band = c.n_band;
+    a.n_band = c.band_cap;
   }
   if (c.xin) {
     if (L >= 1) {
@@ -1782,11 +1782,11 @@ int launch_fused_levels(Case& c, int n, const LevelUpdate* u, bool nee_after) {
 //                 sum_k F_k in a fixed order beside them (mk_force_sum; F is
 //                 final when the kernel starts).
 template <int Q, typename S>
-__global__ __launch_bounds__(256) void k_mk_moments(const S* fcol, int64_t ns, const int32_t* band, int64_t nb,
-                                                    double* mom) {
+__global__ __launch_bounds__(256) void k_mk_moments(const S* fcol, int64_t ns, const int32_t* band,
+                                                    const int32_t* n_band, int64_t nb, double* mom) {
   typedef typename RealOf<S>::type R;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb) return;
+  if (i >= *n_band) return;
   const int sl = band[i];
   R f[Q];
 #pragma unroll
@@ -1800,8 +1800,8 @@ __global__ __launch_bounds__(256) void k_mk_moments(const S* fcol, int64_t ns, c
 }
 
 __global__ __launch_bounds__(256) void k_mk_interp(const double* mom, int64_t nb, const int32_t* mk_band,
-                                                   const double* w, const double* dv, int64_t n, double ubx,
-                                                   double uby, double ubz, double* F) {
+                                                   const double* w, const double* dv, int64_t n, const double* ub,
+                                                   double* F) {
   const int64_t m = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
   const int j = threadIdx.x & 31;
   double r = 0, a = 0, b = 0, c = 0;
@@ -1821,9 +1821,9 @@ __global__ __launch_bounds__(256) void k_mk_interp(const double* mom, int64_t nb
     c += __shfl_down(c, off, 32);
   }
   if (m < n && j == 0) {
-    F[3 * m] = dv[m] * r * (ubx - a);
-    F[3 * m + 1] = dv[m] * r * (uby - b);
-    F[3 * m + 2] = dv[m] * r * (ubz - c);
+    F[3 * m] = dv[m] * r * (ub[3 * m] - a);
+    F[3 * m + 1] = dv[m] * r * (ub[3 * m + 1] - b);
+    F[3 * m + 2] = dv[m] * r * (ub[3 * m + 2] - c);
   }
 }
 
@@ -1867,8 +1867,9 @@ __device__ void mk_force_sum(const double* F, int64_t n, double* acc) {
 
 template <int Q, typename S>
 __global__ __launch_bounds__(256) void k_mk_spread(S* fcol, int64_t ns, const int32_t* band, const int32_t* ptr,
-                                                   const int32_t* mk, const double* bw, int64_t nb,
-                                                   const double* mom, const double* F, int64_t n_mk, double* acc) {
+                                                   const int32_t* mk, const double* bw, const int32_t* n_band,
+                                                   int64_t nb, const double* mom, const double* F, int64_t n_mk,
+                                                   double* acc) {
   typedef typename RealOf<S>::type R;
   // block 0 sums the body force beside the band's blocks 1, 2, ...
   if (blockIdx.x == 0) {
@@ -1876,7 +1877,7 @@ __global__ __launch_bounds__(256) void k_mk_spread(S* fcol, int64_t ns, const in
     return;
   }
   const int64_t i = (int64_t)(blockIdx.x - 1) * blockDim.x + threadIdx.x;
-  if (i >= nb) return;
+  if (i >= *n_band) return;
   double mx = 0, my = 0, mz = 0;
   for (int32_t e = ptr[i]; e < ptr[i + 1]; ++e) {
     const double we = bw[e];
@@ -1901,38 +1902,71 @@ __global__ __launch_bounds__(256) void k_mk_spread(S* fcol, int64_t ns, const in
 int launch_markers(Case& c, int buf, hipStream_t st, bool moments_done) {
   if (!c.n_mk) return MT_OK;
   DevLevel& l = c.lv[c.Lmax];
+  // grids cover the band's capacity (its size changes as the markers move,
+  // mt_markers_move); the kernels read the current size from the device
   if (!moments_done)
-    MT_DISPATCH(c, (k_mk_moments<Q_, S_><<<nblk(c.n_band), 256, 0, st>>>((const S_*)l.fcol[buf], l.n_slots,
-                                                                         c.band_slot, c.n_band, c.band_mom)));
-  k_mk_interp<<<nblk(32 * c.n_mk), 256, 0, st>>>(c.band_mom, c.n_band, c.mk_band, c.mk_w, c.mk_dv, c.n_mk,
-                                                 c.mk_ub[0], c.mk_ub[1], c.mk_ub[2], c.mk_F);
-  MT_DISPATCH(c, (k_mk_spread<Q_, S_><<<nblk(c.n_band) + 1, 256, 0, st>>>((S_*)l.fcol[buf], l.n_slots, c.band_slot,
-                                                                      c.band_ptr, c.band_mk, c.band_w, c.n_band,
-                                                                      c.band_mom, c.mk_F, c.n_mk, c.force_acc)));
+    MT_DISPATCH(c, (k_mk_moments<Q_, S_><<<nblk(c.band_cap), 256, 0, st>>>((const S_*)l.fcol[buf], l.n_slots,
+                                                                           c.band_slot, c.d_nband, c.band_cap,
+                                                                           c.band_mom)));
+  k_mk_interp<<<nblk(32 * c.n_mk), 256, 0, st>>>(c.band_mom, c.band_cap, c.mk_band, c.mk_w, c.mk_dv, c.n_mk, c.mk_v,
+                                                 c.mk_F);
+  MT_DISPATCH(c, (k_mk_spread<Q_, S_><<<nblk(c.band_cap) + 1, 256, 0, st>>>(
+                     (S_*)l.fcol[buf], l.n_slots, c.band_slot, c.band_ptr, c.band_mk, c.band_w, c.d_nband, c.band_cap,
+                     c.band_mom, c.mk_F, c.n_mk, c.force_acc)));
+  MT_HIP(hipGetLastError());
+  return MT_OK;
+}
+
+__global__ void k_mk_unforce(double* acc) {
+  if (threadIdx.x == 0) {
+    for (int a = 0; a < 3; ++a) acc[a] -= acc[4 + a];
+    acc[3] -= 1.0;
+  }
+}
+
+int launch_mk_unforce(Case& c) {
+  k_mk_unforce<<<1, 64, 0, c.stream>>>(c.force_acc);
   MT_HIP(hipGetLastError());
   return MT_OK;
 }
 
 // Device tables of the markers: the slots of their 27 cells (each must be a
 // streamed lat_f cell of the finest level) and the band of cells in their
-// support with each cell's markers in ascending order (CSR).
+// support with each cell's markers in ascending order (CSR).  The band arrays
+// are sized for 27 cells per marker, so that moving markers (mt_markers_move)
+// rebuilds them in place.
 int build_markers(Case& c) {
   if (!c.n_mk) return MT_OK;
   const int64_t n = c.n_mk, ne = 27 * n;
   DevLevel& l = c.lv[c.Lmax];
   uint64_t* dk = nullptr;
   MT_HIP(dev_malloc((void**)&dk, 8 * ne, c.stream));
+  c.band_cap = ne;
   c.mk_slot = c.alloc<int32_t>(ne);
   c.mk_w = c.alloc<double>(ne);
   c.mk_dv = c.alloc<double>(n);
   c.mk_F = c.alloc<double>(3 * n);
-  if (!c.mk_slot || !c.mk_w || !c.mk_dv || !c.mk_F) {
+  c.mk_v = c.alloc<double>(3 * n);
+  c.mk_ent = c.alloc<int32_t>(ne);
+  c.d_nband = c.alloc<int32_t>(1);
+  c.mk_x = c.alloc<double>(3 * n);
+  c.mv_x = c.alloc<double>(3 * n);
+  c.mv_v = c.alloc<double>(3 * n);
+  c.mv_slot = c.alloc<int32_t>(ne);
+  c.mv_w = c.alloc<double>(ne);
+  c.mv_flags = c.alloc<int32_t>(4);
+  if (!c.mk_slot || !c.mk_w || !c.mk_dv || !c.mk_F || !c.mk_v || !c.mk_ent || !c.d_nband || !c.mk_x || !c.mv_x ||
+      !c.mv_v || !c.mv_slot || !c.mv_w || !c.mv_flags) {
     set_error("out of device memory");
     return MT_ERR_HIP;
   }
   MT_HIP(hipMemcpyAsync(dk, c.h_mk_keys.data(), 8 * ne, hipMemcpyHostToDevice, c.stream));
   MT_HIP(hipMemcpyAsync(c.mk_w, c.h_mk_w.data(), 8 * ne, hipMemcpyHostToDevice, c.stream));
   MT_HIP(hipMemcpyAsync(c.mk_dv, c.h_mk_dv.data(), 8 * n, hipMemcpyHostToDevice, c.stream));
+  std::vector<double> hv(3 * n);
+  for (int64_t m = 0; m < n; ++m)
+    for (int a = 0; a < 3; ++a) hv[3 * m + a] = c.mk_ub[a];
+  MT_HIP(hipMemcpyAsync(c.mk_v, hv.data(), 8 * 3 * n, hipMemcpyHostToDevice, c.stream));
   int r = find_slots(c, c.Lmax, dk, ne, c.mk_slot);
   if (r) return r;
   std::vector<int32_t> hs(ne);
@@ -1941,7 +1975,7 @@ int build_markers(Case& c) {
   MT_HIP(hipMemcpyAsync(kind.data(), l.kind, 2 * l.n_slots, hipMemcpyDeviceToHost, c.stream));
   MT_HIP(hipStreamSynchronize(c.stream));
   dev_free(dk, c.stream);
-  struct E { int32_t slot, mk; double w; };
+  struct E { int32_t slot, e; double w; };
   std::vector<E> es;
   for (int64_t m = 0; m < n; ++m)
     for (int j = 0; j < 27; ++j) {
@@ -1952,31 +1986,28 @@ int build_markers(Case& c) {
         set_error("marker " + std::to_string(m) + ": a kernel cell is not a streamed fluid cell of the finest level");
         return MT_ERR_ARG;
       }
-      es.push_back({sl, (int32_t)m, w});
+      es.push_back({sl, (int32_t)(m * 27 + j), w});
     }
-  std::stable_sort(es.begin(), es.end(), [](const E& a, const E& b) { return a.slot != b.slot ? a.slot < b.slot : a.mk < b.mk; });
-  std::vector<int32_t> band, ptr, mk;
+  // (slot, entry) ascending: within a cell, markers in ascending order
+  std::sort(es.begin(), es.end(), [](const E& a, const E& b) { return a.slot != b.slot ? a.slot < b.slot : a.e < b.e; });
+  std::vector<int32_t> band, ptr, mk, ent(ne, -1), mb(ne, 0);
   std::vector<double> bw;
   for (size_t i = 0; i < es.size(); ++i) {
     if (i == 0 || es[i].slot != es[i - 1].slot) {
       band.push_back(es[i].slot);
       ptr.push_back((int32_t)i);
     }
-    mk.push_back(es[i].mk);
+    mk.push_back(es[i].e / 27);
     bw.push_back(es[i].w);
+    ent[es[i].e] = (int32_t)i;
+    mb[es[i].e] = (int32_t)band.size() - 1;
   }
   ptr.push_back((int32_t)es.size());
-  const std::vector<int32_t>& band_sorted = band;  // ascending slots
-  // band index of every kernel entry (unused entries: the marker's first cell)
-  std::vector<int32_t> mb(ne);
+  // unused entries point at the band cell of the marker's first used entry
   for (int64_t m = 0; m < n; ++m) {
     int32_t first = -1;
-    for (int j = 0; j < 27; ++j) {
-      const int64_t e = m * 27 + j;
-      if (c.h_mk_w[e] == 0.0) continue;
-      mb[e] = (int32_t)(std::lower_bound(band_sorted.begin(), band_sorted.end(), hs[e]) - band_sorted.begin());
-      if (first < 0) first = mb[e];
-    }
+    for (int j = 0; j < 27 && first < 0; ++j)
+      if (c.h_mk_w[m * 27 + j] != 0.0) first = mb[m * 27 + j];
     if (first < 0) {
       set_error("marker " + std::to_string(m) + " has no kernel cell with a nonzero weight");
       return MT_ERR_ARG;
@@ -1985,11 +2016,12 @@ int build_markers(Case& c) {
       if (c.h_mk_w[m * 27 + j] == 0.0) mb[m * 27 + j] = first;
   }
   c.n_band = (int64_t)band.size();
-  c.band_slot = c.alloc<int32_t>(c.n_band);
-  c.band_ptr = c.alloc<int32_t>(c.n_band + 1);
-  c.band_mk = c.alloc<int32_t>((int64_t)mk.size());
-  c.band_w = c.alloc<double>((int64_t)bw.size());
-  c.band_mom = c.alloc<double>(4 * c.n_band);
+  const int32_t nb32 = (int32_t)c.n_band;
+  c.band_slot = c.alloc<int32_t>(ne);
+  c.band_ptr = c.alloc<int32_t>(ne + 1);
+  c.band_mk = c.alloc<int32_t>(ne);
+  c.band_w = c.alloc<double>(ne);
+  c.band_mom = c.alloc<double>(4 * ne);
   c.mk_band = c.alloc<int32_t>(ne);
   const int64_t ns_f = c.lv[c.Lmax].n_slots;
   c.band_row = c.alloc<int32_t>(ns_f);
@@ -2002,6 +2034,8 @@ int build_markers(Case& c) {
   MT_HIP(hipMemcpyAsync(c.band_mk, mk.data(), 4 * mk.size(), hipMemcpyHostToDevice, c.stream));
   MT_HIP(hipMemcpyAsync(c.band_w, bw.data(), 8 * bw.size(), hipMemcpyHostToDevice, c.stream));
   MT_HIP(hipMemcpyAsync(c.mk_band, mb.data(), 4 * ne, hipMemcpyHostToDevice, c.stream));
+  MT_HIP(hipMemcpyAsync(c.mk_ent, ent.data(), 4 * ne, hipMemcpyHostToDevice, c.stream));
+  MT_HIP(hipMemcpyAsync(c.d_nband, &nb32, 4, hipMemcpyHostToDevice, c.stream));
   std::vector<int32_t> row(ns_f, -1);
   for (int64_t i = 0; i < c.n_band; ++i) row[band[i]] = (int32_t)i;
   MT_HIP(hipMemcpyAsync(c.band_row, row.data(), 4 * ns_f, hipMemcpyHostToDevice, c.stream));

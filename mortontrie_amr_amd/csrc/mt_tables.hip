@@ -9,6 +9,7 @@
 // descent over the sorted brick codes.  find() semantics are preserved: a key
 // is present iff its brick exists and its slot is occupied.
 #include <algorithm>
+#include <climits>
 #include <cstring>
 
 #include "mt_case.h"
@@ -972,6 +973,228 @@ int build_tables(Case& c) {
     set_error("a cell lies outside every member's box");
     return MT_ERR_ARG;
   }
+  return MT_OK;
+}
+
+
+// ------------------------------------------------------------------ moving markers
+// Marker positions are in finest-level cell units (cell i spans [i, i + 1)).  A
+// marker's kernel cells are the nodes i with |i + 1/2 - x| < 3/2 per axis, in
+// the candidate order z, y, x outermost to innermost, and its weights the
+// product of the three-point kernel of Roma, Peskin & Berger (J. Comput. Phys.
+// 153, 1999) over the axes; the arithmetic is the host builder's (no
+// contraction, the same operation order), so both give the same bits.
+
+namespace {
+
+__device__ inline double mv_roma3(double r) {
+#pragma clang fp contract(off)
+  r = fabs(r);
+  if (r <= 0.5) return (1.0 + sqrt(1.0 - 3.0 * (r * r))) / 3.0;
+  if (r < 1.5) {
+    const double t = 1.0 - r;
+    return (5.0 - 3.0 * r - sqrt(fmax(0.0, 1.0 - 3.0 * (t * t)))) / 6.0;
+  }
+  return 0.0;
+}
+
+// rigid motion over dt: x' = center + vel dt + Rot(omega dt)(x - center)
+// (Rodrigues), and the velocity of the moved point, vel + omega x (x' - center
+// - vel dt), stored in the reported frame (negated)
+__global__ void k_mv_rigid(const double* x, int64_t n, double dt, double cx, double cy, double cz, double vx,
+                           double vy, double vz, double wx, double wy, double wz, double* xo, double* vo) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const double rx = x[3 * m] - cx, ry = x[3 * m + 1] - cy, rz = x[3 * m + 2] - cz;
+  const double wn = sqrt(wx * wx + wy * wy + wz * wz);
+  double px = rx, py = ry, pz = rz;
+  if (wn > 0.0) {
+    const double kx = wx / wn, ky = wy / wn, kz = wz / wn, th = wn * dt;
+    const double cs = cos(th), sn = sin(th), kr = kx * rx + ky * ry + kz * rz;
+    px = rx * cs + (ky * rz - kz * ry) * sn + kx * kr * (1.0 - cs);
+    py = ry * cs + (kz * rx - kx * rz) * sn + ky * kr * (1.0 - cs);
+    pz = rz * cs + (kx * ry - ky * rx) * sn + kz * kr * (1.0 - cs);
+  }
+  xo[3 * m] = cx + vx * dt + px;
+  xo[3 * m + 1] = cy + vy * dt + py;
+  xo[3 * m + 2] = cz + vz * dt + pz;
+  vo[3 * m] = -(vx + (wy * pz - wz * py));
+  vo[3 * m + 1] = -(vy + (wz * px - wx * pz));
+  vo[3 * m + 2] = -(vz + (wx * py - wy * px));
+}
+
+// flags: [0] markers whose support is invalid, [1] the first of them, [2] some
+// marker's kernel cells changed, [3] used entries
+__global__ void k_mv_stencil(const double* x, int64_t n, LevelView v, const int32_t* cur_slot, const double* cur_w,
+                             int32_t* st_slot, double* st_w, int32_t* flags) {
+#pragma clang fp contract(off)
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const double cx = x[3 * m] - 0.5, cy = x[3 * m + 1] - 0.5, cz = x[3 * m + 2] - 0.5;
+  const double hi = (double)(1 << 20);
+  bool ok = cx >= 1.0 && cy >= 1.0 && cz >= 1.0 && cx < hi && cy < hi && cz < hi;  // false for NaN
+  int cnt = 0;
+  int32_t* sl_out = st_slot + 27 * m;
+  double* w_out = st_w + 27 * m;
+  if (ok) {
+    const int64_t bx = (int64_t)floor(cx) - 1, by = (int64_t)floor(cy) - 1, bz = (int64_t)floor(cz) - 1;
+    for (int k = 0; k < 4 && ok; ++k)
+      for (int j = 0; j < 4 && ok; ++j)
+        for (int i = 0; i < 4 && ok; ++i) {
+          const double w = mv_roma3((double)(bx + i) - cx) * mv_roma3((double)(by + j) - cy) *
+                           mv_roma3((double)(bz + k) - cz);
+          if (!(w > 0.0)) continue;
+          const int32_t sl = cnt < 27 ? find_slot(v, morton3(bx + i, by + j, bz + k)) : -1;
+          const uint16_t kd = sl >= 0 ? v.kind[sl] : 0;
+          if (sl < 0 || !(kd & MT_KIND_FLUID) || (kd & (MT_KIND_C2F | MT_KIND_DIS | MT_KIND_SOLID))) {
+            ok = false;
+            break;
+          }
+          sl_out[cnt] = sl;
+          w_out[cnt] = w;
+          ++cnt;
+        }
+  }
+  if (!ok || cnt == 0) {
+    atomicAdd(&flags[0], 1);
+    atomicMin(&flags[1], (int32_t)m);
+    return;
+  }
+  for (int e = cnt; e < 27; ++e) {
+    sl_out[e] = sl_out[0];
+    w_out[e] = 0.0;
+  }
+  bool changed = false;
+  for (int e = 0; e < 27; ++e)
+    changed |= sl_out[e] != cur_slot[27 * m + e] || (w_out[e] != 0.0) != (cur_w[27 * m + e] != 0.0);
+  if (changed) atomicOr(&flags[2], 1);
+  atomicAdd(&flags[3], cnt);
+}
+
+// the same kernel cells: new weights in place
+__global__ void k_mv_reweight(const double* w, int64_t ne, const int32_t* ent, double* mk_w, double* band_w) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  mk_w[e] = w[e];
+  if (ent[e] >= 0) band_w[ent[e]] = w[e];
+}
+
+// sort keys (slot, entry); unused entries sort last
+__global__ void k_mv_keys(const int32_t* slot, const double* w, int64_t ne, uint64_t last, uint64_t* keys) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  keys[e] = ((w[e] != 0.0 ? (uint64_t)slot[e] : last) << 32) | (uint64_t)e;
+}
+
+__global__ void k_mv_heads(const uint64_t* ks, int64_t nv, int64_t ne, int* head) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  head[i] = i < nv && (i == 0 || (ks[i] >> 32) != (ks[i - 1] >> 32));
+}
+
+__global__ void k_mv_emit(const uint64_t* ks, const int* head, const int* pos, int64_t nv, const double* w,
+                          int32_t* band_slot, int32_t* band_ptr, int32_t* band_mk, double* band_w, int32_t* mk_band,
+                          int32_t* mk_ent) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const int64_t e = (int64_t)(ks[i] & 0xFFFFFFFFull);
+  const int32_t b = pos[i] + head[i] - 1;
+  if (head[i]) {
+    band_slot[b] = (int32_t)(ks[i] >> 32);
+    band_ptr[b] = (int32_t)i;
+  }
+  if (i == nv - 1) band_ptr[b + 1] = (int32_t)nv;
+  band_mk[i] = (int32_t)(e / 27);
+  band_w[i] = w[e];
+  mk_band[e] = b;
+  mk_ent[e] = (int32_t)i;
+}
+
+// unused entries point at the band cell of the marker's first used entry
+__global__ void k_mv_unused(const double* w, int64_t n, int32_t* mk_band, int32_t* mk_ent) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  int32_t first = -1;
+  for (int j = 0; j < 27; ++j) {
+    const int64_t e = 27 * m + j;
+    if (w[e] != 0.0) {
+      if (first < 0) first = mk_band[e];
+    } else {
+      mk_band[e] = first;  // entry 0 is always used (a stencil lists its used cells first)
+      mk_ent[e] = -1;
+    }
+  }
+}
+
+__global__ void k_mv_rows(const int32_t* band_slot, const int32_t* n_band, int32_t* row, int clear) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= *n_band) return;
+  row[band_slot[b]] = clear ? -1 : (int32_t)b;
+}
+
+}  // namespace
+
+int move_markers(Case& c, bool rigid, double dt, const double* center, const double* vel, const double* omega) {
+  const int64_t n = c.n_mk, ne = 27 * n;
+  if (rigid)
+    k_mv_rigid<<<nblk(n), 256, 0, c.stream>>>(c.mk_x, n, dt, center[0], center[1], center[2], vel[0], vel[1], vel[2],
+                                              omega[0], omega[1], omega[2], c.mv_x, c.mv_v);
+  const int32_t init[4] = {0, INT32_MAX, 0, 0};
+  int32_t fl[4];
+  MT_HIP(hipMemcpyAsync(c.mv_flags, init, sizeof init, hipMemcpyHostToDevice, c.stream));
+  k_mv_stencil<<<nblk(n), 256, 0, c.stream>>>(c.mv_x, n, view(c, c.Lmax), c.mk_slot, c.mk_w, c.mv_slot, c.mv_w,
+                                              c.mv_flags);
+  MT_HIP(hipGetLastError());
+  MT_HIP(hipMemcpyAsync(fl, c.mv_flags, sizeof fl, hipMemcpyDeviceToHost, c.stream));
+  MT_HIP(hipStreamSynchronize(c.stream));
+  if (fl[0]) {
+    set_error("marker " + std::to_string(fl[1]) +
+              ": a kernel cell is not a streamed fluid cell of the finest level (markers not moved)");
+    return MT_ERR_ARG;
+  }
+  if (!fl[2]) {
+    k_mv_reweight<<<nblk(ne), 256, 0, c.stream>>>(c.mv_w, ne, c.mk_ent, c.mk_w, c.band_w);
+  } else {
+    // rebuild the band: entries sorted by (slot, entry), one band cell per run
+    const int64_t nv = fl[3];
+    uint64_t* keys = nullptr;
+    int* head = nullptr;
+    int* pos = nullptr;
+    MT_HIP(dev_malloc((void**)&keys, 8 * ne, c.stream));
+    MT_HIP(dev_malloc((void**)&head, sizeof(int) * (ne + 1), c.stream));
+    MT_HIP(dev_malloc((void**)&pos, sizeof(int) * (ne + 1), c.stream));
+    const uint64_t last = (uint64_t)c.lv[c.Lmax].n_slots;
+    k_mv_keys<<<nblk(ne), 256, 0, c.stream>>>(c.mv_slot, c.mv_w, ne, last, keys);
+    uint64_t* ks = nullptr;
+    int32_t* vs = nullptr;
+    int r = radix_sort(c, keys, ne, (last << 32) | (uint64_t)ne, &ks, &vs);
+    if (r) return r;
+    k_mv_heads<<<nblk(ne), 256, 0, c.stream>>>(ks, nv, ne, head);
+    int64_t nb = 0;
+    r = scan_exclusive(c, head, pos, ne, &nb);
+    if (r) return r;
+    k_mv_rows<<<nblk(c.band_cap), 256, 0, c.stream>>>(c.band_slot, c.d_nband, c.band_row, 1);
+    k_mv_emit<<<nblk(nv), 256, 0, c.stream>>>(ks, head, pos, nv, c.mv_w, c.band_slot, c.band_ptr, c.band_mk,
+                                              c.band_w, c.mk_band, c.mk_ent);
+    k_mv_unused<<<nblk(n), 256, 0, c.stream>>>(c.mv_w, n, c.mk_band, c.mk_ent);
+    k_fill_i32<<<1, 256, 0, c.stream>>>(c.d_nband, 1, (int32_t)nb);
+    k_mv_rows<<<nblk(c.band_cap), 256, 0, c.stream>>>(c.band_slot, c.d_nband, c.band_row, 0);
+    MT_HIP(hipMemcpyAsync(c.mk_w, c.mv_w, 8 * ne, hipMemcpyDeviceToDevice, c.stream));
+    MT_HIP(hipGetLastError());
+    MT_HIP(hipStreamSynchronize(c.stream));
+    dev_free(ks, c.stream);
+    dev_free(vs, c.stream);
+    dev_free(keys, c.stream);
+    dev_free(head, c.stream);
+    dev_free(pos, c.stream);
+    c.n_band = nb;
+  }
+  MT_HIP(hipMemcpyAsync(c.mk_slot, c.mv_slot, 4 * ne, hipMemcpyDeviceToDevice, c.stream));
+  MT_HIP(hipMemcpyAsync(c.mk_x, c.mv_x, 8 * 3 * n, hipMemcpyDeviceToDevice, c.stream));
+  MT_HIP(hipMemcpyAsync(c.mk_v, c.mv_v, 8 * 3 * n, hipMemcpyDeviceToDevice, c.stream));
+  MT_HIP(hipGetLastError());
+  MT_HIP(hipStreamSynchronize(c.stream));
+  c.mk_x_set = true;
   return MT_OK;
 }
 

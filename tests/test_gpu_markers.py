@@ -4,7 +4,9 @@ Bar: FP64 fields within 1e-12 relative L2 after the stated number of steps; the
 body force sums within 1e-12 relative.  The device's step runs one collision
 ahead (the next step's first collision, with its marker forcing, is done by the
 last fused update), so after n device steps the force log holds one more
-finest-level call than the oracle's after n steps."""
+finest-level call than the oracle's after n steps, until a call that returns
+the case to the reference's buffers (reading fcol with explosions inside the
+updates, writing a field, moving markers) drops that collision with its force."""
 import numpy as np
 import pytest
 
@@ -32,7 +34,7 @@ def small_marker_sphere(**kw):
 def check_force(g, o, case, steps):
     calls_per_step = 2 ** case["Lmax"]  # finest-level collisions per coarse step
     s, n = g.ib_force()
-    assert n == steps * calls_per_step + 1
+    assert n in (steps * calls_per_step, steps * calls_per_step + 1)
     o.step(1)
     want = o.force_log()[:n].sum(0)
     assert rel(s, want) < 1e-12
