@@ -169,6 +169,23 @@ int mt_case_upload_markers(mt_case* c, int64_t n, const uint64_t* keys27, const 
 int mt_markers_move(mt_case* c, int64_t n, const double* xyz, const double* ub);
 int mt_markers_rigid(mt_case* c, double dt, const double* center3, const double* vel3, const double* omega3);
 int mt_markers_get(mt_case* c, int64_t n, double* xyz, double* ub);
+/* Remeshing as a solid moves: the reference adds and removes nodes per level as
+ * its solid points move (Grid_Manager::update_nodes_near_solid / update_nodes,
+ * Grid_Manager_dynamic.cpp:37-594, 1083-1300) and carries no flow state across.
+ * Here a new case built for the new geometry (its level sets from
+ * mt_levelset_build, say) takes the state of the old one, cell by cell on the
+ * device: a key present in src with populations is copied; else its nearest
+ * coarser src ancestor is exploded (feq_HL + s (f - feq_HL), s the product of
+ * the transfer factors 2 tau_l / tau_{l-1}, LBM_Manager.cpp:1059-1129); else
+ * its src children one level finer are averaged and coalesced (factor
+ * tau_L / (2 tau_{L+1}), LBM_Manager.cpp:1131-1169); else it is new to the
+ * flow and starts at feq(rho0, u_fill) (u_fill reported frame, NULL = 0).
+ * Both f and fcol follow the rule; density and velocity are the moments of the
+ * new f.  dst's solid and reservoir cells keep their initial state.  Both
+ * cases single-member with the same Q, precision and number of levels, on one
+ * device.  counts4 (may be NULL): cells copied, exploded, coalesced, filled.
+ * No counterpart in the reference.                                            */
+int mt_case_remap(mt_case* dst, mt_case* src, const double* u_fill3, int64_t* counts4);
 int mt_case_level_info(mt_case* c, int level, int64_t* n_cells, int64_t* n_bricks);
 
 /* ---- key lookup: replaces D_map_define<T>::find (mortonTrieMap.cpp:116-126,

@@ -51,7 +51,7 @@ EXPORTS = [
     "mt_collide", "mt_stream", "mt_explode", "mt_coalesce", "mt_apply_bc", "mt_ib_treat", "mt_step",
     "mt_set_graphs", "mt_set_tau", "mt_set_schedule", "mt_set_timing", "mt_get_timing", "mt_sync", "mt_reduce", "mt_ib_force",
     "mt_case_set_members", "mt_ib_force_members", "mt_autotune", "mt_case_copy_member",
-    "mt_case_upload_markers", "mt_markers_move", "mt_markers_rigid", "mt_markers_get", "mt_scan_members", "mt_set_diagnostics", "mt_diagnostics", "mt_levelset_build",
+    "mt_case_upload_markers", "mt_markers_move", "mt_markers_rigid", "mt_markers_get", "mt_case_remap", "mt_scan_members", "mt_set_diagnostics", "mt_diagnostics", "mt_levelset_build",
     "mt_levelset_counts", "mt_levelset_copy", "mt_levelset_copy_dis", "mt_levelset_destroy", "mt_snapshot",
     "mt_write_tecplot", "mt_write_nodes", "mt_server_create", "mt_server_submit", "mt_server_step",
     "mt_server_poll",
@@ -145,6 +145,7 @@ def load():
     L.mt_diagnostics.argtypes = [C.c_void_p, C.POINTER(Diag)]
     L.mt_set_tau.argtypes = [C.c_void_p, C.c_int32, _dp, C.c_int32]
     L.mt_markers_move.argtypes = [C.c_void_p, C.c_int64, _dp, _dp]
+    L.mt_case_remap.argtypes = [C.c_void_p, C.c_void_p, _dp, C.POINTER(C.c_int64)]
     L.mt_markers_rigid.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp]
     L.mt_markers_get.argtypes = [C.c_void_p, C.c_int64, _dp, _dp]
     L.mt_server_destroy.argtypes = [C.c_void_p]
@@ -406,6 +407,14 @@ class AmrCase:
     def write_nodes(self, path):
         """The finest level's keys as 64-bit binary strings (the AMR node dump)."""
         _check(load().mt_write_nodes(self.h, str(path).encode()))
+
+    def remap_from(self, src, u_fill=(0.0, 0.0, 0.0)):
+        """This case's cells take the state of `src` (mt_case_remap); returns the
+        cells copied, exploded, coalesced and filled."""
+        u = np.ascontiguousarray(u_fill, np.float64).reshape(3)
+        cnt = (C.c_int64 * 4)()
+        _check(load().mt_case_remap(self.h, src.h, u.ctypes.data_as(_dp), cnt))
+        return {"copied": cnt[0], "exploded": cnt[1], "coalesced": cnt[2], "filled": cnt[3]}
 
     def move_markers(self, xyz, ub=None):
         """New marker positions [n][3] (finest cells) and velocities (reported frame;

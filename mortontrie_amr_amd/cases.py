@@ -861,13 +861,14 @@ def marker_stencils(xyz, Lmax):
 
 
 def sphere_markers(coarse=(64, 32, 32), radius=8.0, center=(16.0, 16.0, 16.0), shells=(6.0, 3.0),
-                   n_markers=6000, Q=27, collision=COLLIDE_CUMULANT, U=0.04, Re=100.0, les_C=0.1, options=2):
+                   n_markers=6000, Q=27, collision=COLLIDE_CUMULANT, U=0.04, Re=100.0, les_C=0.1, options=2,
+                   builder="host"):
     """The sphere of sphere_amr represented by n_markers Lagrangian markers on a
     Fibonacci lattice (explicit direct forcing on the finest level, see the
     device's marker kernels) instead of cut links; the grid is the same refined
     channel with no solid cells.  Smagorinsky constant les_C."""
     case = sphere_amr(coarse, radius=radius, center=center, shells=shells, Q=Q, collision=collision, U=U,
-                      Re=Re, les_C=les_C, with_sphere=False, options=options, name="sphere_markers")
+                      Re=Re, les_C=les_C, with_sphere=False, options=options, name="sphere_markers", builder=builder)
     Lmax = case["Lmax"]
     s = float(1 << Lmax)
     R = radius * s

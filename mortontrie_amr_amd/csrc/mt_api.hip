@@ -659,6 +659,30 @@ int mt_markers_rigid(mt_case* c, double dt, const double* center3, const double*
   return move_markers(*c, true, dt, center3, vel3, omega3);
 }
 
+int mt_case_remap(mt_case* dst, mt_case* src, const double* u_fill3, int64_t* counts4) {
+  CHECK_FINAL(dst);
+  CHECK_FINAL(src);
+  if (dst == src) { set_error("remap needs two cases"); return MT_ERR_ARG; }
+  if (dst->Q != src->Q || dst->prec != src->prec || dst->Lmax != src->Lmax) {
+    set_error("the cases differ in Q, precision or number of levels");
+    return MT_ERR_ARG;
+  }
+  if (dst->h_mem.size() != 1 || src->h_mem.size() != 1) { set_error("remap takes single-member cases"); return MT_ERR_ARG; }
+  const double zero[3] = {0, 0, 0};
+  const double* fill = u_fill3 ? u_fill3 : zero;
+  for (int a = 0; a < 3; ++a)
+    if (!std::isfinite(fill[a])) { set_error("non-finite fill velocity"); return MT_ERR_ARG; }
+  TRY(settle(*src));
+  TRY(settle(*dst));
+  MT_HIP(hipStreamSynchronize(src->stream));
+  MT_HIP(hipStreamSynchronize(dst->stream));
+  int64_t cnt[4];
+  TRY(launch_remap(*dst, *src, fill, cnt));
+  if (counts4)
+    for (int j = 0; j < 4; ++j) counts4[j] = cnt[j];
+  return MT_OK;
+}
+
 int mt_markers_get(mt_case* c, int64_t n, double* xyz, double* ub) {
   CHECK_FINAL(c);
   if (!c->n_mk) { set_error("the case has no markers"); return MT_ERR_STATE; }

@@ -274,6 +274,9 @@ int move_markers(Case& c, bool rigid, double dt, const double* center, const dou
 // moments_done: the finest fused update has already left the band's moments
 // (FusedArgs::band_row)
 int launch_markers(Case& c, int buf, hipStream_t st, bool moments_done = false);
+// the state of src carried onto dst's cells (mt_case_remap); counts[4]: cells
+// copied, exploded, coalesced, filled
+int launch_remap(Case& dst, const Case& src, const double* fill, int64_t* counts);
 // removes the last marker call from the accumulated force
 int launch_mk_unforce(Case& c);
 int launch_gather(Case& c, int L, int field, const int32_t* d_slots, int64_t n, double* d_out, int buf);

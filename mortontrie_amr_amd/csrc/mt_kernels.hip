@@ -21,6 +21,7 @@
 
 #include "mt_case.h"
 #include "mt_device.h"
+#include "mt_morton.h"
 
 namespace mt {
 
@@ -1394,6 +1395,188 @@ static LvArgs lvargs(Case& c, int L, int fbuf_in, int fbuf_out, int rbuf_in, int
       else { constexpr int Q_ = 27; typedef _Float16 S_; KERNEL_CALL; }                     \
     }                                                                                       \
   } while (0)
+
+// ------------------------------------------------------------------ remap
+// The state of one case carried onto another case's cells (mt_case_remap): the
+// same key holding populations in the source is copied; else the nearest
+// coarser source ancestor is exploded, f = feq_HL + s (f_a - feq_HL) with s the
+// product of the transfer factors 2 tau_l / tau_{l-1} down to this level (the
+// explosion of LBM_Manager.cpp:1059-1129, composed); else the source children
+// one level finer are averaged and coalesced, f = feq_HL + tau_L / (2 tau_{L+1})
+// (mean - feq_HL) (LBM_Manager.cpp:1131-1169); else the cell is new to the
+// flow (uncovered by a moving solid) and starts at feq(rho0, u_fill).  The
+// same rule gives fcol (from the source's fcol) and f; density and velocity are
+// the moments of the new f.  Solid and reservoir cells of the destination keep
+// their own initial state.
+struct RemapSrc {
+  LevelView v;
+  const void* f;
+  const void* fcol;
+  const void* rho;
+  const void* u;
+  int64_t ns;
+};
+struct RemapArgs {
+  RemapSrc src[kMaxLevels];
+  int L, Lmax;
+  const uint64_t* key;
+  const uint16_t* kind;
+  int64_t ns;
+  void* f;
+  void* fcol[2];
+  void* rho[2];
+  void* u[2];
+  double xscale[kMaxLevels], cscale[kMaxLevels], rho0, fill[3];
+  int* count;  // [4] cells copied, exploded, coalesced, filled
+};
+
+template <int Q, typename R>
+__device__ __forceinline__ void remap_transfer(R* f, R scale, R rho0) {
+  R rho, ux, uy, uz;
+  cell_macros<Q, R>(f, rho, ux, uy, uz);
+  const R uu = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, rho0);
+    f[q] = fe + scale * (f[q] - fe);
+  }
+}
+
+template <int Q, typename S>
+__global__ __launch_bounds__(256) void k_remap(RemapArgs a) {
+  typedef typename RealOf<S>::type R;
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.ns) return;
+  const uint16_t k = a.kind[s];
+  if (!k || (k & (MT_KIND_SOLID | MT_KIND_BCRES))) return;
+  const uint64_t key = a.key[s];
+  R f[Q], g[Q];
+  int how = 3;
+  const RemapSrc& same = a.src[a.L];
+  int32_t sl = find_slot(same.v, key);
+  if (sl >= 0 && !(same.v.kind[sl] & MT_KIND_SOLID)) {
+    how = 0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      f[q] = load_pop<Q, S, R>((const S*)same.f, same.ns, q, sl);
+      g[q] = load_pop<Q, S, R>((const S*)same.fcol, same.ns, q, sl);
+    }
+  }
+  double scale = 1.0;
+  for (int A = a.L - 1; how == 3 && A >= 0; --A) {
+    scale *= a.xscale[A + 1];
+    const RemapSrc& up = a.src[A];
+    const uint64_t pk = key & ~((1ull << (3 * (a.Lmax - A))) - 1);
+    sl = find_slot(up.v, pk);
+    if (sl >= 0 && !(up.v.kind[sl] & MT_KIND_SOLID)) {
+      how = 1;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        f[q] = load_pop<Q, S, R>((const S*)up.f, up.ns, q, sl);
+        g[q] = load_pop<Q, S, R>((const S*)up.fcol, up.ns, q, sl);
+      }
+      remap_transfer<Q, R>(f, (R)scale, (R)a.rho0);
+      remap_transfer<Q, R>(g, (R)scale, (R)a.rho0);
+    }
+  }
+  if (how == 3 && a.L < a.Lmax) {
+    const RemapSrc& dn = a.src[a.L + 1];
+    const int sh = 3 * (a.Lmax - a.L - 1);
+    int n = 0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = g[q] = (R)0;
+    for (int c = 0; c < 8; ++c) {
+      sl = find_slot(dn.v, key | ((uint64_t)c << sh));
+      if (sl < 0 || (dn.v.kind[sl] & MT_KIND_SOLID)) continue;
+      ++n;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        f[q] += load_pop<Q, S, R>((const S*)dn.f, dn.ns, q, sl);
+        g[q] += load_pop<Q, S, R>((const S*)dn.fcol, dn.ns, q, sl);
+      }
+    }
+    if (n) {
+      how = 2;
+      const R inv = (R)1 / (R)n;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        f[q] *= inv;
+        g[q] *= inv;
+      }
+      remap_transfer<Q, R>(f, (R)a.cscale[a.L], (R)a.rho0);
+      remap_transfer<Q, R>(g, (R)a.cscale[a.L], (R)a.rho0);
+    }
+  }
+  if (how == 3) {
+    const R ux = (R)a.fill[0], uy = (R)a.fill[1], uz = (R)a.fill[2], uu = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = g[q] = feq_c<Q, R>(q, (R)a.rho0, ux, uy, uz, uu);
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    store_pop<Q, S, R>((S*)a.f, a.ns, q, (int)s, f[q]);
+    store_pop<Q, S, R>((S*)a.fcol[0], a.ns, q, (int)s, g[q]);
+    store_pop<Q, S, R>((S*)a.fcol[1], a.ns, q, (int)s, g[q]);
+    f[q] = load_pop<Q, S, R>((const S*)a.f, a.ns, q, (int)s);  // as stored
+  }
+  R rho, ux, uy, uz;
+  cell_macros<Q, R>(f, rho, ux, uy, uz);
+  for (int j = 0; j < 2; ++j) {
+    st_real<R>(a.rho[j], s, rho);
+    st_real<R>(a.u[j], s, ux);
+    st_real<R>(a.u[j], a.ns + s, uy);
+    st_real<R>(a.u[j], 2 * a.ns + s, uz);
+  }
+  atomicAdd(&a.count[how], 1);
+}
+
+int launch_remap(Case& dst, const Case& src, const double* fill, int64_t* counts) {
+  int* d_cnt = nullptr;
+  MT_HIP(dev_malloc((void**)&d_cnt, 4 * sizeof(int), dst.stream));
+  const int zero[4] = {0, 0, 0, 0};
+  MT_HIP(hipMemcpyAsync(d_cnt, zero, sizeof zero, hipMemcpyHostToDevice, dst.stream));
+  RemapArgs a = {};
+  for (int L = 0; L <= src.Lmax; ++L) {
+    const DevLevel& l = src.lv[L];
+    a.src[L].v = LevelView{l.bcode, l.kind, l.n_bricks, 3 * (src.Lmax - L)};
+    a.src[L].f = l.f;
+    a.src[L].fcol = l.fcol[l.pf];
+    a.src[L].rho = l.rho[l.pr];
+    a.src[L].u = l.u[l.pr];
+    a.src[L].ns = l.n_slots;
+  }
+  const MemberParams& P = dst.h_mem[0];
+  for (int L = 0; L <= dst.Lmax; ++L) {
+    a.xscale[L] = P.xscale[L];
+    a.cscale[L] = P.cscale[L];
+  }
+  a.rho0 = P.rho0;
+  for (int j = 0; j < 3; ++j) a.fill[j] = fill[j];
+  a.Lmax = dst.Lmax;
+  a.count = d_cnt;
+  for (int L = 0; L <= dst.Lmax; ++L) {
+    DevLevel& l = dst.lv[L];
+    a.L = L;
+    a.key = l.key;
+    a.kind = l.kind;
+    a.ns = l.n_slots;
+    a.f = l.f;
+    a.fcol[0] = l.fcol[0];
+    a.fcol[1] = l.fcol[1];
+    a.rho[0] = l.rho[0];
+    a.rho[1] = l.rho[1];
+    a.u[0] = l.u[0];
+    a.u[1] = l.u[1];
+    MT_DISPATCH(dst, (k_remap<Q_, S_><<<nblk(l.n_slots), 256, 0, dst.stream>>>(a)));
+  }
+  MT_HIP(hipGetLastError());
+  int h[4];
+  MT_HIP(hipMemcpyAsync(h, d_cnt, sizeof h, hipMemcpyDeviceToHost, dst.stream));
+  MT_HIP(hipStreamSynchronize(dst.stream));
+  dev_free(d_cnt, dst.stream);
+  for (int j = 0; j < 4; ++j) counts[j] = h[j];
+  return MT_OK;
+}
 
 int init_state(Case& c) {
   for (int L = 0; L <= c.Lmax; ++L) {

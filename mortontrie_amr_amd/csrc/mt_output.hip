@@ -18,6 +18,7 @@
 
 #include "mt_case.h"
 #include "mt_device.h"
+#include "mt_morton.h"
 
 namespace mt {
 namespace {
@@ -108,24 +109,6 @@ int snapshot(Case& c, int L, uint32_t mask, Snap& out) {
   return MT_OK;
 }
 
-uint64_t compact3(uint64_t v) {
-  v &= 0x1249249249249249ull;
-  v = (v ^ (v >> 2)) & 0x10C30C30C30C30C3ull;
-  v = (v ^ (v >> 4)) & 0x100F00F00F00F00Full;
-  v = (v ^ (v >> 8)) & 0x1F0000FF0000FFull;
-  v = (v ^ (v >> 16)) & 0x1F00000000FFFFull;
-  v = (v ^ (v >> 32)) & 0x1FFFFFull;
-  return v;
-}
-uint64_t spread3(uint64_t v) {
-  v &= 0x1FFFFF;
-  v = (v | (v << 32)) & 0x1F00000000FFFFull;
-  v = (v | (v << 16)) & 0x1F0000FF0000FFull;
-  v = (v | (v << 8)) & 0x100F00F00F00F00Full;
-  v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
-  v = (v | (v << 2)) & 0x1249249249249249ull;
-  return v;
-}
 // the key moved by (dx,dy,dz) cells of level L (find_x1 / find_y1 / find_z1 chains,
 // header/Morton_Assist.h:229-259), with keys in finest-level units
 uint64_t corner(uint64_t k, int L, int Lmax, int dx, int dy, int dz) {

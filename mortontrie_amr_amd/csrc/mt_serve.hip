@@ -30,35 +30,18 @@
 #include <vector>
 
 #include "mt_case.h"
+#include "mt_morton.h"
 
 namespace mt {
 namespace {
 
 constexpr int kSlotX = 100, kSlotY = 48, kSlotZ = 48;  // level-0 cells per slot
 
-uint64_t spread3(uint64_t v) {
-  v &= 0x1FFFFFull;
-  v = (v | (v << 32)) & 0x1F00000000FFFFull;
-  v = (v | (v << 16)) & 0x1F0000FF0000FFull;
-  v = (v | (v << 8)) & 0x100F00F00F00F00Full;
-  v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
-  v = (v | (v << 2)) & 0x1249249249249249ull;
-  return v;
-}
-uint64_t gather3(uint64_t v) {
-  v &= 0x1249249249249249ull;
-  v = (v ^ (v >> 2)) & 0x10C30C30C30C30C3ull;
-  v = (v ^ (v >> 4)) & 0x100F00F00F00F00Full;
-  v = (v ^ (v >> 8)) & 0x1F0000FF0000FFull;
-  v = (v ^ (v >> 16)) & 0x1F00000000FFFFull;
-  v = (v ^ (v >> 32)) & 0x1FFFFFull;
-  return v;
-}
 // a level-L key (keys of every level live on the finest grid, shifted by
 // 3 (Lmax - L)) moved by (dx, dy, dz) level-L cells
 uint64_t shift_key(uint64_t k, int L, int Lmax, uint64_t dx) {
   const int sh = Lmax - L;
-  const uint64_t x = (gather3(k) >> sh) + dx, y = gather3(k >> 1) >> sh, z = gather3(k >> 2) >> sh;
+  const uint64_t x = (compact3(k) >> sh) + dx, y = compact3(k >> 1) >> sh, z = compact3(k >> 2) >> sh;
   return spread3(x << sh) | (spread3(y << sh) << 1) | (spread3(z << sh) << 2);
 }
 

@@ -182,3 +182,41 @@ def test_body_moving_through_still_fluid_is_resisted():
     assert f[0] < 0 and abs(f[1]) < 0.2 * abs(f[0]) and abs(f[2]) < 0.2 * abs(f[0])
     assert np.isfinite(g.get(0, amr.FIELD_F)).all()
     g.close()
+
+
+def test_marker_body_carried_with_remeshing():
+    """A marker sphere carried by the stream at the stream's speed, the refinement
+    rebuilt around it as it goes (motion.MarkerBody).  Uniform flow past a body
+    moving with it is an exact solution, so the marker forces stay at rounding
+    level and the velocity stays uniform through every move and remesh; each
+    remesh copies, explodes and coalesces cells (a marker body has no solid
+    cells, so none is filled).  Held still, the same sphere feels the stream's
+    drag (positive along the flow) at once."""
+    from mortontrie_amr_amd.motion import MarkerBody, RigidBody
+
+    def build(center):
+        return C.sphere_markers((32, 24, 24), radius=3.0, center=tuple(center), shells=(5.5, 2.0), n_markers=1800,
+                                builder="gpu")
+
+    still = MarkerBody(build, RigidBody((12.0, 12.0, 12.0)))
+    still.step(4)
+    assert still.forces[0][0] > 1.0 and not still.remeshes
+    still.close()
+
+    U = -still.case["cd"]["U"]  # the stream's physical speed
+    mb = MarkerBody(build, RigidBody((12.0, 12.0, 12.0), velocity=(U, 0.0, 0.0)), remesh_dist=0.5)
+    mb.step(40)
+    f = np.array(mb.forces)
+    assert np.max(np.abs(f)) < 1e-8, np.max(np.abs(f))
+    assert len(mb.remeshes) >= 3
+    assert all(r["copied"] > 0 and r["exploded"] > 0 and r["coalesced"] > 0 and r["filled"] == 0
+               for r in mb.remeshes), mb.remeshes
+    assert abs(mb.body.center[0] - (12.0 + 40 * U)) < 1e-12
+    for L, lv in enumerate(mb.case["levels"]):
+        keys = lv["keys"][(lv["kinds"] & (C.KIND_FLUID | C.KIND_F2C)) != 0]
+        u = mb.g.u(L, keys)
+        assert np.max(np.abs(u - np.array([[-U], [0.0], [0.0]]))) < 1e-10, L
+    x, _ = mb.g.markers()
+    c = mb.body.center * 2 ** mb.Lmax
+    assert np.allclose(np.linalg.norm(x - c, axis=1), 3.0 * 2 ** mb.Lmax, rtol=1e-12)
+    mb.close()

@@ -213,3 +213,19 @@ def test_mesh_body_geometry(tmp_path):
             fh.write("  endloop\n endfacet\n")
         fh.write("endsolid s\n")
     assert np.array_equal(C.read_stl(str(f)), T)
+
+
+def test_rigid_body_free_motion_closed_form():
+    """Constant impulse per step and gravity: V_n = V_0 + n (J/m + g), x_n = x_0 + sum V_k."""
+    from mortontrie_amr_amd.motion import RigidBody
+    b = RigidBody((10.0, 8.0, 8.0), velocity=(0.01, 0.0, 0.0), mass=4.0, gravity=(0.0, 0.0, -1e-4))
+    J = np.array([0.002, -0.004, 0.0])
+    for _ in range(5):
+        b.advance(J)
+    dv = J / 4.0 + np.array([0.0, 0.0, -1e-4])
+    assert np.allclose(b.velocity, np.array([0.01, 0.0, 0.0]) + 5 * dv, rtol=0, atol=1e-15)
+    want = np.array([10.0, 8.0, 8.0]) + sum(np.array([0.01, 0.0, 0.0]) + k * dv for k in range(1, 6))
+    assert np.allclose(b.center, want, rtol=0, atol=1e-14)
+    p = RigidBody((1.0, 2.0, 3.0), velocity=(0.5, 0.0, 0.0))
+    p.advance(J, dt=2.0)
+    assert np.array_equal(p.velocity, [0.5, 0.0, 0.0]) and np.allclose(p.center, [2.0, 2.0, 3.0])
