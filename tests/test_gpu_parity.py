@@ -440,10 +440,17 @@ def test_missing_key_raises_lookup_error():
 def test_full_sphere_properties():
     """Full-size 3-level sphere (explosion in both sub-steps, the benchmark case):
     the merged schedule and the per-level streams, in different step splits,
-    give the same bits (fields and force sums); finite, mass bounded."""
+    give the same bits (fields and force sums); finite; each level's mass (lat_f
+    and F2C) within 50% of its start after 60 steps.  Mass is not conserved: the
+    reference's link treatment pulls the populations whose source lies in the
+    sphere from the solid cells, which hold zeros (a sink at the wall, the same
+    bits in the oracle and in the reference's compiled code), and the density
+    clamp of the treated cells bounds it; measured -11%, -24%, -39% on levels
+    0, 1, 2."""
     case = C.sphere_amr(options=2)
     a, b = amr.AmrCase(case), amr.AmrCase(case)
     b.set_schedule("streams")
+    m0 = [a.reduce(L)[0] for L in range(3)]
     a.step(60)
     b.step(25)
     b.step(35)
@@ -451,23 +458,29 @@ def test_full_sphere_properties():
         assert np.array_equal(a.f(L), b.f(L))
         assert np.array_equal(a.fcol(L), b.fcol(L))
     assert np.array_equal(a.ib_force()[0], b.ib_force()[0])
-    m, p, nan = a.reduce(2)
-    assert nan == 0 and np.isfinite(m)
+    for L in range(3):
+        m, p, nan = a.reduce(L)
+        print("mass change level", L, m / m0[L] - 1.0, flush=True)
+        assert nan == 0 and abs(m / m0[L] - 1.0) < 0.5
     cd = a.drag_coefficient()
     assert np.isfinite(cd)
 
 
 @pytest.mark.parametrize("prec,tol", [(amr.FP32_SHIFTED, 2e-5), (amr.FP16_STORAGE, 5e-3)])
 def test_reduced_precision_error(prec, tol):
-    """FP32-shifted and FP16-storage variants against FP64: relative L2 of rho and u."""
+    """FP32-shifted and FP16-storage variants against FP64: relative L2 of rho and
+    of u below the same bound (measured: FP32 rho 2.6e-7, u 2.1e-6; FP16 rho
+    7.0e-5, u 4.2e-4 at most over the levels)."""
     case = small_sphere(options=2)
     a, b = amr.AmrCase(case), amr.AmrCase(case, precision=prec)
     a.step(20)
     b.step(20)
     for L in range(case["Lmax"] + 1):
         k = fluid_keys(case, L)
-        assert rel(b.rho(L, k), a.rho(L, k)) < tol
-        assert rel(b.u(L, k), a.u(L, k)) < 50 * tol
+        er, eu = rel(b.rho(L, k), a.rho(L, k)), rel(b.u(L, k), a.u(L, k))
+        print("precision", prec, "level", L, "rho", er, "u", eu, flush=True)
+        assert er < tol
+        assert eu < tol
 
 
 def test_cpp_driver_runs():
