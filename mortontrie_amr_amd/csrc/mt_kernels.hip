@@ -528,8 +528,11 @@ __device__ __forceinline__ void ib_links_cell(R* f, const Args& a, const int* N,
 // With COAL the block also coalesces (AMR_transDDF_coalescence,
 // LBM_Manager.cpp:1131-1169) into the F2C parents whose children it holds:
 // children of a parent are the 8 lanes of an aligned group, in the reference's
-// child order, so their post-stream f are summed in that order with lane
-// shuffles; the group's first lane rescales the average around its He-Luo
+// child order; their post-stream f are summed as a DPP butterfly over the group,
+// ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)), not in the reference's sequential order
+// c0+c1+...+c7 (LBM_Manager.cpp:1151-1155, kept by the seam kernel k_coalesce),
+// so the average can differ from the reference's in the last bit; the group's
+// first lane rescales the average around its He-Luo
 // equilibrium and runs the F2C cell's calculateMacros + collide, which the
 // reference does at the start of the coarse level's next sub-step.
 //

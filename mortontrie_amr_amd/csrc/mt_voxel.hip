@@ -26,6 +26,7 @@
 
 #include "mt_case.h"
 #include "mt_device.h"
+#include "mt_morton.h"
 
 // the geometry's arithmetic is evaluated as written (no contraction into fused
 // multiply-adds), so the analytic sphere gives the host builder's bits
@@ -39,16 +40,6 @@
 
 namespace mt {
 namespace {
-
-uint64_t spread_host(uint64_t v) {
-  v &= 0x1FFFFF;
-  v = (v | (v << 32)) & 0x1F00000000FFFFull;
-  v = (v | (v << 16)) & 0x1F0000FF0000FFull;
-  v = (v | (v << 8)) & 0x100F00F00F00F00Full;
-  v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
-  v = (v | (v << 2)) & 0x1249249249249249ull;
-  return v;
-}
 
 // working flags of a dense level grid
 enum : uint32_t {
@@ -534,8 +525,8 @@ int build(const mt_levelset_desc* d, mt_levelset& out) {
     for (int64_t i = 0; i < n; ++i) {
       if (!hk[i]) continue;
       const int64_t z = i % g[L].n[2], t = i / g[L].n[2], y = t % g[L].n[1], x = t / g[L].n[1];
-      const uint64_t key = spread_host((uint64_t)x << sh) | (spread_host((uint64_t)y << sh) << 1) |
-                           (spread_host((uint64_t)z << sh) << 2);
+      const uint64_t key = spread3((uint64_t)x << sh) | (spread3((uint64_t)y << sh) << 1) |
+                           (spread3((uint64_t)z << sh) << 2);
       ord.emplace_back(key, i);
     }
     std::sort(ord.begin(), ord.end());
