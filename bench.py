@@ -144,6 +144,12 @@ def counter_run(case, steps, warmup, copy_mib):
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
+    # a read-only calibration beside the copy: a full reduction reads the
+    # buffer once and writes a few bytes
+    acc = torch.empty(1, dtype=torch.float64, device="cuda:0")
+    for _ in range(3):
+        torch.sum(src, dim=0, keepdim=True, out=acc)
+    torch.cuda.synchronize()
     print(json.dumps({"counter_run": True, "steps": steps, "warmup": warmup, "copy_bytes": n * 8,
                       "fused_grid_threads": [((nb + 7) // 8) * 8 * 64 for nb in active_bricks(case)]}), flush=True)
 

@@ -9,7 +9,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmt_amr.so")
-SOURCES = ["mt_tables.hip", "mt_kernels.hip", "mt_api.hip", "mt_serve.hip", "mt_voxel.hip", "mt_output.hip"]
+SOURCES = ["mt_fused_q27_f64.hip", "mt_fused_q27_f32.hip", "mt_fused_q27_f16.hip", "mt_fused_q19_f64.hip",
+           "mt_fused_q19_f32.hip", "mt_fused_q19_f16.hip", "mt_tables.hip", "mt_kernels.hip", "mt_api.hip",
+           "mt_serve.hip", "mt_voxel.hip", "mt_output.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MT_OFFLOAD_ARCH", "gfx950")
 
@@ -17,6 +19,10 @@ FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     # packed FP32 instructions are kept out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # a multiply-add is fused only where one source expression holds both, so
+    # the rounding of a formula does not depend on the kernel it is inlined
+    # into (the seam and fused kernels compute the same bits)
+    "-ffp-contract=on",
     "-Wno-unused-result", "-Wno-unused-command-line-argument",
     "-I", os.path.join(ROOT, "include"),
 ]
