@@ -55,8 +55,28 @@ def bytes_per_update(Q, precision):
 
 
 def level_updates(case):
+    """Cell updates of one sub-step per level, as the reference counts them: its
+    stream covers lat_f and the C2F cells in every sub-step (Stream_Manager.cpp:43-76)."""
     from mortontrie_amr_amd import cases as C
     return [c["fluid"] + c["c2f"] for c in C.level_counts(case)]
+
+
+def streamed_cells_per_coarse_step(case):
+    """Cells whose populations a coarse step reads and writes: every lat_f cell in
+    every sub-step; a C2F cell in sub-step B (its stream feeds the coalescence),
+    and in sub-step A only where the explosion does not replace its stream before
+    any read (reference interface semantics, or a C2F cell in dis)."""
+    from mortontrie_amr_amd import cases as C
+    both = (case.get("options", 0) & 2) != 0
+    tot = 0
+    for L, c in enumerate(C.level_counts(case)):
+        if L == 0:
+            tot += c["fluid"] + c["c2f"]
+            continue
+        pairs = 1 << (L - 1)
+        a_c2f = c["c2f_dis"] if both else c["c2f"]
+        tot += pairs * (2 * c["fluid"] + c["c2f"] + a_c2f)
+    return tot
 
 
 def measured_traffic():
@@ -175,8 +195,10 @@ def run_batch(args):
     members = C.cavity_batch(args.batch, seed=0)
     packed = C.pack(members)
     upd_step = 0
+    streamed = 0
     for m in members:
         upd_step += sum((c["fluid"] + c["c2f"]) << L for L, c in enumerate(C.level_counts(m)))
+        streamed += streamed_cells_per_coarse_step(m)
     g = amr.AmrCase(packed, precision=amr.FP32_SHIFTED, graphs=True)
     tune_steps = 3
     tune = g.autotune(tune_steps)
@@ -215,7 +237,7 @@ def run_batch(args):
         "alone_equals_packed": same, "nan_count": int(nan), "diverged_members": len(bad),
         "diverged_tau_lid_edge": [[round(members[i]["tau0"], 4), round(-members[i]["faces"][5][1], 4),
                                    members[i]["dims0"][0]] for i in bad[:12]],
-        "algorithmic_GBs": round(upd_step * 152 * args.steps / secs / 1e9, 1),
+        "algorithmic_GBs": round(streamed * 152 * args.steps / secs / 1e9, 1),
     }
     return out
 
@@ -453,9 +475,13 @@ def measure(case, workload, args, variants=True, cpu=True):
     # step's over its 2^Lmax launches
     merged = Lf >= 1 and all(r["calls"][L] == 0 for L in range(Lf))
 
+    # the bytes a launch must move: 2 Q b per cell it streams (the dominant
+    # kernel's algorithmic traffic; MLUPS keeps the reference's count)
+    streamed = streamed_cells_per_coarse_step(case)
+
     def launch_bytes(prec):
         if merged:
-            return upd_step * bytes_per_update(Q, prec) // (1 << Lf)
+            return streamed * bytes_per_update(Q, prec) // (1 << Lf)
         return per_level[Lf] * bytes_per_update(Q, prec)
 
     bytes_launch = launch_bytes(amr.FP64)
@@ -482,6 +508,7 @@ def measure(case, workload, args, variants=True, cpu=True):
             "coarse": list(case["dims0"]), "levels": Lf + 1,
             **({"sphere_radius": 8} if workload != CAVITY_WORKLOAD else {"tau": case["tau0"], "lid": 0.05}),
             "updates_per_level": per_level, "updates_per_coarse_step": upd_step,
+            "streamed_cells_per_coarse_step": streamed,
             "tau0": case["tau0"], "parallelism": "single_gpu",
         },
         "roofline": {

@@ -435,6 +435,7 @@ def level_counts(case):
             "solid": int(((k & KIND_SOLID) != 0).sum()),
             "bcres": int(((k & KIND_BCRES) != 0).sum()),
             "dis": int(((k & KIND_DIS) != 0).sum()),
+            "c2f_dis": int((((k & KIND_C2F) != 0) & ((k & KIND_DIS) != 0)).sum()),
         })
     return out
 

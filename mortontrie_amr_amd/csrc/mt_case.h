@@ -62,6 +62,12 @@ struct DevLevel {
   // (its force slot)
   int32_t* active_b = nullptr;
   int32_t* slot_b = nullptr;
+  // the bricks a sub-step whose C2F cells neither feed a coalescence nor keep
+  // their own stream needs (some lat_f cell, or a C2F cell in dis), in the
+  // order of `active`, with each one's position there
+  int32_t* active_a = nullptr;
+  int32_t* slot_a = nullptr;
+  int64_t n_active_a = 0;
   int32_t* f2c = nullptr;     // F2C slots of this level (coarse side)
   int64_t n_f2c = 0;
   int32_t* f2c_child = nullptr;        // [n_f2c][8] child slots on level+1

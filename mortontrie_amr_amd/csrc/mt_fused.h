@@ -625,7 +625,12 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const uint32_t b
     asm volatile("" ::"v"(k), "v"(pmx), "v"(tm), "v"(row0), "v"(wm0), "v"(par), "v"(gr), "s"(p_tau), "s"(p_les));
     const int row = (IB && (k & MT_KIND_DIS) && (k & MT_KIND_FLUID)) ? row0 : -1;
     const bool treated = row >= 0;
-    if (k & (MT_KIND_FLUID | MT_KIND_C2F)) {
+    // A C2F cell's stream feeds the coalescence (COAL), its own stored f or
+    // fcol (c2f_f, write_f, c2f_fcol) or, in dis, its fcol; in any other
+    // sub-step nothing reads it before an explosion replaces it, so it is not
+    // done.
+    const bool c2f_live = COAL || a.c2f_f || a.write_f || a.c2f_fcol;
+    if ((k & MT_KIND_FLUID) || ((k & MT_KIND_C2F) && (c2f_live || (k & MT_KIND_DIS)))) {
       int lx, ly, lz;
       brick_coords(lane, lx, ly, lz);
       const S* xb = (const S*)a.xbuf;

@@ -980,6 +980,14 @@ static int fused_args(Case& c, int L, bool write_f, bool write_macros, bool c2f_
   a.active = (coal && l.active_b) ? l.active_b : l.active;
   a.slot_of = (coal && l.active_b) ? l.slot_b : nullptr;
   a.n_active = l.n_active;
+  // a sub-step whose C2F cells' stream is dropped (see fused_block) runs only
+  // the bricks that hold other work
+  const bool c2f_live = coal || write_f || c2f_fcol || (L > 0 && !c.lv[L - 1].coal_in_fused);
+  if (!c2f_live && L > 0 && l.active_a) {
+    a.active = l.active_a;
+    a.slot_of = l.slot_a;
+    a.n_active = l.n_active_a;
+  }
   a.xmap = c.xmap;
   a.model = c.d.collision;
   a.write_f = write_f ? 1 : 0;
@@ -1050,7 +1058,7 @@ static int fused_args(Case& c, int L, bool write_f, bool write_macros, bool c2f_
     a.c_write_macros = (coal->write_macros || c.diag) ? 1 : 0;
   }
   // grid: a multiple of 8 blocks so the XCD remap covers every brick index
-  nb = ((l.n_active + 7) / 8) * 8;
+  nb = ((a.n_active + 7) / 8) * 8;
   if (nb < 8) nb = 8;
   // the domain boundary (level 0 only), the coalescence (finer levels only) and
   // the solid boundary (finest level only) are compiled into separate

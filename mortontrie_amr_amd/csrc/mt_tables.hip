@@ -643,6 +643,31 @@ static int order_active(Case& c, int L) {
     }
   }
   MT_HIP(hipMemcpyAsync(l.active, act.data(), 4 * na, hipMemcpyHostToDevice, c.stream));
+  // the bricks with some lat_f cell or C2F cell in dis (a brick of C2F cells
+  // only has nothing to do in a sub-step that drops their stream)
+  std::vector<int32_t> act_a, slot_a;
+  for (int64_t i = 0; i < na; ++i) {
+    bool need = false;
+    for (int j = 0; j < 64 && !need; ++j) {
+      const uint16_t kk = kind[(int64_t)act[i] * 64 + j];
+      need = (kk & MT_KIND_FLUID) || ((kk & MT_KIND_C2F) && (kk & MT_KIND_DIS));
+    }
+    if (need) {
+      act_a.push_back(act[i]);
+      slot_a.push_back((int32_t)i);
+    }
+  }
+  l.n_active_a = (int64_t)act_a.size();
+  if (l.n_active_a) {
+    l.active_a = c.alloc<int32_t>(l.n_active_a);
+    l.slot_a = c.alloc<int32_t>(l.n_active_a);
+    if (!l.active_a || !l.slot_a) {
+      set_error("out of device memory");
+      return MT_ERR_HIP;
+    }
+    MT_HIP(hipMemcpyAsync(l.active_a, act_a.data(), 4 * l.n_active_a, hipMemcpyHostToDevice, c.stream));
+    MT_HIP(hipMemcpyAsync(l.slot_a, slot_a.data(), 4 * l.n_active_a, hipMemcpyHostToDevice, c.stream));
+  }
   if (l.cpar) {
     l.active_b = c.alloc<int32_t>(na);
     l.slot_b = c.alloc<int32_t>(na);
