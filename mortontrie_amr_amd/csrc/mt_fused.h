@@ -413,6 +413,17 @@ template <int CTRL>
 __device__ __forceinline__ float dpp_get(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
+// The value of the first lane of this lane's aligned group of 8 (a bitmask
+// swizzle, lane & ~7 within each half-wave).  Every lane must be active.
+__device__ __forceinline__ double group_first(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_swizzle((int)b, 0x18);
+  const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x18);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ float group_first(float v) {
+  return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x18));
+}
 
 // AMR_transDDF_explosion (LBM_Manager.cpp:1059-1129) of one parent held in a
 // lane: its post-collision fcol rescaled around the He-Luo equilibrium of its
@@ -770,8 +781,6 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const uint32_t b
     sum += dpp_get<0x141>(sum);
     f[q] = sum / (R)8.0;
   }
-  __shared__ R lavg[8][Q];
-  const int g = lane >> 3;
   if (!(lane & 7) && par >= 0) {
     R* avg = f;
     R rho, ux, uy, uz;
@@ -804,16 +813,19 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const uint32_t b
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)P.rho0);
-        lavg[g][q] = fe + sx * (avg[q] - fe);
+        avg[q] = fe + sx * (avg[q] - fe);
       }
     }
   }
   if (!a.cx_dst) return;  // uniform
-  __syncthreads();
-  // children in dis are not exploded into (LBM_Manager.cpp:1095)
-  if (par >= 0 && !(k & MT_KIND_DIS)) {
+  // the group's first lane hands the explosion to its 8 lanes (a lane swizzle,
+  // every lane active); children in dis are not exploded into
+  // (LBM_Manager.cpp:1095)
+  const bool child = par >= 0 && !(k & MT_KIND_DIS);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)a.cx_dst, ns, q, s, lavg[g][q]);
+  for (int q = 0; q < Q; ++q) {
+    const R v = group_first(f[q]);
+    if (child) store_pop<Q, S, R>((S*)a.cx_dst, ns, q, s, v);
   }
 }
 
