@@ -23,8 +23,9 @@ with each variant's drag, velocity / density L2 and mass drift against FP64.
 --workload batch and --workload serve run the packed-cavity batch and the
 request stream.
 
---counters runs a short eager pass plus a calibration copy for the rocprofv3
-FETCH_SIZE / WRITE_SIZE passes whose summary (profiles/) gives roofline.traffic.
+--counters runs a short eager pass plus a calibration copy and a read-only
+reduction for the rocprofv3 passes over the L2's size-resolved read and write
+requests, whose summary (profiles/) gives roofline.traffic.
 """
 from __future__ import annotations
 
@@ -148,8 +149,9 @@ def rel_l2(a, b):
 
 def counter_run(case, steps, warmup, copy_mib):
     """Short eager run for hardware-counter passes: `steps` coarse steps with one
-    dispatch per kernel (no graphs), then a plain device copy of `copy_mib` MiB
-    whose counter readings calibrate bytes per counter unit."""
+    dispatch per kernel (no graphs), then a plain device copy and a read-only
+    reduction of `copy_mib` MiB, whose counter readings check bytes per counter
+    unit."""
     import torch
     from mortontrie_amr_amd import amr
     g = amr.AmrCase(case, precision=amr.FP64, graphs=False)
