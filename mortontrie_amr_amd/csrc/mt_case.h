@@ -93,9 +93,13 @@ struct DevLevel {
   int32_t* cpar = nullptr;
   // explosions inside the fused updates (Case::xin).  Fine side: per slot, bit q
   // set when the pull source x + e_q is an explosion target (C2F / ghost_C2F, not
-  // in dis), read from the pair's target buffer; the target slots.  Coarse side:
+  // in dis), read from the pair's target groups; the target slots.  Coarse side:
   // per slot, the row of a ghost_F2C parent in gf2c_child, else -1.
   uint32_t* tmask = nullptr;
+  // the explosion targets of the current and the next pair: one value per
+  // population per aligned group of 8 slots (the 8 children of one parent all
+  // receive the same values), [Q][n_slots / 8] of the storage format
+  void* xg[2] = {nullptr, nullptr};
   int32_t* xt_list = nullptr;
   int64_t n_xt = 0;
   int32_t* gxrow = nullptr;

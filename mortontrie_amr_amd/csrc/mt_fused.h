@@ -385,7 +385,8 @@ struct FusedArgs {
   // explosions inside the update (Case::xin)
   const uint32_t* __restrict__ tmask;  // bit q: the pull source of q is an explosion target
                                        // (bit 0: the cell itself)
-  const void* xbuf;                    // this pair's explosion targets (fcol[pf of level L-1])
+  const void* xbuf;                    // this pair's explosion targets (xg[pf of level L-1]),
+                                       // one entry per aligned group of 8 slots
   const int32_t* __restrict__ gxrow;   // ghost_F2C parent -> row of gx_child, -1 none
   const int32_t* __restrict__ gx_child;
   void* gx_dst;                        // level L+1 targets of the next pair
@@ -413,36 +414,25 @@ template <int CTRL>
 __device__ __forceinline__ float dpp_get(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
-// The value of the first lane of this lane's aligned group of 8 (a bitmask
-// swizzle, lane & ~7 within each half-wave).  Every lane must be active.
-__device__ __forceinline__ double group_first(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_ds_swizzle((int)b, 0x18);
-  const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x18);
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-__device__ __forceinline__ float group_first(float v) {
-  return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x18));
-}
-
 // AMR_transDDF_explosion (LBM_Manager.cpp:1059-1129) of one parent held in a
 // lane: its post-collision fcol rescaled around the He-Luo equilibrium of its
 // own moments, written to its (up to 8, -1 = none) targets.
 template <int Q, typename S, typename R>
 __device__ __forceinline__ void explode_lane(const R* fc, const int32_t* child, S* dst, int64_t ns, R sc, R rho0) {
+  // the children are one aligned group of 8 slots of the finer level (the 2x2x2
+  // block of a parent), and all receive the same values: the group's target
+  // entry, [Q][ns / 8], is written once
+  int t = -1;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) t = t < 0 ? child[j] : t;
+  if (t < 0) return;
   R rho, ux, uy, uz;
   cell_macros<Q, R>(fc, rho, ux, uy, uz);
   const R uu = ux * ux + uy * uy + uz * uz;
-  int t[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) t[j] = child[j];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, rho0);
-    const R v = fe + sc * (fc[q] - fe);
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (t[j] >= 0) store_pop<Q, S, R>(dst, ns, q, t[j], v);
+    store_pop<Q, S, R>(dst, ns >> 3, q, t >> 3, fe + sc * (fc[q] - fe));
   }
 }
 
@@ -644,8 +634,10 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const uint32_t b
     if ((k & MT_KIND_FLUID) || ((k & MT_KIND_C2F) && (c2f_live || (k & MT_KIND_DIS)))) {
       int lx, ly, lz;
       brick_coords(lane, lx, ly, lz);
-      const S* xb = (const S*)a.xbuf;
-      f[0] = load_pop<Q, S, R>((tm & 1u) ? xb : fin, ns, 0, s);
+      // the pair's explosion targets: one entry per aligned group of 8 slots
+      const S* xg = (const S*)a.xbuf;
+      const int64_t ng = ns >> 3;
+      f[0] = (tm & 1u) ? load_pop<Q, S, R>(xg, ng, 0, s >> 3) : load_pop<Q, S, R>(fin, ns, 0, s);
       // pull f_q(x) <- fcol_q(x + e_q); a missing source keeps the cell's own f_q
       // (Stream_Manager.cpp:53-62), except on a treated solid-boundary cell,
       // which bounces back its own fcol_inv(q) (ComplexBoundary_Manager.cpp:
@@ -655,9 +647,8 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const uint32_t b
         constexpr int q = decltype(qc)::value;
         const int t = nbr_slot_c<kEX[q], kEY[q], kEZ[q]>(N, lx, ly, lz);
         const bool p = (pmx >> q) & 1;
-        const S* pb = ((tm >> q) & 1) ? xb : fin;
-        const S* src = p ? pb + (int64_t)q * ns + t
-                         : (treated ? fin + (int64_t)qinv(q) * ns + s : fst + (int64_t)q * ns + s);
+        const S* pb = ((tm >> q) & 1) ? xg + (int64_t)q * ng + (t >> 3) : fin + (int64_t)q * ns + t;
+        const S* src = p ? pb : (treated ? fin + (int64_t)qinv(q) * ns + s : fst + (int64_t)q * ns + s);
         if (IB && treated && ((wm0 >> q) & 1)) {
           // a wall link's population is interpolated from fcol_q(x - e_q) (or
           // fcol_inv(q)(x) when x - e_q is missing): load that here, with the
@@ -804,7 +795,10 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const uint32_t b
 #pragma unroll
     for (int q = 0; q < Q; ++q) store_pop<Q, S, R>((S*)a.cfcol_out, a.ns_c, q, par, avg[q]);
     if (a.cx_dst) {
-      // the parent's explosion for the next pair, handed to its 8 children
+      // the parent's explosion for the next pair: its 8 children (this group)
+      // all receive these values, kept once as the group's target entry
+      // (children in dis, which are not exploded into (LBM_Manager.cpp:1095),
+      // never read it)
 #pragma unroll
       for (int q = 0; q < Q; ++q) avg[q] = snap_pop<Q, S, R>(q, avg[q]);
       cell_macros<Q, R>(avg, rho, ux, uy, uz);
@@ -813,19 +807,9 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const uint32_t b
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         const R fe = feq_hl<Q, R>(q, rho, ux, uy, uz, uu, (R)P.rho0);
-        avg[q] = fe + sx * (avg[q] - fe);
+        store_pop<Q, S, R>((S*)a.cx_dst, ns >> 3, q, s >> 3, fe + sx * (avg[q] - fe));
       }
     }
-  }
-  if (!a.cx_dst) return;  // uniform
-  // the group's first lane hands the explosion to its 8 lanes (a lane swizzle,
-  // every lane active); children in dis are not exploded into
-  // (LBM_Manager.cpp:1095)
-  const bool child = par >= 0 && !(k & MT_KIND_DIS);
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const R v = group_first(f[q]);
-    if (child) store_pop<Q, S, R>((S*)a.cx_dst, ns, q, s, v);
   }
 }
 

@@ -265,6 +265,7 @@ struct XferArgs {
   const void* src;     // coarse fcol (explosion) / fine f (coalescence)
   void* dst;           // fine fcol (explosion) / coarse f (coalescence)
   void* dst2;          // explosion: second fine fcol buffer for ghost_C2F targets
+  int group;           // explosion: dst holds one value per aligned group of 8 children
   const int32_t* list; // parent slots
   int64_t n;
   const int32_t* child;  // [n][8]
@@ -318,6 +319,14 @@ __global__ __launch_bounds__(256) void k_explode(XferArgs a, int64_t n_plain) {
   if (ip >= a.n) return;
   const int dst = a.child[ip * 8 + j];
   if (dst < 0) return;
+  if (a.group) {
+    // the children are one aligned group of 8 slots and all receive these
+    // values: the group's entry is written once, by its first present child
+    for (int jj = 0; jj < j; ++jj)
+      if (a.child[ip * 8 + jj] >= 0) return;
+    for (int r = w; r < Q; r += 4) store_pop<Q, S, R>((S*)a.dst, a.ns_f >> 3, r, dst >> 3, lv[pp][r]);
+    return;
+  }
   const bool ghost = ip >= n_plain;
   for (int r = w; r < Q; r += 4) {
     store_pop<Q, S, R>((S*)a.dst, a.ns_f, r, dst, lv[pp][r]);
@@ -881,6 +890,7 @@ int launch_explode(Case& c, int L, bool mirror_all) {
   a.src = lc.fcol[lc.pf];
   a.dst = lf.fcol[lf.pf];
   a.dst2 = lf.fcol[1 - lf.pf];
+  a.group = 0;
   a.mp = c.d_mem;
   a.bmember_c = lc.bmember;
   a.Lc = L - 1;
@@ -895,7 +905,7 @@ int launch_explode(Case& c, int L, bool mirror_all) {
   return MT_OK;
 }
 
-// All targets of level L's explosion, once, into fcol[dst_buf] of level L
+// All targets of level L's explosion, once, into its target groups xg[dst_buf]
 // (explosions inside the fused updates: priming the first pair).
 int launch_explode_to(Case& c, int L, int dst_buf) {
   if (L <= 0) return MT_OK;
@@ -906,8 +916,9 @@ int launch_explode_to(Case& c, int L, int dst_buf) {
   a.ns_c = lc.n_slots;
   a.ns_f = lf.n_slots;
   a.src = lc.fcol[lc.pf];
-  a.dst = lf.fcol[dst_buf];
+  a.dst = lf.xg[dst_buf];
   a.dst2 = nullptr;
+  a.group = 1;
   a.mp = c.d_mem;
   a.bmember_c = lc.bmember;
   a.Lc = L - 1;
@@ -926,15 +937,16 @@ __global__ void k_copy_targets(const int32_t* list, int64_t n, int64_t ns, const
   if (i >= n) return;
   const int64_t s = list[i];
 #pragma unroll
-  for (int q = 0; q < Q; ++q) to[q * ns + s] = from[q * ns + s];
+  for (int q = 0; q < Q; ++q) to[q * ns + s] = from[q * (ns >> 3) + (s >> 3)];
 }
 
-// Copy the explosion targets of level L between its fcol buffers.
+// Write level L's explosion targets of the target groups xg[from_buf] into the
+// target slots of fcol[to_buf].
 int launch_copy_targets(Case& c, int L, int from_buf, int to_buf) {
   DevLevel& l = c.lv[L];
-  if (!l.n_xt || from_buf == to_buf) return MT_OK;
+  if (!l.n_xt || !l.xg[from_buf]) return MT_OK;
   MT_DISPATCH(c, (k_copy_targets<Q_, S_><<<nblk(l.n_xt), 256, 0, c.stream>>>(
-                     l.xt_list, l.n_xt, l.n_slots, (const S_*)l.fcol[from_buf], (S_*)l.fcol[to_buf])));
+                     l.xt_list, l.n_xt, l.n_slots, (const S_*)l.xg[from_buf], (S_*)l.fcol[to_buf])));
   MT_HIP(hipGetLastError());
   return MT_OK;
 }
@@ -1030,17 +1042,17 @@ static int fused_args(Case& c, int L, bool write_f, bool write_macros, bool c2f_
   if (c.xin) {
     if (L >= 1) {
       a.tmask = l.tmask;
-      a.xbuf = l.fcol[c.lv[L - 1].pf];
+      a.xbuf = l.xg[c.lv[L - 1].pf];
     }
     if (L < c.Lmax && l.gxrow) {
       DevLevel& lf = c.lv[L + 1];
       a.gxrow = l.gxrow;
       a.gx_child = l.gf2c_child;
-      a.gx_dst = lf.fcol[1 - l.pf];
+      a.gx_dst = lf.xg[1 - l.pf];
       a.ns_f = lf.n_slots;
     }
     if (coal) {
-      a.cx_dst = l.fcol[1 - c.lv[L - 1].pf];
+      a.cx_dst = l.xg[1 - c.lv[L - 1].pf];
     }
   }
   if (coal) {

@@ -926,6 +926,13 @@ int build_tables(Case& c) {
           return MT_ERR_HIP;
         }
         k_tmask<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.bnbr, l.kind, l.pmask, l.n_slots, c.Q, l.tmask);
+        for (int j = 0; j < 2; ++j) {
+          l.xg[j] = c.alloc<char>((l.n_slots / 8) * c.Q * (int64_t)c.pop_bytes());
+          if (!l.xg[j]) {
+            set_error("out of device memory");
+            return MT_ERR_HIP;
+          }
+        }
       }
       if (L < c.Lmax && l.n_gf2c) {
         l.gxrow = c.alloc<int32_t>(l.n_slots);
