@@ -170,6 +170,7 @@ struct Case {
   // span ends with it (takes precedence over level_streams)
   bool merge_levels = true;
   bool wet_any = false;  // some member has a Zou-He / outflow face
+  bool nee_code = false;  // some wet-node cell may need the NEE extrapolation (fused kernels carry it)
   bool res_any = false;  // some member has a reservoir face
   // Explosions inside the fused updates (with MT_OPT_EXPLODE_BOTH).  Both
   // explosions of a fine pair write the same values, so the pair's targets are

@@ -144,7 +144,9 @@ __device__ __forceinline__ void nee_at(R* f, int t, R wx, R wy, R wz, bool own_u
 // nb(dx,dy,dz) returns the slot of the cell at offset d (-1: none, the cell's
 // own value stands in for it); val(t, q) the post-stream f_q of the cell at
 // slot t.
-template <int Q, typename R, typename NB, typename VAL>
+// NEE = false: a kernel for cases with no NEE face and no edge or corner of wet
+// faces (the host checks), which leaves the extrapolation's code out.
+template <int Q, typename R, bool NEE = true, typename NB, typename VAL>
 __device__ __forceinline__ void wet_bc(R* f, uint16_t k, const BcCfg& bc, NB nb, VAL val) {
   const int act = ((k >> 8) & 63) & bc.wet_mask;
   if (!act) return;
@@ -152,11 +154,13 @@ __device__ __forceinline__ void wet_bc(R* f, uint16_t k, const BcCfg& bc, NB nb,
     const int face = __ffs(act) - 1;
     const int ax = face >> 1;
     const int sg = (face & 1) ? 1 : -1;
-    if (bc.kind[face] == MT_BC_NEE) {
-      // f_q = feq_q(rho_n, u_w) + f_q(x - n) - feq_q(rho_n, u_n), all q
-      const int nx = ax == 0 ? sg : 0, ny = ax == 1 ? sg : 0, nz = ax == 2 ? sg : 0;
-      nee_at<Q, R>(f, nb(-nx, -ny, -nz), (R)bc.u[face][0], (R)bc.u[face][1], (R)bc.u[face][2], false, val);
-      return;
+    if constexpr (NEE) {
+      if (bc.kind[face] == MT_BC_NEE) {
+        // f_q = feq_q(rho_n, u_w) + f_q(x - n) - feq_q(rho_n, u_n), all q
+        const int nx = ax == 0 ? sg : 0, ny = ax == 1 ? sg : 0, nz = ax == 2 ? sg : 0;
+        nee_at<Q, R>(f, nb(-nx, -ny, -nz), (R)bc.u[face][0], (R)bc.u[face][1], (R)bc.u[face][2], false, val);
+        return;
+      }
     }
     if (bc.kind[face] == MT_BC_OUTFLOW) {
       const int nx = ax == 0 ? sg : 0, ny = ax == 1 ? sg : 0, nz = ax == 2 ? sg : 0;
@@ -211,19 +215,21 @@ __device__ __forceinline__ void wet_bc(R* f, uint16_t k, const BcCfg& bc, NB nb,
     }
     return;
   }
-  // edge / corner: non-equilibrium extrapolation from the inward diagonal cell
-  int dx = 0, dy = 0, dz = 0, uface = -1;
-  for (int ff = 0; ff < 6; ++ff)
-    if (act & (1 << ff)) {
-      const int sg = (ff & 1) ? 1 : -1, ax = ff >> 1;
-      if (ax == 0) dx -= sg; else if (ax == 1) dy -= sg; else dz -= sg;
-      if (bc.kind[ff] == MT_BC_ZOUHE || bc.kind[ff] == MT_BC_NEE) uface = ff;
-    }
-  const int t = nb(dx, dy, dz);
-  if (uface >= 0)
-    nee_at<Q, R>(f, t, (R)bc.u[uface][0], (R)bc.u[uface][1], (R)bc.u[uface][2], false, val);
-  else
-    nee_at<Q, R>(f, t, (R)0, (R)0, (R)0, true, val);
+  if constexpr (NEE) {
+    // edge / corner: non-equilibrium extrapolation from the inward diagonal cell
+    int dx = 0, dy = 0, dz = 0, uface = -1;
+    for (int ff = 0; ff < 6; ++ff)
+      if (act & (1 << ff)) {
+        const int sg = (ff & 1) ? 1 : -1, ax = ff >> 1;
+        if (ax == 0) dx -= sg; else if (ax == 1) dy -= sg; else dz -= sg;
+        if (bc.kind[ff] == MT_BC_ZOUHE || bc.kind[ff] == MT_BC_NEE) uface = ff;
+      }
+    const int t = nb(dx, dy, dz);
+    if (uface >= 0)
+      nee_at<Q, R>(f, t, (R)bc.u[uface][0], (R)bc.u[uface][1], (R)bc.u[uface][2], false, val);
+    else
+      nee_at<Q, R>(f, t, (R)0, (R)0, (R)0, true, val);
+  }
 }
 
 // Solid boundary on one dis cell (SinglePointInterpolation::treat,
@@ -448,7 +454,7 @@ __device__ __forceinline__ void explode_lane(const R* fc, const int32_t* child, 
 // m <- m + 3 W dM with W = sum_wall w_q e_q e_q^T: the sweeps reduce to that
 // 3-vector recurrence, and each wall link finally receives 3 w_q e_q.D with D
 // the sum of the sweeps' dM (f_0: -3 V.D, V = sum_wall w_q e_q).
-template <int Q, typename S, typename R, typename Args>
+template <int Q, typename S, typename R, int CH, typename Args>
 __device__ __forceinline__ void ib_links_cell(R* f, const Args& a, const int* N, int lx, int ly, int lz, int s,
                                               int row, uint32_t pm, double& fx, double& fy, double& fz) {
   const S* fin = (const S*)a.v.fcol_in;
@@ -459,8 +465,8 @@ __device__ __forceinline__ void ib_links_cell(R* f, const Args& a, const int* N,
   R wxx = 0, wyy = 0, wzz = 0, wxy = 0, wxz = 0, wyz = 0, vx = 0, vy = 0, vz = 0;
   // Wall links in chunks of CH directions: the chunk's loads (fraction,
   // fcol_inv(q)(x), fcol_q(x - e_q)) are issued together, then used, so a cell
-  // waits for memory once per chunk instead of once per wall link.
-  constexpr int CH = 9;
+  // waits for memory once per chunk instead of once per wall link (CH is set by
+  // the registers the kernel can give them).
   static_for<0, (Q - 1 + CH - 1) / CH>([&](auto cc) {
     constexpr int c0 = 1 + decltype(cc)::value * CH;
     double dist_c[CH];
@@ -583,7 +589,10 @@ __device__ __forceinline__ void ib_links_cell(R* f, const Args& a, const int* N,
 // fused_block is the update of block `blk` of a launch of `nb` blocks (a
 // multiple of 8) for one level; k_fused launches one level, k_fused_levels
 // several levels side by side.
-template <int Q, typename S, int CK, bool COAL, bool WET, bool IB>
+// ROOM: the kernel already holds a variant whose registers leave no fourth wave
+// per SIMD (the domain boundary's NEE extrapolation), so the solid boundary
+// loads its wall links in fewer, larger chunks (fewer memory round trips).
+template <int Q, typename S, int CK, bool COAL, bool WET, bool IB, bool ROOM = false, bool NEE = true>
 __device__ __forceinline__ void fused_block(const FusedArgs& a, const uint32_t blk, const int64_t nb) {
   typedef typename RealOf<S>::type R;
   const int lane = threadIdx.x;
@@ -662,7 +671,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const uint32_t b
         f[q] = v;
       });
       if (treated) {
-        ib_links_cell<Q, S, R>(f, a, N, lx, ly, lz, s, row, pmx, fx, fy, fz);
+        ib_links_cell<Q, S, R, (ROOM ? (sizeof(R) == 8 ? 13 : 26) : 9)>(f, a, N, lx, ly, lz, s, row, pmx, fx, fy, fz);
 #pragma unroll
         for (int q = 0; q < Q; ++q) f[q] = snap_pop<Q, S, R>(q, f[q]);
       }
@@ -679,7 +688,7 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const uint32_t b
           }
           return load_pop<Q, S, R>(fst, ns, q, t);
         };
-        wet_bc<Q, R>(f, k, P.bc, nb, val);
+        wet_bc<Q, R, NEE>(f, k, P.bc, nb, val);
 #pragma unroll
         for (int q = 0; q < Q; ++q) f[q] = snap_pop<Q, S, R>(q, f[q]);
       }
@@ -831,9 +840,10 @@ struct LevelsArgs {
 };
 
 // The variants a launch of several levels may hold (bit 0: a level with the
-// domain boundary, bit 1: a level with the solid boundary): the kernel carries
-// only those, so a launch without them is not sized for their registers.
-enum { VM_WET = 1, VM_IB = 2 };
+// domain boundary, bit 1: a level with the solid boundary, bit 2: the domain
+// boundary's NEE extrapolation): the kernel carries only those, so a launch
+// without them is not sized for their registers.
+enum { VM_WET = 1, VM_IB = 2, VM_NEE = 4 };
 
 template <int Q, typename S, int CK, int VM>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_fused_levels(LevelsArgs m) {
@@ -847,13 +857,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     case 0: fused_block<Q, S, CK, false, false, false>(a, blk - b0, nb); break;
     case 1: fused_block<Q, S, CK, true, false, false>(a, blk - b0, nb); break;
     case 2:
-      if constexpr ((VM & VM_WET) != 0) fused_block<Q, S, CK, false, true, false>(a, blk - b0, nb);
+      if constexpr ((VM & VM_WET) != 0) fused_block<Q, S, CK, false, true, false, false, (VM & VM_NEE) != 0>(a, blk - b0, nb);
       break;
     case 4:
-      if constexpr ((VM & VM_IB) != 0) fused_block<Q, S, CK, false, false, true>(a, blk - b0, nb);
+      if constexpr ((VM & VM_IB) != 0) fused_block<Q, S, CK, false, false, true, (VM & VM_NEE) != 0>(a, blk - b0, nb);
       break;
     case 5:
-      if constexpr ((VM & VM_IB) != 0) fused_block<Q, S, CK, true, false, true>(a, blk - b0, nb);
+      if constexpr ((VM & VM_IB) != 0) fused_block<Q, S, CK, true, false, true, (VM & VM_NEE) != 0>(a, blk - b0, nb);
       break;
     default: break;  // (a level with both boundaries is a single-level case)
   }
@@ -892,11 +902,13 @@ static int launch_fused_ck(int variant, FusedArgs& a, int64_t nb, hipStream_t st
 
 template <int Q, typename S, int CK>
 static int launch_fused_levels_ck(int vm, LevelsArgs& m, int64_t nb, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
-  switch (vm & 3) {
-    case 0: return launch_events(k_fused_levels<Q, S, CK, 0>, nb, 64, st, m, e0, e1);
+  switch (vm & 7) {
+    case 0: case 4: return launch_events(k_fused_levels<Q, S, CK, 0>, nb, 64, st, m, e0, e1);
     case 1: return launch_events(k_fused_levels<Q, S, CK, VM_WET>, nb, 64, st, m, e0, e1);
-    case 2: return launch_events(k_fused_levels<Q, S, CK, VM_IB>, nb, 64, st, m, e0, e1);
-    default: return launch_events(k_fused_levels<Q, S, CK, VM_WET | VM_IB>, nb, 64, st, m, e0, e1);
+    case 5: return launch_events(k_fused_levels<Q, S, CK, VM_WET | VM_NEE>, nb, 64, st, m, e0, e1);
+    case 2: case 6: return launch_events(k_fused_levels<Q, S, CK, VM_IB>, nb, 64, st, m, e0, e1);
+    case 3: return launch_events(k_fused_levels<Q, S, CK, VM_WET | VM_IB>, nb, 64, st, m, e0, e1);
+    default: return launch_events(k_fused_levels<Q, S, CK, VM_WET | VM_IB | VM_NEE>, nb, 64, st, m, e0, e1);
   }
 }
 

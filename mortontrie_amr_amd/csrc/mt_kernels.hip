@@ -1166,6 +1166,7 @@ int launch_fused_levels(Case& c, int n, const LevelUpdate* u, bool nee_after) {
   for (int i = 0; i < n; ++i) Lf = std::max(Lf, u[i].L);
   int vm = 0;
   for (int i = 0; i < n; ++i) vm |= (m.variant[i] & 2) ? VM_WET : 0, vm |= (m.variant[i] & 4) ? VM_IB : 0;
+  if ((vm & VM_WET) && c.nee_code) vm |= VM_NEE;
   int r = MT_OK;
   const int ck = collide_kind(c);
   MT_DISPATCH(c, (r = launch_fused_levels_qs<Q_, S_>(ck, vm, m, total, c.st[Lf], e0, e1)));

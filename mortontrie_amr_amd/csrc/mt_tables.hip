@@ -547,7 +547,7 @@ void set_member_tau(const Case& c, MemberParams& P, const double* tau) {
 // option), the wet-node face configuration and the member's box per level.
 int build_members(Case& c) {
   c.h_mem.clear();
-  c.wet_any = c.res_any = false;
+  c.wet_any = c.res_any = c.nee_code = false;
   for (const mt_member_desc& m : c.h_mdesc) {
     MemberParams P = {};
     double tau[kMaxLevels];
@@ -565,6 +565,14 @@ int build_members(Case& c) {
       if (m.face_kind[f] == MT_BC_RESERVOIR) c.res_any = true;
     }
     if (P.bc.wet_mask) c.wet_any = true;
+    // the non-equilibrium extrapolation runs on an NEE face and on the edges and
+    // corners where wet faces of two axes meet
+    int axes = 0;
+    for (int f = 0; f < 6; ++f) {
+      if (m.face_kind[f] == MT_BC_NEE) c.nee_code = true;
+      if ((P.bc.wet_mask >> f) & 1) axes |= 1 << (f >> 1);
+    }
+    if (__builtin_popcount(axes) >= 2) c.nee_code = true;
     for (int L = 0; L <= c.Lmax; ++L)
       for (int a = 0; a < 3; ++a) {
         P.lo[L][a] = m.origin0[a] << L;
