@@ -255,6 +255,12 @@ int mt_set_tau(mt_case* c, int32_t member, const double* tau, int32_t n_levels);
 #define MT_SCHED_STREAMS 1
 #define MT_SCHED_SERIAL 2
 int mt_set_schedule(mt_case* c, int schedule);
+/* The schedule mt_step runs (no counterpart in the reference): the one set,
+ * except MT_SCHED_MERGED where the explosions do not run inside the updates
+ * (a case without MT_OPT_EXPLODE_BOTH, or an interface whose coalescence is not
+ * fused), which runs as MT_SCHED_SERIAL, or as MT_SCHED_STREAMS when no
+ * schedule was set.  Valid after mt_case_finalize.                            */
+int mt_get_schedule(mt_case* c, int32_t* schedule);
 /* Launch-configuration autotuner (no counterpart in the reference): tries the
  * fused kernel's block -> brick maps and the schedules MT_SCHED_STREAMS and
  * MT_SCHED_MERGED, twice each, for `steps` coarse steps of the real run (+1

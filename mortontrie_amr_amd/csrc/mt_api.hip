@@ -1022,6 +1022,12 @@ int mt_set_schedule(mt_case* c, int schedule) {
   drop_graphs(*c);
   return MT_OK;
 }
+int mt_get_schedule(mt_case* c, int32_t* schedule) {
+  CHECK_FINAL(c);
+  if (!schedule) { set_error("null output"); return MT_ERR_ARG; }
+  *schedule = merged(*c) ? MT_SCHED_MERGED : c->level_streams ? MT_SCHED_STREAMS : MT_SCHED_SERIAL;
+  return MT_OK;
+}
 int mt_set_timing(mt_case* c, int on) {
   CHECK_CASE(c);
   c->timing = on != 0;

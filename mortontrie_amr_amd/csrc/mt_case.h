@@ -92,8 +92,9 @@ struct DevLevel {
   bool coal_in_fused = false;
   int32_t* cpar = nullptr;
   // explosions inside the fused updates (Case::xin).  Fine side: per slot, bit q
-  // set when the pull source x + e_q is an explosion target (C2F / ghost_C2F, not
-  // in dis), read from the pair's target groups; the target slots.  Coarse side:
+  // set when the pull source x + e_q (x - e_q on a treated cell's wall link) is
+  // an explosion target (C2F / ghost_C2F, not in dis), read from the pair's
+  // target groups; the target slots.  Coarse side:
   // per slot, the row of a ghost_F2C parent in gf2c_child, else -1.
   uint32_t* tmask = nullptr;
   // the explosion targets of the current and the next pair: one value per

@@ -389,7 +389,7 @@ struct FusedArgs {
   void* cu_out;
   int c_write_macros;
   // explosions inside the update (Case::xin)
-  const uint32_t* __restrict__ tmask;  // bit q: the pull source of q is an explosion target
+  const uint32_t* __restrict__ tmask;  // bit q: the pull source of q (x - e_q on a wall link) is an explosion target
                                        // (bit 0: the cell itself)
   const void* xbuf;                    // this pair's explosion targets (xg[pf of level L-1]),
                                        // one entry per aligned group of 8 slots
@@ -661,9 +661,11 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const uint32_t b
         if (IB && treated && ((wm0 >> q) & 1)) {
           // a wall link's population is interpolated from fcol_q(x - e_q) (or
           // fcol_inv(q)(x) when x - e_q is missing): load that here, with the
-          // pull, instead of the source inside the wall
+          // pull, instead of the source inside the wall; tmask bit q marks an
+          // x - e_q that is an explosion target
           constexpr int qi = qinv(q);
-          src = ((pmx >> qi) & 1) ? fin + (int64_t)q * ns + nbr_slot_c<-kEX[q], -kEY[q], -kEZ[q]>(N, lx, ly, lz)
+          const int tb = nbr_slot_c<-kEX[q], -kEY[q], -kEZ[q]>(N, lx, ly, lz);
+          src = ((pmx >> qi) & 1) ? (((tm >> q) & 1) ? xg + (int64_t)q * ng + (tb >> 3) : fin + (int64_t)q * ns + tb)
                                   : fin + (int64_t)qi * ns + s;
         }
         R v = (R)Store<S>::ld(src);

@@ -306,10 +306,12 @@ __global__ void k_cpar(const int32_t* f2c, const int32_t* child, int64_t n, cons
   else *bad = 1;
 }
 // Explosions inside the fused updates: bit q of tmask[s] is set when the pull
-// source x + e_q of a streamed cell is an explosion target (C2F / ghost_C2F not
-// in dis), whose value the pull then reads from the pair's target buffer.
-__global__ void k_tmask(const int32_t* bnbr, const uint16_t* kind, const uint32_t* pmask, int64_t ns, int Q,
-                        uint32_t* tmask) {
+// source of population q of a streamed cell is an explosion target (C2F /
+// ghost_C2F not in dis), whose value the pull then reads from the pair's
+// target buffer.  The source is x + e_q, except on a wall link of a treated
+// solid-boundary cell (wmask, finest level), whose pull loads x - e_q instead.
+__global__ void k_tmask(const int32_t* bnbr, const uint16_t* kind, const uint32_t* pmask, const uint32_t* wmask,
+                        int64_t ns, int Q, uint32_t* tmask) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   uint32_t m = 0;
@@ -318,9 +320,13 @@ __global__ void k_tmask(const int32_t* bnbr, const uint16_t* kind, const uint32_
     brick_coords((int)(s & 63), lx, ly, lz);
     const int* nb27 = bnbr + (s >> 6) * 27;
     const uint32_t pm = pmask[s];
+    const bool treated = wmask && (kind[s] & MT_KIND_FLUID) && (kind[s] & MT_KIND_DIS);
+    const uint32_t wm = treated ? wmask[s] : 0u;
     for (int q = 1; q < Q; ++q) {
-      if (!((pm >> q) & 1)) continue;
-      const int t = slot_at(nb27, lx, ly, lz, kEX[q], kEY[q], kEZ[q]);
+      const bool wall = (wm >> q) & 1;
+      if (!((pm >> (wall ? qinv(q) : q)) & 1)) continue;
+      const int t = wall ? slot_at(nb27, lx, ly, lz, -kEX[q], -kEY[q], -kEZ[q])
+                         : slot_at(nb27, lx, ly, lz, kEX[q], kEY[q], kEZ[q]);
       if (t >= 0 && (kind[t] & (MT_KIND_C2F | MT_KIND_GHOST_C2F)) && !(kind[t] & MT_KIND_DIS)) m |= 1u << q;
     }
     // bit 0: the cell is a target itself, so its rest population (the pull
@@ -337,24 +343,6 @@ __global__ void k_flag_target(const uint16_t* kind, int64_t ns, int* flag) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   flag[s] = (kind[s] & (MT_KIND_C2F | MT_KIND_GHOST_C2F)) && !(kind[s] & MT_KIND_DIS);
-}
-// flag[i] = 1 when treated cell list[i] has an explosion target among its 26
-// neighbours (its link treatment would read the pair's target buffer)
-__global__ void k_ib_touch(const int32_t* list, int64_t n, const int32_t* bnbr, const uint16_t* kind, int* flag) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int s = list[i];
-  int lx, ly, lz;
-  brick_coords(s & 63, lx, ly, lz);
-  const int* nb27 = bnbr + (int64_t)(s >> 6) * 27;
-  int hit = 0;
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int t = slot_at(nb27, lx, ly, lz, dx, dy, dz);
-        if (t >= 0 && (kind[t] & (MT_KIND_C2F | MT_KIND_GHOST_C2F))) hit = 1;
-      }
-  flag[i] = hit;
 }
 __global__ void k_flag_brick(const uint16_t* kind, int64_t nb, uint16_t mask, int* flag) {
   int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -903,21 +891,9 @@ int build_tables(Case& c) {
       l.coal_in_fused = hbad == 0;
     }
     // explosions inside the fused updates: needs the in-kernel coalescence on
-    // every interface and no treated cell next to an explosion target
+    // every interface
     bool xin = (c.d.options & MT_OPT_EXPLODE_BOTH) && c.Lmax >= 1;
     for (int L = 0; L < c.Lmax && xin; ++L) xin = c.lv[L].coal_in_fused;
-    DevLevel& lm = c.lv[c.Lmax];
-    if (xin && lm.n_ibl) {
-      int* d_hit = nullptr;
-      MT_HIP(dev_malloc((void**)&d_hit, 4 * lm.n_ibl, c.stream));
-      k_ib_touch<<<nblk(lm.n_ibl), 256, 0, c.stream>>>(lm.ibl, lm.n_ibl, lm.bnbr, lm.kind, d_hit);
-      MT_HIP(hipGetLastError());
-      std::vector<int> h(lm.n_ibl);
-      MT_HIP(hipMemcpyAsync(h.data(), d_hit, 4 * lm.n_ibl, hipMemcpyDeviceToHost, c.stream));
-      MT_HIP(hipStreamSynchronize(c.stream));
-      dev_free(d_hit, c.stream);
-      for (int v : h) xin = xin && v == 0;
-    }
     for (int L = 0; L <= c.Lmax && xin; ++L) {
       DevLevel& l = c.lv[L];
       if (L >= 1) {
@@ -933,7 +909,8 @@ int build_tables(Case& c) {
           set_error("out of device memory");
           return MT_ERR_HIP;
         }
-        k_tmask<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.bnbr, l.kind, l.pmask, l.n_slots, c.Q, l.tmask);
+        k_tmask<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.bnbr, l.kind, l.pmask, L == c.Lmax ? l.wmask : nullptr,
+                                                       l.n_slots, c.Q, l.tmask);
         for (int j = 0; j < 2; ++j) {
           l.xg[j] = c.alloc<char>((l.n_slots / 8) * c.Q * (int64_t)c.pop_bytes());
           if (!l.xg[j]) {

@@ -147,17 +147,60 @@ def test_sphere_three_levels(kw):
 
 @pytest.mark.parametrize("opts", [2, 3])
 def test_sphere_explosions_inside_updates(opts):
-    """A wider finest shell keeps the treated cells away from the C2F layer, so
-    with both-sub-step explosions the fused updates carry the explosions
+    """With both-sub-step explosions the fused updates carry the explosions
     themselves (targets double-buffered per coarse parity).  Steps 2+ read the
     target buffer that differs from the level's own fcol, the case that shows
     whether every read of a target (including a C2F cell's rest population)
     goes to the target buffer."""
     case = C.sphere_amr((32, 24, 24), radius=3.0, center=(12, 12, 12), shells=(5.5, 2.0), options=opts)
     g, o = amr.AmrCase(case), O.OracleCase(case)
+    assert g.schedule() == "merged"
     g.step(5)
     o.step(5)
     compare(g, o, case)
+    fl = o.force_log()
+    s, n = g.ib_force()
+    assert n == len(fl)
+    assert np.allclose(s, fl.sum(0), rtol=1e-11, atol=1e-11)
+
+
+def treated_next_to_targets(case):
+    """Treated finest-level cells (fluid, in dis) with an explosion target
+    (C2F / ghost_C2F not in dis) among their 26 neighbours."""
+    Lm = case["Lmax"]
+    lv = case["levels"][Lm]
+    x, y, z = C.decode_key(lv["keys"], Lm, Lm)
+    kind = {(int(a), int(b), int(c)): int(k) for a, b, c, k in zip(x, y, z, lv["kinds"])}
+    dx, dy, dz = C.decode_key(case["dis_keys"], Lm, Lm)
+    n = 0
+    for p in zip(dx.tolist(), dy.tolist(), dz.tolist()):
+        if not kind.get(p, 0) & C.KIND_FLUID:
+            continue
+        for o in np.ndindex(3, 3, 3):
+            t = kind.get((p[0] + o[0] - 1, p[1] + o[1] - 1, p[2] + o[2] - 1), 0)
+            if t & (C.KIND_C2F | C.KIND_GHOST_C2F) and not t & C.KIND_DIS:
+                n += 1
+                break
+    return n
+
+
+@pytest.mark.parametrize("collision", [C.COLLIDE_CUMULANT, C.COLLIDE_LBGK])
+def test_explosions_inside_updates_next_to_treated_cells(collision):
+    """A thin finest shell (the served requests' 3 coarse cells) puts explosion
+    targets next to treated cells.  Their wall links interpolate from their own
+    slot and from x - e_q, toward the wall; every other population comes from
+    the pull, which reads a target from the pair's target buffer.  So the
+    explosions still run inside the updates (merged schedule), and the run is
+    the oracle's."""
+    case = C.sphere_amr((48, 24, 24), radius=5.0, center=(16, 12, 12), shells=(3.0,), Re=50.0, options=2,
+                        collision=collision)
+    assert treated_next_to_targets(case) > 0
+    g, o = amr.AmrCase(case), O.OracleCase(case)
+    assert g.schedule() == "merged"
+    for n in (3, 4):
+        g.step(n)
+        o.step(n)
+        compare(g, o, case)
     fl = o.force_log()
     s, n = g.ib_force()
     assert n == len(fl)
