@@ -20,8 +20,11 @@
 //   * after every chunk the device scans each member for non-finite
 //     populations (mt_scan_members, the reference's NaN checks per member); a
 //     member that has any, or whose drag is not finite, finishes at once with
-//     the status MT_REQ_DIVERGED and is evicted at the next step's repack, which
-//     is then due whatever the admission interval.
+//     the status MT_REQ_DIVERGED;
+//   * a repack that evicts the finished and diverged members is due, whatever
+//     the admission interval, when it pays: their share of the batch's
+//     footprint times the chunks the batch may still run (the most any running
+//     member has left) reaches kRepackChunks, the cost of a repack in chunks.
 #include <algorithm>
 #include <cmath>
 #include <deque>
@@ -36,6 +39,10 @@ namespace mt {
 namespace {
 
 constexpr int kSlotX = 100, kSlotY = 48, kSlotZ = 48;  // level-0 cells per slot
+// A repack (build the packed case's tables, copy the running members' state)
+// costs about as much as one and a half chunks of 100 steps of a full batch of
+// served requests on one MI355X (about 50 ms against 38 ms).
+constexpr double kRepackChunks = 1.5;
 
 // a level-L key (keys of every level live on the finest grid, shifted by
 // 3 (Lmax - L)) moved by (dx, dy, dz) level-L cells
@@ -75,7 +82,6 @@ struct Server {
   std::vector<std::unique_ptr<Request>> results;
   mt_case* batch = nullptr;
   int since = 1 << 30;
-  bool evict = false;  // a member diverged: repack at the next step
   std::string broken;  // set when a repack or a step failed; every later call reports it
 };
 
@@ -279,15 +285,18 @@ int mt_server_submit(mt_server* s, mt_case* req, int64_t id, int32_t max_steps, 
 int mt_server_step(mt_server* s, int32_t* n_running, int32_t* n_waiting) {
   if (!s) { set_error("null server"); return MT_ERR_ARG; }
   if (!s->broken.empty()) { set_error(s->broken); return MT_ERR_STATE; }
-  int64_t active = 0, used = 0;
+  int64_t active = 0, used = 0, retired = 0, left = 0;
   for (auto& r : s->running)
     if (!r->done) {
       ++active;
       used += r->footprint;
+      left = std::max<int64_t>(left, (r->max_steps - r->steps + s->chunk - 1) / s->chunk);
+    } else {
+      retired += r->footprint;
     }
   const bool due = !s->waiting.empty() && (s->since >= s->admit_every || active == 0);
-  if (due || s->evict || (!s->running.empty() && active == 0)) {
-    s->evict = false;
+  const bool evict = active > 0 && retired > 0 && (double)retired / (double)(retired + used) * (double)left >= kRepackChunks;
+  if (due || evict || (!s->running.empty() && active == 0)) {
     std::vector<Request*> old;
     for (auto& r : s->running) old.push_back(r.get());
     std::vector<std::unique_ptr<Request>> next;
@@ -340,7 +349,6 @@ int mt_server_step(mt_server* s, int32_t* n_running, int32_t* n_waiting) {
       if (diverged || r.steps >= r.max_steps || (k >= 2 && std::fabs(r.cd[k - 1] - r.cd[k - 2]) < r.tol)) {
         r.done = true;
         r.status = diverged ? MT_REQ_DIVERGED : MT_REQ_FINISHED;
-        if (diverged) s->evict = true;
         auto res = std::make_unique<Request>();
         res->id = r.id;
         res->steps = r.steps;

@@ -14,8 +14,11 @@ chunks of `chunk` coarse steps; between chunks it
     member keeps stepping with the batch; its result is already final.
   * scans every member for non-finite populations on the device
     (mt_scan_members, the reference's NaN checks per member); a member that has
-    any, or whose drag is not finite, finishes as diverged (`failed`) and is
-    evicted by a repack at the next chunk, whatever the admission interval.
+    any, or whose drag is not finite, finishes as diverged (`failed`) at once.
+  * repacks to evict the finished and diverged members, whatever the admission
+    interval, when that pays: their share of the batch's footprint times the
+    chunks the batch may still run (the most any running member has left)
+    reaches `repack_chunks`, the cost of a repack in chunks of the batch.
 Latency is completion time minus arrival time, on the server's wall clock.
 """
 from __future__ import annotations
@@ -36,6 +39,12 @@ def footprint_bytes(case, precision):
     rb = 8 if precision == amr.FP64 else 4
     cells = sum(lv["keys"].size for lv in case["levels"])
     return int(cells * 1.3 * (3 * case["Q"] * pb + 8 * rb + 24))
+
+
+# A repack (pack, build the packed case's tables, copy the running members' state)
+# costs about as much as one and a half chunks of 100 steps of a full batch of
+# served requests on one MI355X (about 50 ms against 38 ms).
+REPACK_CHUNKS = 1.5
 
 
 class Request:
@@ -63,9 +72,10 @@ class Server:
     (each chunk takes that long, idle waits are skipped) for reproducible tests."""
 
     def __init__(self, precision=amr.FP32_SHIFTED, chunk=100, hbm_budget=200 << 30, chunk_seconds=None,
-                 slot_dims=(100, 48, 48), admit_every=1):
+                 slot_dims=(100, 48, 48), admit_every=1, repack_chunks=REPACK_CHUNKS):
         self.precision, self.chunk, self.budget = precision, chunk, hbm_budget
         self.admit_every = admit_every
+        self.repack_chunks = repack_chunks
         self.chunk_seconds = chunk_seconds
         self.vtime = 0.0
         # every running request keeps one slot (a fixed box along x) for its life,
@@ -96,6 +106,17 @@ class Server:
             self.slot_of[id(r)] = k
         return self.slot_of[id(r)]
 
+    def _evict_due(self):
+        """Whether evicting the finished members pays for a repack."""
+        done = [r for r in self.members if r.done is not None]
+        active = [r for r in self.members if r.done is None]
+        if not done or not active:
+            return False
+        fp = [footprint_bytes(r.case, self.precision) for r in done]
+        total = sum(fp) + sum(footprint_bytes(r.case, self.precision) for r in active)
+        left = max(-(-(r.max_steps - r.steps) // self.chunk) for r in active)
+        return sum(fp) / total * left >= self.repack_chunks
+
     def _repack(self, running):
         live = {id(r) for r in running}
         self.slot_of = {k: v for k, v in self.slot_of.items() if k in live}
@@ -124,13 +145,11 @@ class Server:
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         running, done = [], []     # running: the batch's members, finished or not
         since = self.admit_every   # chunks since the last repack
-        evict = False              # a member diverged in the last chunk
         while pending or any(r.done is None for r in running):
             now = self._now(t0)
             active = [r for r in running if r.done is None]
             arrived = bool(pending) and pending[0].arrival <= now
-            if (arrived and (since >= self.admit_every or not active)) or (running and not active) or evict:
-                evict = False
+            if (arrived and (since >= self.admit_every or not active)) or (running and not active) or self._evict_due():
                 used = sum(footprint_bytes(r.case, self.precision) for r in active)
                 admitted = []
                 while pending and pending[0].arrival <= now:
@@ -167,7 +186,6 @@ class Server:
                 r.cd.append(r.cd_of(sums[i, 0] / max(n, 1)))
                 if bad[i] > 0 or not np.isfinite(r.cd[-1]):
                     r.failed = True
-                    evict = True
                 if r.failed or r.finished():
                     r.done = now
                     done.append(r)

@@ -133,10 +133,13 @@ def poisoned(req):
 
 def test_diverged_request_is_flagged_and_evicted():
     """A request whose state goes non-finite is found by the per-member device scan
-    after its first chunk, finishes as failed and leaves the batch at the next
-    chunk's repack; the requests beside it compute what they compute alone."""
+    after its first chunk and finishes as failed; with about half the batch's
+    footprint and 5 chunks left to run, evicting it pays, so it leaves the
+    batch at the next chunk's repack; the requests beside it compute what they
+    compute alone."""
     rng = np.random.default_rng(10)
-    reqs = [small_request(0, 0.0, rng), poisoned(small_request(1, 0.0, rng)), small_request(2, 1.5, rng)]
+    reqs = [small_request(0, 0.0, rng, max_steps=600), poisoned(small_request(1, 0.0, rng)),
+            small_request(2, 1.5, rng)]
     srv = serve.Server(precision=amr.FP32_SHIFTED, chunk=100, chunk_seconds=1.0, admit_every=4)
     done = srv.run(reqs)
     by_id = {r.id: r for r in done}

@@ -229,3 +229,34 @@ def test_rigid_body_free_motion_closed_form():
     p = RigidBody((1.0, 2.0, 3.0), velocity=(0.5, 0.0, 0.0))
     p.advance(J, dt=2.0)
     assert np.array_equal(p.velocity, [0.5, 0.0, 0.0]) and np.allclose(p.center, [2.0, 2.0, 3.0])
+
+
+def test_serving_eviction_repack_pays_for_itself():
+    """An eviction-only repack is due when the finished members' share of the
+    batch's footprint times the chunks the batch may still run reaches the
+    repack cost (in chunks); a batch with nothing running or nothing finished
+    never asks for one."""
+    from mortontrie_amr_amd import serve
+
+    def req(cells, max_steps, steps, done):
+        case = {"Q": 27, "levels": [{"keys": np.zeros(cells, dtype=np.uint64)}]}
+        r = serve.Request(0, case, 0.0, max_steps=max_steps)
+        r.steps, r.done = steps, (1.0 if done else None)
+        return r
+
+    srv = serve.Server(chunk=100, repack_chunks=1.5)
+    # a quarter of the footprint finished, 5 chunks left: 1.25 < 1.5
+    srv.members = [req(100, 300, 300, True), req(300, 800, 300, False)]
+    assert not srv._evict_due()
+    # 6 chunks left (a partial chunk counts whole): 1.5 >= 1.5
+    srv.members = [req(100, 300, 300, True), req(300, 850, 300, False)]
+    assert srv._evict_due()
+    # half finished, 2 chunks left: 1.0 < 1.5; a third member with 4 left: 2.0
+    srv.members = [req(200, 100, 100, True), req(200, 500, 300, False)]
+    assert not srv._evict_due()
+    srv.members.append(req(0, 700, 300, False))
+    assert srv._evict_due()
+    srv.members = [req(200, 100, 100, True)]
+    assert not srv._evict_due()
+    srv.members = [req(200, 900, 100, False)]
+    assert not srv._evict_due()
