@@ -344,6 +344,24 @@ __global__ void k_flag_target(const uint16_t* kind, int64_t ns, int* flag) {
   if (s >= ns) return;
   flag[s] = (kind[s] & (MT_KIND_C2F | MT_KIND_GHOST_C2F)) && !(kind[s] & MT_KIND_DIS);
 }
+// The classes order_active sorts the active bricks by, one 64-lane block per
+// entry of the active list: bit 0 a treated solid-boundary cell, bit 1 a
+// coalescence group (its first slot has a parent), bit 2 a ghost_F2C parent,
+// bit 3 a cell that a sub-step without the C2F stream still updates (a lat_f
+// cell, or a C2F cell in dis).
+__global__ __launch_bounds__(64) void k_brick_order_flags(const int32_t* act, const uint16_t* kind,
+                                                          const int32_t* dis_row, const int32_t* cpar,
+                                                          const int32_t* gxrow, uint8_t* out) {
+  const int i = threadIdx.x;
+  const int64_t s = (int64_t)act[blockIdx.x] * 64 + i;
+  const uint16_t k = kind[s];
+  const bool t = (k & MT_KIND_DIS) && (k & MT_KIND_FLUID) && dis_row[s] >= 0;
+  const bool g = cpar && (i & 7) == 0 && cpar[s >> 3] >= 0;
+  const bool x = gxrow && gxrow[s] >= 0;
+  const bool n = (k & MT_KIND_FLUID) || ((k & MT_KIND_C2F) && (k & MT_KIND_DIS));
+  const uint64_t bt = __ballot(t), bg = __ballot(g), bx = __ballot(x), bn = __ballot(n);
+  if (i == 0) out[blockIdx.x] = (uint8_t)((bt ? 1 : 0) | (bg ? 2 : 0) | (bx ? 4 : 0) | (bn ? 8 : 0));
+}
 __global__ void k_flag_brick(const uint16_t* kind, int64_t nb, uint16_t mask, int* flag) {
   int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
@@ -590,32 +608,19 @@ static int order_active(Case& c, int L) {
   DevLevel& l = c.lv[L];
   const int64_t na = l.n_active;
   if (na <= 1) return MT_OK;
+  // the classes are found on the device, one byte per active brick, so only
+  // the active list and these bytes cross to the host
+  uint8_t* d_fl = nullptr;
+  MT_HIP(dev_malloc((void**)&d_fl, na, c.stream));
+  k_brick_order_flags<<<(unsigned)na, 64, 0, c.stream>>>(l.active, l.kind, l.dis_row, l.cpar, l.gxrow, d_fl);
+  MT_HIP(hipGetLastError());
   std::vector<int32_t> act(na);
-  std::vector<uint16_t> kind(l.n_slots);
-  std::vector<int32_t> drow(l.n_slots), cpar, gx;
+  std::vector<uint8_t> fl(na);
   MT_HIP(hipMemcpyAsync(act.data(), l.active, 4 * na, hipMemcpyDeviceToHost, c.stream));
-  MT_HIP(hipMemcpyAsync(kind.data(), l.kind, 2 * l.n_slots, hipMemcpyDeviceToHost, c.stream));
-  MT_HIP(hipMemcpyAsync(drow.data(), l.dis_row, 4 * l.n_slots, hipMemcpyDeviceToHost, c.stream));
-  if (l.cpar) {
-    cpar.resize(l.n_slots / 8);
-    MT_HIP(hipMemcpyAsync(cpar.data(), l.cpar, 4 * (l.n_slots / 8), hipMemcpyDeviceToHost, c.stream));
-  }
-  if (l.gxrow) {
-    gx.resize(l.n_slots);
-    MT_HIP(hipMemcpyAsync(gx.data(), l.gxrow, 4 * l.n_slots, hipMemcpyDeviceToHost, c.stream));
-  }
+  MT_HIP(hipMemcpyAsync(fl.data(), d_fl, na, hipMemcpyDeviceToHost, c.stream));
   MT_HIP(hipStreamSynchronize(c.stream));
+  dev_free(d_fl, c.stream);
   // bit 0: treated cells, bit 1: coalescence groups, bit 2: ghost_F2C parents
-  auto flags = [&](int32_t b) {
-    int k = 0;
-    for (int i = 0; i < 64; ++i) {
-      const int64_t s = (int64_t)b * 64 + i;
-      if ((kind[s] & MT_KIND_DIS) && (kind[s] & MT_KIND_FLUID) && drow[s] >= 0) k |= 1;
-      if (!cpar.empty() && (i & 7) == 0 && cpar[s >> 3] >= 0) k |= 2;
-      if (!gx.empty() && gx[s] >= 0) k |= 4;
-    }
-    return k;
-  };
   auto rank_a = [](int k) { return (k & 1) ? 0 : (k & 2) ? 1 : (k & 4) ? 2 : 3; };
   // in a sub-step that coalesces, a coalescing brick's tail (the parents'
   // rescale, collision and explosion) is the longest
@@ -625,11 +630,12 @@ static int order_active(Case& c, int L) {
   for (int64_t x = 0; x * per < na; ++x) {
     const int64_t lo = x * per, hi = std::min<int64_t>(na, (x + 1) * per);
     std::vector<std::pair<int, int32_t>> v;
-    for (int64_t i = lo; i < hi; ++i) v.push_back({flags(act[i]), act[i]});
+    for (int64_t i = lo; i < hi; ++i) v.push_back({fl[i], act[i]});
     std::stable_sort(v.begin(), v.end(), [&](auto& p, auto& q) { return rank_a(p.first) < rank_a(q.first); });
     std::vector<int64_t> ib(v.size());
     for (size_t i = 0; i < v.size(); ++i) {
       act[lo + i] = v[i].second;
+      fl[lo + i] = (uint8_t)v[i].first;
       ib[i] = (int64_t)i;
     }
     std::stable_sort(ib.begin(), ib.end(), [&](int64_t p, int64_t q) { return rank_b(v[p].first) < rank_b(v[q].first); });
@@ -643,12 +649,7 @@ static int order_active(Case& c, int L) {
   // only has nothing to do in a sub-step that drops their stream)
   std::vector<int32_t> act_a, slot_a;
   for (int64_t i = 0; i < na; ++i) {
-    bool need = false;
-    for (int j = 0; j < 64 && !need; ++j) {
-      const uint16_t kk = kind[(int64_t)act[i] * 64 + j];
-      need = (kk & MT_KIND_FLUID) || ((kk & MT_KIND_C2F) && (kk & MT_KIND_DIS));
-    }
-    if (need) {
+    if (fl[i] & 8) {
       act_a.push_back(act[i]);
       slot_a.push_back((int32_t)i);
     }
