@@ -97,6 +97,12 @@ struct DevLevel {
   // target groups; the target slots.  Coarse side:
   // per slot, the row of a ghost_F2C parent in gf2c_child, else -1.
   uint32_t* tmask = nullptr;
+  // with the reference's explosion (sub-step A only), sub-step B reads the C2F
+  // cells' own fcol (sub-step A's copy) and only the ghost_C2F targets from the
+  // target groups: the same mask and slot list restricted to ghost_C2F targets
+  uint32_t* tmask_g = nullptr;
+  int32_t* xtg_list = nullptr;
+  int64_t n_xtg = 0;
   // the explosion targets of the current and the next pair: one value per
   // population per aligned group of 8 slots (the 8 children of one parent all
   // receive the same values), [Q][n_slots / 8] of the storage format

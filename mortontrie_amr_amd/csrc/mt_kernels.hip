@@ -941,12 +941,16 @@ __global__ void k_copy_targets(const int32_t* list, int64_t n, int64_t ns, const
 }
 
 // Write level L's explosion targets of the target groups xg[from_buf] into the
-// target slots of fcol[to_buf].
+// target slots of fcol[to_buf]: every target, or with the reference's
+// explosion (sub-step A only) the ghost_C2F targets, since there the C2F
+// cells' fcol is sub-step A's copy.
 int launch_copy_targets(Case& c, int L, int from_buf, int to_buf) {
   DevLevel& l = c.lv[L];
-  if (!l.n_xt || !l.xg[from_buf]) return MT_OK;
-  MT_DISPATCH(c, (k_copy_targets<Q_, S_><<<nblk(l.n_xt), 256, 0, c.stream>>>(
-                     l.xt_list, l.n_xt, l.n_slots, (const S_*)l.xg[from_buf], (S_*)l.fcol[to_buf])));
+  const int32_t* list = l.tmask_g ? l.xtg_list : l.xt_list;
+  const int64_t n = l.tmask_g ? l.n_xtg : l.n_xt;
+  if (!n || !l.xg[from_buf]) return MT_OK;
+  MT_DISPATCH(c, (k_copy_targets<Q_, S_><<<nblk(n), 256, 0, c.stream>>>(
+                     list, n, l.n_slots, (const S_*)l.xg[from_buf], (S_*)l.fcol[to_buf])));
   MT_HIP(hipGetLastError());
   return MT_OK;
 }
@@ -1041,7 +1045,10 @@ static int fused_args(Case& c, int L, bool write_f, bool write_macros, bool c2f_
   }
   if (c.xin) {
     if (L >= 1) {
-      a.tmask = l.tmask;
+      // the reference explodes before sub-step A only: sub-step B (the one that
+      // coalesces) reads the C2F cells' fcol, which sub-step A wrote (C2F
+      // copied), and only the ghost_C2F targets from the target groups
+      a.tmask = (coal && l.tmask_g) ? l.tmask_g : l.tmask;
       a.xbuf = l.xg[c.lv[L - 1].pf];
     }
     if (L < c.Lmax && l.gxrow) {

@@ -311,7 +311,7 @@ __global__ void k_cpar(const int32_t* f2c, const int32_t* child, int64_t n, cons
 // target buffer.  The source is x + e_q, except on a wall link of a treated
 // solid-boundary cell (wmask, finest level), whose pull loads x - e_q instead.
 __global__ void k_tmask(const int32_t* bnbr, const uint16_t* kind, const uint32_t* pmask, const uint32_t* wmask,
-                        int64_t ns, int Q, uint32_t* tmask) {
+                        int64_t ns, int Q, uint16_t targets, uint32_t* tmask) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   uint32_t m = 0;
@@ -327,11 +327,11 @@ __global__ void k_tmask(const int32_t* bnbr, const uint16_t* kind, const uint32_
       if (!((pm >> (wall ? qinv(q) : q)) & 1)) continue;
       const int t = wall ? slot_at(nb27, lx, ly, lz, -kEX[q], -kEY[q], -kEZ[q])
                          : slot_at(nb27, lx, ly, lz, kEX[q], kEY[q], kEZ[q]);
-      if (t >= 0 && (kind[t] & (MT_KIND_C2F | MT_KIND_GHOST_C2F)) && !(kind[t] & MT_KIND_DIS)) m |= 1u << q;
+      if (t >= 0 && (kind[t] & targets) && !(kind[t] & MT_KIND_DIS)) m |= 1u << q;
     }
     // bit 0: the cell is a target itself, so its rest population (the pull
     // from its own slot) is in the target buffer too
-    if ((kind[s] & MT_KIND_C2F) && !(kind[s] & MT_KIND_DIS)) m |= 1u;
+    if ((kind[s] & MT_KIND_C2F & targets) && !(kind[s] & MT_KIND_DIS)) m |= 1u;
   }
   tmask[s] = m;
 }
@@ -339,10 +339,10 @@ __global__ void k_gxrow(const int32_t* gf2c, int64_t n, int32_t* gxrow) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) gxrow[gf2c[i]] = (int32_t)i;
 }
-__global__ void k_flag_target(const uint16_t* kind, int64_t ns, int* flag) {
+__global__ void k_flag_target(const uint16_t* kind, int64_t ns, uint16_t targets, int* flag) {
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
-  flag[s] = (kind[s] & (MT_KIND_C2F | MT_KIND_GHOST_C2F)) && !(kind[s] & MT_KIND_DIS);
+  flag[s] = (kind[s] & targets) && !(kind[s] & MT_KIND_DIS);
 }
 // The classes order_active sorts the active bricks by, one 64-lane block per
 // entry of the active list: bit 0 a treated solid-boundary cell, bit 1 a
@@ -893,25 +893,32 @@ int build_tables(Case& c) {
     }
     // explosions inside the fused updates: needs the in-kernel coalescence on
     // every interface
-    bool xin = (c.d.options & MT_OPT_EXPLODE_BOTH) && c.Lmax >= 1;
+    const bool both = (c.d.options & MT_OPT_EXPLODE_BOTH) != 0;
+    bool xin = c.Lmax >= 1;
     for (int L = 0; L < c.Lmax && xin; ++L) xin = c.lv[L].coal_in_fused;
     for (int L = 0; L <= c.Lmax && xin; ++L) {
       DevLevel& l = c.lv[L];
       if (L >= 1) {
-        l.tmask = c.alloc<uint32_t>(l.n_slots);
-        int* flag = nullptr;
-        MT_HIP(dev_malloc((void**)&flag, 4 * (l.n_slots + 1), c.stream));
-        k_flag_target<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.kind, l.n_slots, flag);
-        int r = compact_list(c, flag, l.n_slots, &l.xt_list, &l.n_xt);
-        if (r) return r;
-        MT_HIP(hipStreamSynchronize(c.stream));
-        dev_free(flag, c.stream);
-        if (!l.tmask) {
-          set_error("out of device memory");
-          return MT_ERR_HIP;
+        const uint32_t* wm = L == c.Lmax ? l.wmask : nullptr;
+        // every target, and (reference explosion) the ghost_C2F targets alone
+        for (int g = 0; g < (both ? 1 : 2); ++g) {
+          const uint16_t targets = g ? MT_KIND_GHOST_C2F : (MT_KIND_C2F | MT_KIND_GHOST_C2F);
+          uint32_t*& mask = g ? l.tmask_g : l.tmask;
+          mask = c.alloc<uint32_t>(l.n_slots);
+          int* flag = nullptr;
+          MT_HIP(dev_malloc((void**)&flag, 4 * (l.n_slots + 1), c.stream));
+          k_flag_target<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.kind, l.n_slots, targets, flag);
+          int r = g ? compact_list(c, flag, l.n_slots, &l.xtg_list, &l.n_xtg)
+                    : compact_list(c, flag, l.n_slots, &l.xt_list, &l.n_xt);
+          if (r) return r;
+          MT_HIP(hipStreamSynchronize(c.stream));
+          dev_free(flag, c.stream);
+          if (!mask) {
+            set_error("out of device memory");
+            return MT_ERR_HIP;
+          }
+          k_tmask<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.bnbr, l.kind, l.pmask, wm, l.n_slots, c.Q, targets, mask);
         }
-        k_tmask<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.bnbr, l.kind, l.pmask, L == c.Lmax ? l.wmask : nullptr,
-                                                       l.n_slots, c.Q, l.tmask);
         for (int j = 0; j < 2; ++j) {
           l.xg[j] = c.alloc<char>((l.n_slots / 8) * c.Q * (int64_t)c.pop_bytes());
           if (!l.xg[j]) {

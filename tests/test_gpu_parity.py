@@ -133,9 +133,11 @@ def test_two_level_cavity():
 ])
 def test_sphere_three_levels(kw):
     """3-level sphere: Zou-He inflow, outflow, periodic y/z, link IB on the finest
-    level; fields and the per-treat() IB force against the oracle."""
+    level; fields and the per-treat() IB force against the oracle (the merged
+    schedule, explosions inside the updates)."""
     case = small_sphere(**kw)
     g, o = amr.AmrCase(case), O.OracleCase(case)
+    assert g.schedule() == "merged"
     g.step(6)
     o.step(6)
     compare(g, o, case)
@@ -145,19 +147,41 @@ def test_sphere_three_levels(kw):
     assert np.allclose(s, fl.sum(0), rtol=1e-11, atol=1e-11)
 
 
-@pytest.mark.parametrize("opts", [2, 3])
+@pytest.mark.parametrize("opts", [0, 1, 2, 3])
 def test_sphere_explosions_inside_updates(opts):
-    """With both-sub-step explosions the fused updates carry the explosions
-    themselves (targets double-buffered per coarse parity).  Steps 2+ read the
-    target buffer that differs from the level's own fcol, the case that shows
-    whether every read of a target (including a C2F cell's rest population)
-    goes to the target buffer."""
+    """The fused updates carry the explosions themselves (targets double-buffered
+    per coarse parity), with the reference's explosion (sub-step A only: sub-step
+    B reads the C2F cells' own copy and the ghost_C2F targets) and with
+    both-sub-step explosions.  Steps 2+ read the target buffer that differs from
+    the level's own fcol, the case that shows whether every read of a target
+    (including a C2F cell's rest population) goes to the target buffer; reading
+    the fields mid-run returns the targets to the reference's buffers."""
     case = C.sphere_amr((32, 24, 24), radius=3.0, center=(12, 12, 12), shells=(5.5, 2.0), options=opts)
     g, o = amr.AmrCase(case), O.OracleCase(case)
     assert g.schedule() == "merged"
-    g.step(5)
-    o.step(5)
-    compare(g, o, case)
+    for n in (2, 3):
+        g.step(n)
+        o.step(n)
+        compare(g, o, case)
+    fl = o.force_log()
+    s, n = g.ib_force()
+    assert n == len(fl)
+    assert np.allclose(s, fl.sum(0), rtol=1e-11, atol=1e-11)
+
+
+@pytest.mark.parametrize("opts", [0, 2])
+def test_explosion_kernels_where_coalescence_is_not_fused(opts):
+    """A shell so thin that some F2C parents have treated children: the
+    coalescence cannot run inside the fine update, so neither can the
+    explosions; mt_step falls back to the per-level streams with explosion and
+    coalescence kernels, and still computes the oracle's run."""
+    case = C.sphere_amr((32, 24, 24), radius=4.0, center=(12, 12, 12), shells=(2.0,), options=opts)
+    g, o = amr.AmrCase(case), O.OracleCase(case)
+    assert g.schedule() == "streams"
+    for n in (2, 3):
+        g.step(n)
+        o.step(n)
+        compare(g, o, case)
     fl = o.force_log()
     s, n = g.ib_force()
     assert n == len(fl)
@@ -207,14 +231,14 @@ def test_explosions_inside_updates_next_to_treated_cells(collision):
     assert np.allclose(s, fl.sum(0), rtol=1e-11, atol=1e-11)
 
 
-@pytest.mark.parametrize("graphs", [True, False])
-def test_schedules_compute_the_same_bits(graphs):
+@pytest.mark.parametrize("graphs,opts", [(True, 2), (False, 2), (True, 0)])
+def test_schedules_compute_the_same_bits(graphs, opts):
     """The merged schedule (one launch per finest sub-step, the default where the
     explosions run inside the updates), a stream per level and the serial order
     give identical fields, force sums and force counts, in graphs and in eager
     launches, over step splits that end inside a graph group; the merged run is
     also the oracle's."""
-    case = C.sphere_amr((32, 24, 24), radius=3.0, center=(12, 12, 12), shells=(5.5, 2.0), options=2)
+    case = C.sphere_amr((32, 24, 24), radius=3.0, center=(12, 12, 12), shells=(5.5, 2.0), options=opts)
     runs = {}
     for sched in ("merged", "streams", "serial"):
         g = amr.AmrCase(case, graphs=graphs)
