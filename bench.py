@@ -99,13 +99,15 @@ def total_mass(g, Lmax):
 def warm_graphs(g, steps):
     """Captures, before the timed region, every HIP graph a call of mt_step(steps)
     replays: graphs are keyed by the levels' buffer parities at their start (level
-    0 alternates per coarse step), their length (4, or steps mod 4 for the last)
-    and whether they end the call, so 8 + steps mod 4 coarse steps at each
-    level-0 parity build them all."""
-    r = steps % 4
-    g.step(8 + r)
+    0 alternates per coarse step), their length (G = amr.steps_per_graph(), or
+    steps mod G for the last) and whether they end the call, so 2G + steps mod G
+    coarse steps at each level-0 parity build them all."""
+    from mortontrie_amr_amd import amr
+    G = amr.steps_per_graph()
+    r = steps % G
+    g.step(2 * G + r)
     g.step(1)
-    g.step(8 + r)
+    g.step(2 * G + r)
     g.sync()
 
 
@@ -412,22 +414,27 @@ def reference_schedule(args):
     only, LBM_Simulator.cpp:47-51): the explosions run inside the fused updates
     and the merged schedule applies; sub-step A also streams the C2F cells
     (their copy feeds sub-step B).  The flow is unstable at tau0 = 0.5192 under
-    these semantics (finite for ~100 coarse steps), so 20 steps are timed."""
+    these semantics (finite for ~100 coarse steps), so one graph's worth of
+    steps (16) is timed, after 4 steps and one untimed call that captures it."""
     from mortontrie_amr_amd import amr, cases as C
     case = C.sphere_amr(options=0)
     upd = sum(n << L for L, n in enumerate(level_updates(case)))
     g = amr.AmrCase(case, precision=amr.FP64, graphs=True)
+    n = amr.steps_per_graph()
     g.step(4)
-    warm_graphs(g, 20)
+    # n is even, so the timed call starts at the parities of this one and
+    # replays the graph it captured
+    g.step(n)
+    g.sync()
     t0 = time.perf_counter()
-    g.step(20)
+    g.step(n)
     g.sync()
     secs = time.perf_counter() - t0
     nan = sum(g.reduce(L)[2] for L in range(case["Lmax"] + 1))
     sched = g.schedule()
     g.close()
-    gbs = streamed_cells_per_coarse_step(case) * 432 * 20 / secs / 1e9
-    return {"MLUPS": round(upd * 20 / secs / 1e6, 2), "ms_per_step": round(secs / 20 * 1e3, 5), "steps": 20,
+    gbs = streamed_cells_per_coarse_step(case) * 432 * n / secs / 1e9
+    return {"MLUPS": round(upd * n / secs / 1e6, 2), "ms_per_step": round(secs / n * 1e3, 5), "steps": n,
             "nan_count": int(nan), "schedule": sched,
             "algorithmic_GBs": round(gbs, 1), "frac_of_hbm_peak": round(gbs / 8000.0, 3),
             "semantics": "reference interface semantics: explosion in sub-step A only"}

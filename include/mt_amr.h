@@ -261,6 +261,11 @@ int mt_set_schedule(mt_case* c, int schedule);
  * fused), which runs as MT_SCHED_SERIAL, or as MT_SCHED_STREAMS when no
  * schedule was set.  Valid after mt_case_finalize.                            */
 int mt_get_schedule(mt_case* c, int32_t* schedule);
+/* The coarse steps one captured HIP graph holds (no counterpart in the
+ * reference): mt_step(n) replays graphs of this many steps and one of
+ * n mod this many (or of this many) that ends the call; each graph is captured
+ * on its first replay, per level buffer parities, length and end flag.        */
+int mt_steps_per_graph(void);
 /* Launch-configuration autotuner (no counterpart in the reference): tries the
  * fused kernel's block -> brick maps and the schedules MT_SCHED_STREAMS and
  * MT_SCHED_MERGED, twice each, for `steps` coarse steps of the real run (+1

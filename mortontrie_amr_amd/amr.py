@@ -49,7 +49,7 @@ EXPORTS = [
     "mt_last_error", "mt_case_create", "mt_case_destroy", "mt_case_upload_level", "mt_case_upload_dis",
     "mt_case_finalize", "mt_case_level_info", "mt_find", "mt_find_noalloc", "mt_get_field", "mt_set_field", "mt_macros",
     "mt_collide", "mt_stream", "mt_explode", "mt_coalesce", "mt_apply_bc", "mt_ib_treat", "mt_step",
-    "mt_set_graphs", "mt_set_tau", "mt_set_schedule", "mt_get_schedule", "mt_set_timing", "mt_get_timing", "mt_sync", "mt_reduce", "mt_ib_force",
+    "mt_set_graphs", "mt_set_tau", "mt_set_schedule", "mt_get_schedule", "mt_steps_per_graph", "mt_set_timing", "mt_get_timing", "mt_sync", "mt_reduce", "mt_ib_force",
     "mt_case_set_members", "mt_ib_force_members", "mt_autotune", "mt_case_copy_member",
     "mt_case_upload_markers", "mt_markers_move", "mt_markers_rigid", "mt_markers_get", "mt_case_remap", "mt_scan_members", "mt_set_diagnostics", "mt_diagnostics", "mt_levelset_build",
     "mt_levelset_counts", "mt_levelset_copy", "mt_levelset_copy_dis", "mt_levelset_destroy", "mt_snapshot",
@@ -90,6 +90,11 @@ class MtError(RuntimeError):
     pass
 
 
+def steps_per_graph():
+    """Coarse steps one captured HIP graph of mt_step holds."""
+    return int(load().mt_steps_per_graph())
+
+
 def load():
     """Load the HIP library (raises if it has not been built)."""
     global _lib
@@ -117,6 +122,8 @@ def load():
     L.mt_set_graphs.argtypes = [C.c_void_p, C.c_int]
     L.mt_set_schedule.argtypes = [C.c_void_p, C.c_int]
     L.mt_get_schedule.argtypes = [C.c_void_p, _i32p]
+    L.mt_steps_per_graph.argtypes = []
+    L.mt_steps_per_graph.restype = C.c_int
     L.mt_set_timing.argtypes = [C.c_void_p, C.c_int]
     L.mt_get_timing.argtypes = [C.c_void_p, _dp, _i64p, C.c_int]
     L.mt_sync.argtypes = [C.c_void_p]
