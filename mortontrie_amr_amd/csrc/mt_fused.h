@@ -614,13 +614,14 @@ __device__ __forceinline__ void fused_block(const FusedArgs& a, const uint32_t b
   }
   double fx = 0, fy = 0, fz = 0;  // momentum exchange of this lane's treated cell
   const int s = (b << 6) | lane;
-  // the cell's tables, loaded together up front (one memory round trip before
-  // the pull instead of one per table)
   // FP64: the wave issues its tables and its pull at a raised priority, so the
   // waves of a SIMD get their loads in flight before any of them takes the
   // VALU for a collision (measured 2% per coarse step on the sphere); FP32 and
-  // FP16 storage, with half and a quarter of the bytes, lose by it
+  // FP16 storage, with half and a quarter of the bytes, lose by it.  (A wave
+  // with no cell to stream keeps the priority to its end: it has little left.)
   if constexpr (sizeof(S) == 8) __builtin_amdgcn_s_setprio(2);
+  // the cell's tables, loaded together up front (one memory round trip before
+  // the pull instead of one per table)
   const uint16_t k = a.v.kind[s];
   const uint32_t pmx = a.v.pmask[s];
   const uint32_t tm = a.tmask ? a.tmask[s] : 0u;
