@@ -1298,25 +1298,30 @@ __global__ __launch_bounds__(256) void k_mk_spread(S* fcol, int64_t ns, const in
   }
   const int64_t i = (int64_t)(blockIdx.x - 1) * blockDim.x + threadIdx.x;
   if (i >= *n_band) return;
+  // the cell's moments and populations do not depend on the markers' forces:
+  // their loads go out first and overlap the walk over the cell's markers
+  const int sl = band[i];
+  const int32_t e0 = ptr[i], e1 = ptr[i + 1];
+  // the moments were computed in R from the same fcol: the round trip through
+  // double is exact
+  const R rho = (R)mom[i], ux = (R)mom[nb + i], uy = (R)mom[2 * nb + i], uz = (R)mom[3 * nb + i];
+  R fq[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) fq[q] = load_pop<Q, S, R>(fcol, ns, q, sl);
   double mx = 0, my = 0, mz = 0;
-  for (int32_t e = ptr[i]; e < ptr[i + 1]; ++e) {
+  for (int32_t e = e0; e < e1; ++e) {
     const double we = bw[e];
     const int32_t k = mk[e];
     mx += we * F[3 * k];
     my += we * F[3 * k + 1];
     mz += we * F[3 * k + 2];
   }
-  const int sl = band[i];
-  // the moments were computed in R from the same fcol: the round trip through
-  // double is exact
-  const R rho = (R)mom[i], ux = (R)mom[nb + i], uy = (R)mom[2 * nb + i], uz = (R)mom[3 * nb + i];
   const R vx = ux + (R)(mx / (double)rho), vy = uy + (R)(my / (double)rho), vz = uz + (R)(mz / (double)rho);
   const R uu = ux * ux + uy * uy + uz * uz, vv = vx * vx + vy * vy + vz * vz;
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const R fq = load_pop<Q, S, R>(fcol, ns, q, sl);
-    store_pop<Q, S, R>(fcol, ns, q, sl, fq + (feq_c<Q, R>(q, rho, vx, vy, vz, vv) - feq_c<Q, R>(q, rho, ux, uy, uz, uu)));
-  }
+  for (int q = 0; q < Q; ++q)
+    store_pop<Q, S, R>(fcol, ns, q, sl,
+                       fq[q] + (feq_c<Q, R>(q, rho, vx, vy, vz, vv) - feq_c<Q, R>(q, rho, ux, uy, uz, uu)));
 }
 
 int launch_markers(Case& c, int buf, hipStream_t st, bool moments_done) {
