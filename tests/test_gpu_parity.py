@@ -446,7 +446,11 @@ def test_graph_replay_bit_identical():
     with host reads in between."""
     case = small_sphere()
     a, b = amr.AmrCase(case, graphs=True), amr.AmrCase(case, graphs=False)
-    for n in (1, 3, 2, 4):
+    G = amr.steps_per_graph()
+    # calls shorter than a graph, and calls of several whole graphs plus a
+    # remainder, at both level-0 parities; the long calls replay each graph
+    # they captured
+    for n in (1, 3, 2, 4, 2 * G + 3, 1, 2 * G + 3, G):
         a.step(n)
         b.step(n)
         for L in range(case["Lmax"] + 1):
