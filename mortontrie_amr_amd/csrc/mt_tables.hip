@@ -348,7 +348,8 @@ __global__ void k_flag_target(const uint16_t* kind, int64_t ns, uint16_t targets
 // entry of the active list: bit 0 a treated solid-boundary cell, bit 1 a
 // coalescence group (its first slot has a parent), bit 2 a ghost_F2C parent,
 // bit 3 a cell that a sub-step without the C2F stream still updates (a lat_f
-// cell, or a C2F cell in dis).
+// cell, or a C2F cell in dis), bit 4 a cell of the domain boundary that the
+// update reconstructs (wet faces, edges and corners: face bits 0x3F00).
 __global__ __launch_bounds__(64) void k_brick_order_flags(const int32_t* act, const uint16_t* kind,
                                                           const int32_t* dis_row, const int32_t* cpar,
                                                           const int32_t* gxrow, uint8_t* out) {
@@ -359,8 +360,9 @@ __global__ __launch_bounds__(64) void k_brick_order_flags(const int32_t* act, co
   const bool g = cpar && (i & 7) == 0 && cpar[s >> 3] >= 0;
   const bool x = gxrow && gxrow[s] >= 0;
   const bool n = (k & MT_KIND_FLUID) || ((k & MT_KIND_C2F) && (k & MT_KIND_DIS));
-  const uint64_t bt = __ballot(t), bg = __ballot(g), bx = __ballot(x), bn = __ballot(n);
-  if (i == 0) out[blockIdx.x] = (uint8_t)((bt ? 1 : 0) | (bg ? 2 : 0) | (bx ? 4 : 0) | (bn ? 8 : 0));
+  const bool w = (k & 0x3F00) != 0;
+  const uint64_t bt = __ballot(t), bg = __ballot(g), bx = __ballot(x), bn = __ballot(n), bw = __ballot(w);
+  if (i == 0) out[blockIdx.x] = (uint8_t)((bt ? 1 : 0) | (bg ? 2 : 0) | (bx ? 4 : 0) | (bn ? 8 : 0) | (bw ? 16 : 0));
 }
 __global__ void k_flag_brick(const uint16_t* kind, int64_t nb, uint16_t mask, int* flag) {
   int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -600,8 +602,10 @@ int build_members(Case& c) {
 // The fused kernel gives XCD x the active-list range [x*per, (x+1)*per), per =
 // ceil(n_active/8), and dispatches each range in list order.  Bricks whose
 // update has a longer dependent tail go first within their range, so they do
-// not finish last: treated solid-boundary cells, then coalescence groups, then
-// ghost_F2C parents (explosion stores), then the rest; Morton order otherwise.
+// not finish last: treated solid-boundary cells, then wet domain-boundary cells
+// (the reconstruction's neighbour loads; the 64^3 cavity's FP32 step runs 11%
+// faster with its wall bricks first), then coalescence groups, then ghost_F2C
+// parents (explosion stores), then the rest; Morton order otherwise.
 // A level that coalesces into the coarser one gets a second order for the
 // sub-step that does (coalescence groups first), over the same XCD ranges.
 static int order_active(Case& c, int L) {
@@ -620,8 +624,9 @@ static int order_active(Case& c, int L) {
   MT_HIP(hipMemcpyAsync(fl.data(), d_fl, na, hipMemcpyDeviceToHost, c.stream));
   MT_HIP(hipStreamSynchronize(c.stream));
   dev_free(d_fl, c.stream);
-  // bit 0: treated cells, bit 1: coalescence groups, bit 2: ghost_F2C parents
-  auto rank_a = [](int k) { return (k & 1) ? 0 : (k & 2) ? 1 : (k & 4) ? 2 : 3; };
+  // bit 0: treated cells, bit 1: coalescence groups, bit 2: ghost_F2C parents,
+  // bit 4: wet domain-boundary cells
+  auto rank_a = [](int k) { return (k & 1) ? 0 : (k & 16) ? 1 : (k & 2) ? 2 : (k & 4) ? 3 : 4; };
   // in a sub-step that coalesces, a coalescing brick's tail (the parents'
   // rescale, collision and explosion) is the longest
   auto rank_b = [](int k) { return (k & 2) ? 0 : (k & 1) ? 1 : (k & 4) ? 2 : 3; };
