@@ -529,6 +529,8 @@ int mt_case_destroy(mt_case* c) {
     hipEventDestroy(p.second.second);
   }
   for (void* p : c->allocs) dev_free(p, nullptr);
+  for (int L = 0; L < kMaxLevels; ++L) drop_level_upload(c->lv[L], nullptr);
+  drop_dis_upload(*c, nullptr);
   for (int L = 0; L < kMaxLevels; ++L) {
     if (c->ev_ready[L]) hipEventDestroy(c->ev_ready[L]);
     if (c->ev_done[L]) hipEventDestroy(c->ev_done[L]);
@@ -562,9 +564,20 @@ int mt_case_upload_level(mt_case* c, int level, const uint64_t* keys, const uint
       return MT_ERR_ARG;
     }
   }
+  uint64_t mx = 0;
+  for (int64_t i = 0; i < n; ++i) mx = std::max(mx, keys[i]);
+  // staged on the device at once (the caller may free its arrays on return)
   DevLevel& l = c->lv[level];
-  l.h_keys.assign(keys, keys + n);
-  l.h_kinds.assign(kinds, kinds + n);
+  drop_level_upload(l, c->stream);
+  if (n > 0) {
+    MT_HIP(dev_malloc((void**)&l.up_keys, 8 * n, c->stream));
+    MT_HIP(dev_malloc((void**)&l.up_kinds, 2 * n, c->stream));
+    MT_HIP(hipMemcpyAsync(l.up_keys, keys, 8 * n, hipMemcpyHostToDevice, c->stream));
+    MT_HIP(hipMemcpyAsync(l.up_kinds, kinds, 2 * n, hipMemcpyHostToDevice, c->stream));
+    MT_HIP(hipStreamSynchronize(c->stream));
+  }
+  l.n_up = n;
+  l.up_max = mx;
   return MT_OK;
 }
 
@@ -572,8 +585,19 @@ int mt_case_upload_dis(mt_case* c, const uint64_t* keys, const double* vals, int
   CHECK_CASE(c);
   if (c->finalized) { set_error("case already finalized"); return MT_ERR_STATE; }
   if (n < 0 || (n && (!keys || !vals))) { set_error("bad dis array"); return MT_ERR_ARG; }
-  c->h_dis_keys.assign(keys, keys + n);
-  c->h_dis_vals.assign(vals, vals + n * (c->Q - 1));
+  uint64_t mx = 0;
+  for (int64_t i = 0; i < n; ++i) mx = std::max(mx, keys[i]);
+  drop_dis_upload(*c, c->stream);
+  if (n > 0) {
+    const int64_t m = c->Q - 1;
+    MT_HIP(dev_malloc((void**)&c->up_dis_keys, 8 * n, c->stream));
+    MT_HIP(dev_malloc((void**)&c->up_dis_vals, 8 * n * m, c->stream));
+    MT_HIP(hipMemcpyAsync(c->up_dis_keys, keys, 8 * n, hipMemcpyHostToDevice, c->stream));
+    MT_HIP(hipMemcpyAsync(c->up_dis_vals, vals, 8 * n * m, hipMemcpyHostToDevice, c->stream));
+    MT_HIP(hipStreamSynchronize(c->stream));
+  }
+  c->n_up_dis = n;
+  c->up_dis_max = mx;
   return MT_OK;
 }
 
@@ -732,10 +756,6 @@ int mt_case_finalize(mt_case* c) {
   MT_HIP(hipMemsetAsync(c->force_slots, 0, 3 * max_fp * sizeof(double), c->stream));
   TRY(init_state(*c));
   MT_HIP(hipStreamSynchronize(c->stream));
-  for (int L = 0; L <= c->Lmax; ++L) {
-    std::vector<uint64_t>().swap(c->lv[L].h_keys);
-    std::vector<uint16_t>().swap(c->lv[L].h_kinds);
-  }
   c->finalized = true;
   return MT_OK;
 }

@@ -38,9 +38,12 @@ struct MemberParams {
 struct DevLevel {
   int64_t n_cells = 0, n_bricks = 0, n_slots = 0;
   int32_t dims[3] = {0, 0, 0};
-  // upload staging (host)
-  std::vector<uint64_t> h_keys;
-  std::vector<uint16_t> h_kinds;
+  // the uploaded level set, staged on the device until mt_case_finalize sorts
+  // it into the tables (n_up = -1: nothing uploaded)
+  uint64_t* up_keys = nullptr;
+  uint16_t* up_kinds = nullptr;
+  int64_t n_up = -1;
+  uint64_t up_max = 0;
   // tables
   uint64_t* bcode = nullptr;  // [n_bricks] sorted brick Morton codes
   int32_t* bnbr = nullptr;    // [n_bricks][27] neighbour bricks (-1 none)
@@ -161,8 +164,12 @@ struct Case {
   int32_t* mv_slot = nullptr;
   double* mv_w = nullptr;
   int32_t* mv_flags = nullptr;
-  std::vector<uint64_t> h_dis_keys;
-  std::vector<double> h_dis_vals;
+  // the uploaded dis keys and fractions, staged on the device like the level
+  // sets
+  uint64_t* up_dis_keys = nullptr;
+  double* up_dis_vals = nullptr;
+  int64_t n_up_dis = 0;
+  uint64_t up_dis_max = 0;
   bool finalized = false;
   bool ahead = false;  // fcol[pf] already holds the next collision (mt_step state)
   bool graphs = true;
@@ -233,6 +240,25 @@ void dev_free(void* p, hipStream_t s);
 void pool_quiesce(hipStream_t s);
 hipError_t stream_acquire(hipStream_t* s);
 void stream_release(hipStream_t s);
+
+// Frees the device staging of an uploaded level set (of the uploaded dis),
+// in the order of stream s.
+inline void drop_level_upload(DevLevel& l, hipStream_t s) {
+  if (l.up_keys) dev_free(l.up_keys, s);
+  if (l.up_kinds) dev_free(l.up_kinds, s);
+  l.up_keys = nullptr;
+  l.up_kinds = nullptr;
+  l.n_up = -1;
+  l.up_max = 0;
+}
+inline void drop_dis_upload(Case& c, hipStream_t s) {
+  if (c.up_dis_keys) dev_free(c.up_dis_keys, s);
+  if (c.up_dis_vals) dev_free(c.up_dis_vals, s);
+  c.up_dis_keys = nullptr;
+  c.up_dis_vals = nullptr;
+  c.n_up_dis = 0;
+  c.up_dis_max = 0;
+}
 
 template <typename T>
 T* Case::alloc(int64_t n) {

@@ -266,12 +266,31 @@ int mt_server_submit(mt_server* s, mt_case* req, int64_t id, int32_t max_steps, 
   r->id = id;
   r->d = d;
   r->m = c.h_mdesc[0];
+  // the request's level sets and dis, staged on the device by its uploads,
+  // come back to the host: every repack translates them into the packed case
+  hipError_t e = hipSuccess;
   for (int L = 0; L <= d.Lmax; ++L) {
-    r->keys.push_back(c.lv[L].h_keys);
-    r->kinds.push_back(c.lv[L].h_kinds);
+    const DevLevel& l = c.lv[L];
+    const int64_t n = std::max<int64_t>(l.n_up, 0);
+    std::vector<uint64_t> keys(n);
+    std::vector<uint16_t> kinds(n);
+    if (n && e == hipSuccess) e = hipMemcpyAsync(keys.data(), l.up_keys, 8 * n, hipMemcpyDeviceToHost, c.stream);
+    if (n && e == hipSuccess) e = hipMemcpyAsync(kinds.data(), l.up_kinds, 2 * n, hipMemcpyDeviceToHost, c.stream);
+    r->keys.push_back(std::move(keys));
+    r->kinds.push_back(std::move(kinds));
   }
-  r->dis_keys = c.h_dis_keys;
-  r->dis_vals = c.h_dis_vals;
+  const int64_t nd = c.n_up_dis;
+  r->dis_keys.resize(nd);
+  r->dis_vals.resize(nd * (d.Q - 1));
+  if (nd && e == hipSuccess) e = hipMemcpyAsync(r->dis_keys.data(), c.up_dis_keys, 8 * nd, hipMemcpyDeviceToHost, c.stream);
+  if (nd && e == hipSuccess)
+    e = hipMemcpyAsync(r->dis_vals.data(), c.up_dis_vals, 8 * nd * (d.Q - 1), hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  if (e != hipSuccess) {
+    mt_case_destroy(req);
+    set_error(std::string("request download: ") + hipGetErrorString(e));
+    return MT_ERR_HIP;
+  }
   r->max_steps = max_steps;
   r->tol = drag_tol;
   r->cd_U = cd_U;

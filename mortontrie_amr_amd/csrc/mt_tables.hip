@@ -720,18 +720,15 @@ int build_tables(Case& c) {
   MT_HIP(hipMemsetAsync(d_err, 0, sizeof(int), c.stream));
   for (int L = 0; L <= c.Lmax; ++L) {
     DevLevel& l = c.lv[L];
-    int64_t n = (int64_t)l.h_keys.size();
-    if (n == 0) { set_error("level " + std::to_string(L) + " has no cells"); return MT_ERR_ARG; }
+    const int64_t n = l.n_up;
+    if (n <= 0) { set_error("level " + std::to_string(L) + " has no cells"); return MT_ERR_ARG; }
     for (int a = 0; a < 3; ++a) l.dims[a] = c.d.dims0[a] << L;
-    uint64_t *d_keys = nullptr, *ks = nullptr;
-    uint16_t* d_kinds = nullptr;
+    // the uploaded keys and kinds, staged on the device by mt_case_upload_level
+    const uint64_t* d_keys = l.up_keys;
+    const uint16_t* d_kinds = l.up_kinds;
+    uint64_t* ks = nullptr;
     int32_t* vs = nullptr;
-    MT_HIP(dev_malloc((void**)&d_keys, 8 * n, c.stream));
-    MT_HIP(dev_malloc((void**)&d_kinds, 2 * n, c.stream));
-    MT_HIP(hipMemcpyAsync(d_keys, l.h_keys.data(), 8 * n, hipMemcpyHostToDevice, c.stream));
-    MT_HIP(hipMemcpyAsync(d_kinds, l.h_kinds.data(), 2 * n, hipMemcpyHostToDevice, c.stream));
-    uint64_t mx = *std::max_element(l.h_keys.begin(), l.h_keys.end());
-    int r = radix_sort(c, d_keys, n, mx, &ks, &vs);
+    int r = radix_sort(c, d_keys, n, l.up_max, &ks, &vs);
     if (r) return r;
     int shift = 3 * (c.Lmax - L);
     int *head = nullptr, *bidx = nullptr;
@@ -775,8 +772,7 @@ int build_tables(Case& c) {
     k_pmask<<<nblk(l.n_slots), 256, 0, c.stream>>>(l.bnbr, l.kind, l.n_slots, c.Q, l.pmask);
     MT_HIP(hipGetLastError());
     MT_HIP(hipStreamSynchronize(c.stream));
-    dev_free(d_keys, c.stream);
-    dev_free(d_kinds, c.stream);
+    drop_level_upload(l, c.stream);
     dev_free(ks, c.stream);
     dev_free(vs, c.stream);
     dev_free(head, c.stream);
@@ -786,18 +782,14 @@ int build_tables(Case& c) {
   int Lm = c.Lmax;
   DevLevel& lm = c.lv[Lm];
   uint64_t* d_dis_sorted = nullptr;
-  int64_t nd = (int64_t)c.h_dis_keys.size();
+  const int64_t nd = c.n_up_dis;
   if (nd > 0) {
     int m = c.Q - 1;
-    uint64_t *d_k = nullptr, *ks = nullptr;
+    uint64_t* ks = nullptr;
     int32_t* vs = nullptr;
-    double* d_v = nullptr;
-    MT_HIP(dev_malloc((void**)&d_k, 8 * nd, c.stream));
-    MT_HIP(dev_malloc((void**)&d_v, 8 * nd * m, c.stream));
-    MT_HIP(hipMemcpyAsync(d_k, c.h_dis_keys.data(), 8 * nd, hipMemcpyHostToDevice, c.stream));
-    MT_HIP(hipMemcpyAsync(d_v, c.h_dis_vals.data(), 8 * nd * m, hipMemcpyHostToDevice, c.stream));
-    uint64_t mx = *std::max_element(c.h_dis_keys.begin(), c.h_dis_keys.end());
-    int r = radix_sort(c, d_k, nd, mx, &ks, &vs);
+    // the uploaded dis, staged on the device by mt_case_upload_dis
+    const double* d_v = c.up_dis_vals;
+    int r = radix_sort(c, c.up_dis_keys, nd, c.up_dis_max, &ks, &vs);
     if (r) return r;
     lm.dis = c.alloc<double>(nd * m);
     lm.n_dis = nd;
@@ -811,8 +803,7 @@ int build_tables(Case& c) {
     d_dis_sorted = c.alloc<uint64_t>(nd);
     MT_HIP(hipMemcpyAsync(d_dis_sorted, ks, 8 * nd, hipMemcpyDeviceToDevice, c.stream));
     MT_HIP(hipStreamSynchronize(c.stream));
-    dev_free(d_k, c.stream);
-    dev_free(d_v, c.stream);
+    drop_dis_upload(c, c.stream);
     dev_free(ks, c.stream);
     dev_free(vs, c.stream);
   }
