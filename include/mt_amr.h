@@ -257,9 +257,9 @@ int mt_set_tau(mt_case* c, int32_t member, const double* tau, int32_t n_levels);
 int mt_set_schedule(mt_case* c, int schedule);
 /* The schedule mt_step runs (no counterpart in the reference): the one set,
  * except MT_SCHED_MERGED where the explosions do not run inside the updates
- * (a case without MT_OPT_EXPLODE_BOTH, or an interface whose coalescence is not
- * fused), which runs as MT_SCHED_SERIAL, or as MT_SCHED_STREAMS when no
- * schedule was set.  Valid after mt_case_finalize.                            */
+ * (an interface whose coalescence cannot be fused, e.g. an F2C parent with a
+ * treated child), which runs as MT_SCHED_SERIAL, or as MT_SCHED_STREAMS when
+ * no schedule was set.  Valid after mt_case_finalize.                          */
 int mt_get_schedule(mt_case* c, int32_t* schedule);
 /* The coarse steps one captured HIP graph holds (no counterpart in the
  * reference): mt_step(n) replays graphs of this many steps and one of
