@@ -260,3 +260,11 @@ def test_serving_eviction_repack_pays_for_itself():
     assert not srv._evict_due()
     srv.members = [req(200, 900, 100, False)]
     assert not srv._evict_due()
+
+
+def test_graph_length_is_even():
+    """mt_step replays graphs of mt_steps_per_graph() coarse steps; level 0's
+    buffers flip once per coarse step, so an even length is what lets a timed
+    call of one graph's length replay the graph an earlier call captured."""
+    G = amr.steps_per_graph()
+    assert G >= 2 and G % 2 == 0
