@@ -37,6 +37,27 @@ def test_staggered_admission_matches_alone():
     check_against_alone(reqs, done, amr.FP32_SHIFTED)
 
 
+def test_staggered_admission_reference_explosion_matches_alone():
+    """With the reference's explosion (sub-step A only) the explosions also run
+    inside the updates; a repack copies each member's state, C2F copies and
+    ghost targets included, and a member still computes what it computes alone
+    (Re 8-12, tau0 about 0.6: stable under these semantics)."""
+    rng = np.random.default_rng(12)
+    reqs = []
+    for i, t in enumerate((0.0, 1.5, 2.5)):
+        r = float(rng.uniform(3.0, 5.0))
+        case = C.sphere_amr((48, 24, 24), radius=r, center=(16.0, 12.0, 12.0), shells=(3.0,), U=0.04,
+                            Re=float(rng.uniform(8, 12)), options=0)
+        reqs.append(serve.Request(i, case, t, max_steps=300, tol=0.0))
+    srv = serve.Server(precision=amr.FP32_SHIFTED, chunk=100, chunk_seconds=1.0)
+    done = srv.run(reqs)
+    assert srv.repacks >= 3
+    g = amr.AmrCase(reqs[0].case)
+    assert g.schedule() == "merged"
+    g.close()
+    check_against_alone(reqs, done, amr.FP32_SHIFTED)
+
+
 def test_budget_limits_concurrency():
     """A budget that fits one request serves them one after another; the results
     do not change."""
