@@ -120,9 +120,12 @@ const char* mt_last_error(void);
  *      (LBM_Manager.cpp:32-95, 482-592) -------------------------------------- */
 int mt_case_create(const mt_case_desc* desc, mt_case** out);
 int mt_case_destroy(mt_case* c);
-/* one level's cell set (any order); kinds use MT_KIND_* (Lat_Manager.hpp:245-316) */
+/* one level's cell set (any order); kinds use MT_KIND_* (Lat_Manager.hpp:245-316).
+ * The arrays are copied to the device before the call returns (the caller may
+ * free them); a second upload of the level replaces the first.                */
 int mt_case_upload_level(mt_case* c, int level, const uint64_t* keys, const uint16_t* kinds, int64_t n);
-/* Lat_Manager::dis: n finest-level keys, n x (Q-1) link fractions (-1 = no wall) */
+/* Lat_Manager::dis: n finest-level keys, n x (Q-1) link fractions (-1 = no
+ * wall), copied to the device before the call returns                         */
 int mt_case_upload_dis(mt_case* c, const uint64_t* keys, const double* fractions, int64_t n);
 /* sorts the keys on the device, builds bricks, neighbour / parent-child tables,
  * sets f = fcol = feq(rho0,u0) (solid 0) -- allocMacrosAndDDF + initialBC       */
