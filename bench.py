@@ -47,6 +47,7 @@ WORKLOAD = "sphere_amr_3level_d3q27_cumulant_fp64"
 MARKER_WORKLOAD = "sphere_markers6000_3level_d3q27_cumulant_les0.1_fp64"
 CAVITY_WORKLOAD = "cavity64_d3q19_lbgk_zouhe_fp64"
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_fused_levels_sphere_amr_fp64_hbm_traffic.md")
+CAVITY_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_fused_cavity64_d3q19_fp64_trace_and_traffic.md")
 
 
 def bytes_per_update(Q, precision):
@@ -80,11 +81,12 @@ def streamed_cells_per_coarse_step(case):
     return tot
 
 
-def measured_traffic():
-    """HBM bytes per launch of the finest-level fused kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None when absent)."""
+def measured_traffic(path=TRAFFIC_PROFILE):
+    """HBM bytes per launch of the dominant fused kernel from the committed
+    rocprofv3 summary of the size-resolved read and write requests, steady
+    state (None when absent)."""
     try:
-        txt = open(TRAFFIC_PROFILE).read()
+        txt = open(path).read()
     except OSError:
         return None
     m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", txt, re.M)
@@ -528,7 +530,8 @@ def measure(case, workload, args, variants=True, cpu=True):
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             # the committed --pmc summary covers the sphere workload's finest kernel
-            "traffic": measured_traffic() if workload == WORKLOAD else None,
+            "traffic": (measured_traffic() if workload == WORKLOAD else
+                        measured_traffic(CAVITY_TRAFFIC_PROFILE) if workload == CAVITY_WORKLOAD else None),
             "kernel": kernel, "bytes_per_launch": bytes_launch,
             "avg_launch_ms": round(avg_ms, 5), "launches_timed": int(r["calls"][Lf]),
             "per_level_avg_ms": [round(r["ms"][L] / max(r["calls"][L], 1), 5) for L in range(Lf + 1)],
