@@ -152,10 +152,12 @@ def rel_l2(a, b):
 
 
 def counter_run(case, steps, warmup, copy_mib):
-    """Short eager run for hardware-counter passes: `steps` coarse steps with one
-    dispatch per kernel (no graphs), then a plain device copy and a read-only
-    reduction of `copy_mib` MiB, whose counter readings check bytes per counter
-    unit."""
+    """Short eager run for hardware-counter passes: `warmup` then `steps` coarse
+    steps (two mt_step calls) with one dispatch per kernel (no graphs), then a
+    plain device copy and a read-only reduction of `copy_mib` MiB, whose counter
+    readings check bytes per counter unit.  The last coarse step of each call
+    also writes f and the density/velocity maps for the caller, so a
+    steady-state reading leaves those two steps out."""
     import torch
     from mortontrie_amr_amd import amr
     g = amr.AmrCase(case, precision=amr.FP64, graphs=False)
