@@ -459,7 +459,7 @@ def extras(args):
         out[name] = {k: m[k] for k in ("value", "unit", "steps", "ms_per_step", "dtype", "drag_coefficient",
                                        "mass_drift", "nan_count") if k in m}
         out[name]["workload"] = workload
-        out[name]["roofline"] = {k: m["roofline"][k] for k in ("achieved", "frac", "kernel", "avg_launch_ms")}
+        out[name]["roofline"] = {k: m["roofline"][k] for k in ("achieved", "frac", "kernel", "avg_launch_ms", "traffic")}
         out[name]["variants"] = m.get("variants")
         print(json.dumps({"progress": name, "MLUPS": m["value"]}), file=sys.stderr, flush=True)
     out["reference_schedule"] = reference_schedule(args)
