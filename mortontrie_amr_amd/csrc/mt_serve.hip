@@ -39,9 +39,12 @@ namespace mt {
 namespace {
 
 constexpr int kSlotX = 100, kSlotY = 48, kSlotZ = 48;  // level-0 cells per slot
-// A repack (build the packed case's tables, copy the running members' state)
-// costs about as much as one and a half chunks of 100 steps of a full batch of
-// served requests on one MI355X (about 50 ms against 38 ms).
+// Evicting pays when the retired share x the chunks left reaches this many
+// chunks.  A repack (build the packed case's tables, copy the running members'
+// state, capture the new case's graphs on its first chunk) costs about 30 ms on
+// one MI355X against about 38 ms per 100-step chunk of a full batch of served
+// requests; thresholds of 0.75, 1.5 and 3 chunks served alike within the
+// run-to-run spread.
 constexpr double kRepackChunks = 1.5;
 
 // a level-L key (keys of every level live on the finest grid, shifted by

@@ -41,9 +41,12 @@ def footprint_bytes(case, precision):
     return int(cells * 1.3 * (3 * case["Q"] * pb + 8 * rb + 24))
 
 
-# A repack (pack, build the packed case's tables, copy the running members' state)
-# costs about as much as one and a half chunks of 100 steps of a full batch of
-# served requests on one MI355X (about 50 ms against 38 ms).
+# Evicting pays when the retired share x the chunks left reaches this many
+# chunks.  A repack (pack, build the packed case's tables, copy the running
+# members' state, capture the new case's graphs on its first chunk) costs
+# about 30 ms on one MI355X against about 38 ms per 100-step chunk of a full
+# batch of served requests; thresholds of 0.75, 1.5 and 3 chunks served the
+# bench's stream alike within the run-to-run spread.
 REPACK_CHUNKS = 1.5
 
 
